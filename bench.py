@@ -1,0 +1,174 @@
+#!/usr/bin/env python
+"""Cone-beam projector-pair benchmark on one MI355X.
+
+Workload: 512^3 FP32 volume (0.5 mm voxels) from a seeded 30-ellipsoid
+phantom, 360 views over 2*pi, 512 x 512 detector (0.8 mm), DSO 1000 mm,
+DSD 1536 mm.  One step = Ax('Siddon') + Atb('FDK') over all 360 views on
+device-resident data (the operator pair OS-SART and ASD-POCS use).
+
+Metric: voxel-view updates per second of the pair, 2 * nVoxel * nViews per
+step (Gupdates/s).  The roofline entry prices the dominant kernel against
+HBM with the per-view sweep byte model: Ax moves n (V + P) * 4 bytes, Atb
+n (2V + P) * 4 bytes.
+
+Prints exactly one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512, help="volume edge / detector edge")
+    ap.add_argument("--views", type=int, default=360)
+    ap.add_argument("--cpu-views", type=int, default=8, help="views in the CPU-baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import tigre_amd
+    from tigre_amd.geometry import cone_geometry, prepared
+    from tigre_amd.phantom import ellipsoid_phantom_torch
+
+    dev = torch.device("cuda:0")
+    n, nviews = args.n, args.views
+    geo = cone_geometry((n, n, n), (256.0, 256.0, 256.0), (n, n), (0.8 * 512 / n, 0.8 * 512 / n), 1000.0, 1536.0)
+    angles = np.linspace(0, 2 * np.pi, nviews, endpoint=False)
+    log("phantom %d^3 ..." % n)
+    img = ellipsoid_phantom_torch(geo.nVoxel, geo.sVoxel, 30, seed=20261202 + 2, device=dev)
+    torch.cuda.synchronize()
+    V = n ** 3
+    P = n * n
+
+    def ax():
+        return tigre_amd.Ax(img, geo, angles, "Siddon")
+
+    proj = ax()
+
+    def atb():
+        return tigre_amd.Atb(proj, geo, angles, "FDK")
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ax()
+        atb()
+    torch.cuda.synchronize()
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        ax()
+        ev[k][1].record(stream)
+        atb()
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ax_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    atb_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+    updates = 2.0 * V * nviews
+    value = updates * args.steps / wall / 1e9
+    log("Ax Siddon %.2f ms  Atb FDK %.2f ms  step %.2f ms" % (ax_ms, atb_ms, ms_per_step))
+
+    # the other two operators of the pair, timed the same way
+    def timed(fn, reps=3):
+        fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record(stream)
+        for _ in range(reps):
+            fn()
+        e.record(stream)
+        torch.cuda.synchronize()
+        return s.elapsed_time(e) / reps
+
+    interp_ms = timed(lambda: tigre_amd.Ax(img, geo, angles, "interpolated"))
+    matched_ms = timed(lambda: tigre_amd.Atb(proj, geo, angles, "matched"))
+    log("Ax interpolated %.2f ms  Atb matched %.2f ms" % (interp_ms, matched_ms))
+
+    b_ax = nviews * (V + P) * 4.0
+    b_atb = nviews * (2 * V + P) * 4.0
+    if atb_ms >= ax_ms:
+        dom, dom_ms, dom_b = "atb_fdk", atb_ms, b_atb
+    else:
+        dom, dom_ms, dom_b = "ax_siddon", ax_ms, b_ax
+    achieved = dom_b / (dom_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "Ax+Atb Gupdates/s (Siddon + FDK, cone beam)",
+        "value": round(value, 3),
+        "unit": "Gupdates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 30-ellipsoid phantom)",
+        "config": {"workload": "cone-beam Ax Siddon + Atb FDK, %d^3 volume, %d views, %dx%d detector" % (
+            n, nviews, n, n), "nVoxel": [n, n, n], "nViews": nviews, "nDetector": [n, n],
+            "DSO_mm": 1000.0, "DSD_mm": 1536.0},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "model_bytes_per_launch": dom_b},
+        "ops_ms": {"ax_siddon": round(ax_ms, 3), "atb_fdk": round(atb_ms, 3), "ax_interpolated": round(interp_ms, 3),
+                   "atb_matched": round(matched_ms, 3)},
+        "pair_ms_model_60pct": round((b_ax + b_atb) / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 2),
+    }
+
+    if args.cpu_views > 0:
+        import oracle
+
+        threads = os.cpu_count() or 1
+        oracle.set_threads(threads)
+        idx = np.linspace(0, nviews, args.cpu_views, endpoint=False).astype(int)
+        sub = angles[idx]
+        gp = prepared(geo, sub)
+        img_h = img.cpu().numpy()
+        t = time.perf_counter()
+        ref_p = oracle.ax(img_h, gp, "Siddon")
+        t_ax = time.perf_counter() - t
+        proj_sub = proj[torch.as_tensor(idx, device=dev)].contiguous()
+        t = time.perf_counter()
+        ref_v = oracle.atb(proj_sub.cpu().numpy(), gp, "FDK")
+        t_atb = time.perf_counter() - t
+        got_v = tigre_amd.Atb(proj_sub, geo, sub, "FDK").cpu().numpy()
+        cpu_rate = 2.0 * V * len(idx) / (t_ax + t_atb) / 1e9
+        result["cpu_baseline"] = {
+            "value": round(cpu_rate, 4), "unit": "Gupdates/s", "cores": oracle.get_threads(), "kind": "port",
+            "sample": "FP64 OpenMP oracle: Ax Siddon + Atb FDK over %d of the %d views (%.1f s)" % (
+                len(idx), nviews, t_ax + t_atb)}
+        result["parity_full_size"] = {
+            "ax_siddon_rel_l2": oracle.rel_l2(proj_sub.cpu().numpy(), ref_p),
+            "atb_fdk_rel_l2": oracle.rel_l2(got_v, ref_v), "views": [int(i) for i in idx]}
+        log("cpu baseline %.4f Gupdates/s on %d threads" % (cpu_rate, oracle.get_threads()))
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

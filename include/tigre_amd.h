@@ -1,0 +1,146 @@
+/*
+ * tigre_amd -- C ABI of the MI355X (gfx950) cone-beam projector pair and TV step.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (0 = success, the codes of the reference's Common/CUDA/errors.hpp:1-7
+ * otherwise) and never exits the process or resets the device.  The message of
+ * the last failure on the calling thread is available from tigre_last_error().
+ *
+ * Two families of entry points:
+ *  - host-pointer entry points keep the reference contract (host memory in,
+ *    host memory out, blocking) so an existing Cython/MEX gateway can call them;
+ *  - device-pointer entry points (suffix _device) take device pointers and a
+ *    hipStream_t (passed as void*), enqueue work on that stream and return
+ *    without synchronising; inputs and outputs stay resident in HBM.
+ */
+#ifndef TIGRE_AMD_H
+#define TIGRE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: identical numbering to Common/CUDA/errors.hpp:1-7, plus one
+ * code for arguments the library rejects before touching the device. */
+#define TIGRE_OK                     0
+#define TIGRE_ERR_HIP                1
+#define TIGRE_ERR_NO_CAPABLE_DEVICES 2
+#define TIGRE_ERR_NO_FREE_DEVICES    3
+#define TIGRE_ERR_BAD_ASSERT         4
+#define TIGRE_ERR_ASSERT_FAIL        5
+#define TIGRE_ERR_INVALID_ARGUMENT   6
+
+/* Detector layouts: the reference chooses one at compile time
+ * (IS_FOR_PYTIGRE, Common/CUDA/TIGRE_common.hpp:10-23); here it is a flag. */
+#define TIGRE_LAYOUT_PYTHON 0   /* projection index view*nV*nU + v*nU + u */
+#define TIGRE_LAYOUT_MATLAB 1   /* projection index view*nV*nU + u*nV + v */
+
+/* Projector / backprojector selectors (names as in Ax.py / Atb.py). */
+#define TIGRE_PROJ_SIDDON       0
+#define TIGRE_PROJ_INTERPOLATED 1
+#define TIGRE_BACKPROJ_FDK      0
+#define TIGRE_BACKPROJ_MATCHED  1
+
+/* Storage types of volumes and projections (accumulation is always FP32). */
+#define TIGRE_DTYPE_F32 0
+
+/*
+ * Per-projection geometry.  Field-for-field and byte-for-byte the layout of
+ * the reference's struct Geometry (Common/CUDA/types_TIGRE.hpp:48-85), so a
+ * gateway that fills the reference struct can pass it here unchanged.  The
+ * per-view arrays hold one float per projection; alpha, theta, psi, the
+ * unitX/Y/Z fields and maxLength are scratch in the reference and ignored here.
+ */
+typedef struct TigreGeometry {
+    int   nVoxelX, nVoxelY, nVoxelZ;
+    float sVoxelX, sVoxelY, sVoxelZ;
+    float dVoxelX, dVoxelY, dVoxelZ;
+    float *offOrigX, *offOrigY, *offOrigZ;
+    float *DSO;
+    int   nDetecU, nDetecV;
+    float sDetecU, sDetecV;
+    float dDetecU, dDetecV;
+    float *offDetecU, *offDetecV;
+    float *DSD;
+    float *dRoll;
+    float *dPitch;
+    float *dYaw;
+    float unitX, unitY, unitZ;
+    float alpha, theta, psi;
+    float *COR;
+    float maxLength;
+    float accuracy;
+} TigreGeometry;
+
+/* ---------------------------------------------------------------------- */
+/* Host-pointer entry points (reference contract)                          */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces siddon_ray_projection (Common/CUDA/Siddon_projection.hpp:54) and
+ * interpolation_projection (Common/CUDA/ray_interpolated_projection.hpp:55).
+ * img: nVoxelZ x nVoxelY x nVoxelX floats, x fastest.  result[i] points at
+ * the nDetecU*nDetecV floats of projection i (the reference's float**).
+ * angles: nangles x 3 floats (alpha, theta, psi).  gpuids/ngpus select the
+ * device (only the first id is used: one device per call). */
+int tigre_ax(const float* img, const TigreGeometry* geo, float** result,
+             const float* angles, int nangles, int projection_type, int layout,
+             const int* gpuids, int ngpus);
+
+/* Replaces voxel_backprojection (Common/CUDA/voxel_backprojection.hpp:52)
+ * and voxel_backprojection2 (Common/CUDA/voxel_backprojection2.hpp:57).
+ * projections: nangles projections of nDetecU*nDetecV floats; result is
+ * overwritten with nVoxelZ x nVoxelY x nVoxelX floats. */
+int tigre_atb(const float* projections, const TigreGeometry* geo, float* result,
+              const float* angles, int nangles, int backprojection_type, int layout,
+              const int* gpuids, int ngpus);
+
+/* Replaces pocs_tv (Common/CUDA/GD_TV.hpp:58).  image_size = {X, Y, Z}. */
+int tigre_pocs_tv(const float* img, float* dst, float alpha, const long* image_size,
+                  int maxIter, const int* gpuids, int ngpus);
+
+/* Replaces tvdenoising (Common/CUDA/tv_proximal.hpp:52).  spacing = {dx, dy, dz}. */
+int tigre_tvdenoising(const float* src, float* dst, float lambda, const float* spacing,
+                      const long* image_size, int maxIter, const int* gpuids, int ngpus);
+
+/* ---------------------------------------------------------------------- */
+/* Device-pointer entry points (resident data, caller's stream)            */
+/* ---------------------------------------------------------------------- */
+
+/* Forward projection of a device volume into device projections laid out as
+ * in tigre_ax (contiguous, nangles * nDetecU * nDetecV floats). */
+int tigre_ax_device(const float* d_img, const TigreGeometry* geo, float* d_proj,
+                    const float* angles, int nangles, int projection_type, int layout,
+                    void* stream);
+
+/* Backprojection of device projections into a device volume (overwritten). */
+int tigre_atb_device(const float* d_proj, const TigreGeometry* geo, float* d_vol,
+                     const float* angles, int nangles, int backprojection_type, int layout,
+                     void* stream);
+
+/* maxIter TV steepest-descent steps on a device volume (X fastest); d_dst may
+ * equal d_img.  image_size = {X, Y, Z}. */
+int tigre_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size,
+                         int maxIter, void* stream);
+
+/* ROF TV denoise of a device volume (d_dst may equal d_src). */
+int tigre_tvdenoising_device(const float* d_src, float* d_dst, float lambda, const float* spacing,
+                             const long* image_size, int maxIter, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Devices and diagnostics                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Replace GetGpuCount / GetGpuName (Common/CUDA/gpuUtils.hpp:5-15). */
+int tigre_get_gpu_count(void);
+int tigre_get_gpu_name(int device, char* name, int len);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tigre_last_error(void);
+
+/* Library version string. */
+const char* tigre_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIGRE_AMD_H */

@@ -1,0 +1,158 @@
+"""ctypes binding of the gfx950 library ``tigre_amd/lib/libtigre_amd.so``.
+
+The shared library is the product: every operator in this package calls it.
+There is no CPU fallback; importing an operator without the built library
+raises ``ImportError`` with the build command.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libtigre_amd.so")
+
+LAYOUT_PYTHON = 0
+LAYOUT_MATLAB = 1
+PROJ_SIDDON = 0
+PROJ_INTERPOLATED = 1
+BACKPROJ_FDK = 0
+BACKPROJ_MATCHED = 1
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class TigreGeometry(ctypes.Structure):
+    """Mirror of ``TigreGeometry`` (include/tigre_amd.h), itself byte-compatible
+    with the reference ``Geometry`` (Common/CUDA/types_TIGRE.hpp:48-85)."""
+
+    _fields_ = [
+        ("nVoxelX", ctypes.c_int), ("nVoxelY", ctypes.c_int), ("nVoxelZ", ctypes.c_int),
+        ("sVoxelX", ctypes.c_float), ("sVoxelY", ctypes.c_float), ("sVoxelZ", ctypes.c_float),
+        ("dVoxelX", ctypes.c_float), ("dVoxelY", ctypes.c_float), ("dVoxelZ", ctypes.c_float),
+        ("offOrigX", _fp), ("offOrigY", _fp), ("offOrigZ", _fp),
+        ("DSO", _fp),
+        ("nDetecU", ctypes.c_int), ("nDetecV", ctypes.c_int),
+        ("sDetecU", ctypes.c_float), ("sDetecV", ctypes.c_float),
+        ("dDetecU", ctypes.c_float), ("dDetecV", ctypes.c_float),
+        ("offDetecU", _fp), ("offDetecV", _fp),
+        ("DSD", _fp),
+        ("dRoll", _fp), ("dPitch", _fp), ("dYaw", _fp),
+        ("unitX", ctypes.c_float), ("unitY", ctypes.c_float), ("unitZ", ctypes.c_float),
+        ("alpha", ctypes.c_float), ("theta", ctypes.c_float), ("psi", ctypes.c_float),
+        ("COR", _fp),
+        ("maxLength", ctypes.c_float),
+        ("accuracy", ctypes.c_float),
+    ]
+
+
+# Every symbol include/tigre_amd.h declares, with its ctypes signature.
+SIGNATURES = {
+    "tigre_ax": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.POINTER(_fp),
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_void_p, ctypes.c_int]),
+    "tigre_atb": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_int]),
+    "tigre_pocs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                     ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "tigre_tvdenoising": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "tigre_ax_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]),
+    "tigre_atb_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p]),
+    "tigre_pocs_tv_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                            ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "tigre_tvdenoising_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.c_void_p]),
+    "tigre_get_gpu_count": (ctypes.c_int, []),
+    "tigre_get_gpu_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "tigre_last_error": (ctypes.c_char_p, []),
+    "tigre_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load (once) and return the library; raise ImportError when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "tigre_amd: native library not built (%s). Build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C tigre_amd/csrc`." % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().tigre_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int, where: str) -> None:
+    """Turn a library status code into the reference's exception types."""
+    if status == 0:
+        return
+    from .errors import TigreCudaCallError
+
+    msg = last_error()
+    if status == 6:
+        raise ValueError("%s %s" % (where, msg))
+    raise TigreCudaCallError(where, status, msg)
+
+
+class CGeometry:
+    """A filled ``TigreGeometry`` plus the float32 arrays it points into.
+
+    ``geo`` must already be expanded by ``check_geo`` and rounded by
+    ``cast_to_single``.  Axis conventions follow convert_to_c_geometry
+    (Python/tigre/utilities/cuda_interface/_types.pxd:69-166): Python arrays
+    are (z, y, x) / (v, u), the C struct is (X, Y, Z) / (U, V), and
+    rotDetector[i] = (yaw, pitch, roll)."""
+
+    def __init__(self, geo, nangles: int):
+        f32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))  # noqa: E731
+        off = np.asarray(geo.offOrigin, dtype=np.float32).reshape(nangles, 3)
+        offd = np.asarray(geo.offDetector, dtype=np.float32).reshape(nangles, 2)
+        rot = np.asarray(geo.rotDetector, dtype=np.float32).reshape(nangles, 3)
+        self._keep = {
+            "offOrigX": f32(off[:, 2]), "offOrigY": f32(off[:, 1]), "offOrigZ": f32(off[:, 0]),
+            "DSO": f32(np.broadcast_to(np.asarray(geo.DSO, dtype=np.float32), (nangles,))),
+            "DSD": f32(np.broadcast_to(np.asarray(geo.DSD, dtype=np.float32), (nangles,))),
+            "offDetecU": f32(offd[:, 1]), "offDetecV": f32(offd[:, 0]),
+            "dRoll": f32(rot[:, 2]), "dPitch": f32(rot[:, 1]), "dYaw": f32(rot[:, 0]),
+            "COR": f32(np.broadcast_to(np.asarray(geo.COR, dtype=np.float32), (nangles,))),
+        }
+        s = TigreGeometry()
+        nv = [int(x) for x in np.asarray(geo.nVoxel).reshape(3)]
+        sv = np.asarray(geo.sVoxel, dtype=np.float32).reshape(3)
+        dv = np.asarray(geo.dVoxel, dtype=np.float32).reshape(3)
+        nd = [int(x) for x in np.asarray(geo.nDetector).reshape(2)]
+        sd = np.asarray(geo.sDetector, dtype=np.float32).reshape(2)
+        dd = np.asarray(geo.dDetector, dtype=np.float32).reshape(2)
+        s.nVoxelX, s.nVoxelY, s.nVoxelZ = nv[2], nv[1], nv[0]
+        s.sVoxelX, s.sVoxelY, s.sVoxelZ = float(sv[2]), float(sv[1]), float(sv[0])
+        s.dVoxelX, s.dVoxelY, s.dVoxelZ = float(dv[2]), float(dv[1]), float(dv[0])
+        s.nDetecU, s.nDetecV = nd[1], nd[0]
+        s.sDetecU, s.sDetecV = float(sd[1]), float(sd[0])
+        s.dDetecU, s.dDetecV = float(dd[1]), float(dd[0])
+        s.unitX = s.unitY = s.unitZ = 1.0
+        s.accuracy = float(np.float32(geo.accuracy))
+        for name, arr in self._keep.items():
+            setattr(s, name, arr.ctypes.data_as(_fp))
+        self.struct = s
+
+    def ptr(self):
+        return ctypes.byref(self.struct)

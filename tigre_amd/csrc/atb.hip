@@ -1,0 +1,194 @@
+// Cone-beam voxel-driven backprojectors for gfx950 ('FDK' and 'matched').
+//
+// A 256-thread workgroup owns a 16 x 16 column block of the volume and kZ
+// consecutive slices; each lane keeps its kZ voxels in registers across all
+// views, so the volume is read once and written once per call.  The four
+// 64-lane wavefronts each cover an 8 x 8 column patch: the patch projects
+// to a compact detector footprint, so the bilinear gathers of a wavefront
+// hit a few cache lines.  Per-view parameters are uniform across the
+// workgroup and are fetched with scalar loads.
+//
+// Projections are read from a zero-padded copy (one pixel before, two after
+// on both axes), which replaces the border-addressed linear texture of the
+// reference: taps outside the detector read zeros without bounds checks.
+//
+// kAxial: when no view tilts the rotation axis (dZ has no x/y component, the
+// case of every planar orbit without detector pitch/yaw) the column projects
+// to one detector column, so the u interpolation weights, the 1/x division
+// and the FDK weight are computed once per column and only v steps with z.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace tigre {
+
+namespace {
+constexpr int kZ = 8;
+}
+
+template <bool kMatched, bool kAxial>
+__global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
+                                                  const BwdView* __restrict__ views, float* __restrict__ vol, int nx,
+                                                  int ny, int nz, float dvx, float dvy, float dvz, float dU, float dV,
+                                                  float scale) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int z0 = blockIdx.z * kZ;
+    if (ix >= nx || iy >= ny) return;
+
+    const long long PU = padded_det(nU);
+    const long long PP = PU * padded_det(nV);
+    const float hu = (float)nU, hv = (float)nV;
+    const float fx = (float)ix, fy = (float)iy;
+
+    float acc[kZ];
+#pragma unroll
+    for (int k = 0; k < kZ; ++k) acc[k] = 0.0f;
+
+    for (int p = 0; p < nviews; ++p) {
+        const BwdView& w = views[p];
+        const float* pj = proj + (long long)p * PP;
+        // source -> voxel (ix, iy, z0) vector, frame of computeDeltasCube
+        const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
+        const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
+        const float bz = fmaf(fy, w.dY[2], fmaf(fx, w.dX[2], w.org[2]));
+        // FDK distance weight, constant along the column (voxel_backprojection.cu:241-249)
+        float wfdk = 0.0f;
+        if (!kMatched) {
+            const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
+            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) / w.DSO;
+            wfdk = 1.0f / (ww * ww);
+        }
+        // matched weight: squared in-plane source-voxel distance, constant along z
+        float Axy = 0.0f;
+        if (kMatched) {
+            const float ex = w.Sx - fmaf(fx, dvx, w.rvx0);
+            const float ey = w.Sy - fmaf(fy, dvy, w.rvy0);
+            Axy = ex * ex + ey * ey;
+        }
+        if (kAxial) {
+            const float inv = 1.0f / bx;
+            const float t = w.tnum * inv;
+            const float pu_raw = fmaf(by, t, w.uoff);
+            const float pu = clampf(pu_raw, hu);
+            const float flu = floorf(pu);
+            const float wu = pu - flu;
+            const float* col = pj + ((int)flu + 1);
+            const float dpv = w.dZ[2] * t;
+            const float pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
+            float Day2 = 0.0f;
+            if (kMatched) {
+                const float Day = fmaf(pu_raw, dU, w.day0);
+                Day2 = fmaf(Day, Day, w.DSD * w.DSD);
+            }
+#pragma unroll
+            for (int k = 0; k < kZ; ++k) {
+                const float pv_raw = fmaf((float)k, dpv, pv0);
+                const float pv = clampf(pv_raw, hv);
+                const float flv = floorf(pv);
+                const float wv = pv - flv;
+                const float* q = col + (long long)((int)flv + 1) * PU;
+                const float a0 = fmaf(wu, q[1] - q[0], q[0]);
+                const float a1 = fmaf(wu, q[PU + 1] - q[PU], q[PU]);
+                const float s = fmaf(wv, a1 - a0, a0);
+                if (kMatched) {
+                    const float Dz = fmaf(pv_raw, dV, w.dz0);
+                    const float L2 = fmaf(Dz, Dz, Day2);
+                    const float rvz = fmaf((float)(z0 + k), dvz, w.rvz0);
+                    const float lsq = fmaf(rvz, rvz, Axy);
+                    acc[k] = fmaf(s, L2 * sqrtf(L2) / (w.DSD * lsq), acc[k]);
+                } else {
+                    acc[k] = fmaf(s, wfdk, acc[k]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kZ; ++k) {
+                const float fz = (float)(z0 + k);
+                const float vx = fmaf(fz, w.dZ[0], bx), vy = fmaf(fz, w.dZ[1], by), vz = fmaf(fz, w.dZ[2], bz);
+                const float t = w.tnum / vx;
+                const float pu_raw = fmaf(vy, t, w.uoff), pv_raw = fmaf(vz, t, w.voff);
+                const float pu = clampf(pu_raw, hu);
+                const float pv = clampf(pv_raw, hv);
+                const float flu = floorf(pu), flv = floorf(pv);
+                const float wu = pu - flu, wv = pv - flv;
+                const float* q = pj + ((long long)((int)flv + 1) * PU + ((int)flu + 1));
+                const float a0 = fmaf(wu, q[1] - q[0], q[0]);
+                const float a1 = fmaf(wu, q[PU + 1] - q[PU], q[PU]);
+                const float s = fmaf(wv, a1 - a0, a0);
+                if (kMatched) {
+                    const float Day = fmaf(pu_raw, dU, w.day0);
+                    const float Dz = fmaf(pv_raw, dV, w.dz0);
+                    const float L2 = fmaf(Dz, Dz, fmaf(Day, Day, w.DSD * w.DSD));
+                    const float rvz = fmaf(fz, dvz, w.rvz0);
+                    const float lsq = fmaf(rvz, rvz, Axy);
+                    acc[k] = fmaf(s, L2 * sqrtf(L2) / (w.DSD * lsq), acc[k]);
+                } else {
+                    acc[k] = fmaf(s, wfdk, acc[k]);
+                }
+            }
+        }
+    }
+    const long long plane = (long long)nx * ny;
+    float* out = vol + (long long)z0 * plane + (long long)iy * nx + ix;
+#pragma unroll
+    for (int k = 0; k < kZ; ++k)
+        if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
+}
+
+// Zero-padded, internally laid-out copy of the projections: view-major,
+// rows of padded_det(nU) pixels, one zero pixel before and two after on
+// both axes.  Reads either detector layout.
+__global__ __launch_bounds__(256) void pad_projections_kernel(const float* __restrict__ proj, int nU, int nV,
+                                                              int nviews, int layout, float* __restrict__ padded) {
+    const long long PU = padded_det(nU), PV = padded_det(nV);
+    const long long total = PU * PV * nviews;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long u = q % PU - 1, v = (q / PU) % PV - 1, p = q / (PU * PV);
+        float val = 0.0f;
+        if (u >= 0 && u < nU && v >= 0 && v < nV) {
+            const long long base = p * (long long)nU * nV;
+            val = (layout == TIGRE_LAYOUT_MATLAB) ? proj[base + u * nV + v] : proj[base + v * nU + u];
+        }
+        padded[q] = val;
+    }
+}
+
+int launch_pad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* padded,
+                           hipStream_t stream) {
+    const long long total = padded_det(nU) * padded_det(nV) * (long long)nviews;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(pad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, nU, nV, nviews,
+                       layout, padded);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d_views, bool matched, bool axial,
+               float* vol, hipStream_t stream) {
+    const int nx = g.nVoxel[0], ny = g.nVoxel[1], nz = g.nVoxel[2];
+    dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + kZ - 1) / kZ);
+    TIGRE_REQUIRE(grid.z <= 65535u && grid.y <= 65535u, "volume too large for one launch");
+    // matched: final scaling by dVx dVy dVz / (dU dV) (voxel_backprojection2.cu:585)
+    const float scale = matched ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
+    const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
+                        (float)g.dDetec[1], scale, 0.0f};
+#define TIGRE_ATB_LAUNCH(M, A)                                                                                   \
+    hipLaunchKernelGGL((atb_kernel<M, A>), grid, dim3(256), 0, stream, padded_proj, g.nDetec[0], g.nDetec[1],  \
+                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5])
+    if (matched) {
+        if (axial) TIGRE_ATB_LAUNCH(true, true);
+        else TIGRE_ATB_LAUNCH(true, false);
+    } else {
+        if (axial) TIGRE_ATB_LAUNCH(false, true);
+        else TIGRE_ATB_LAUNCH(false, false);
+    }
+#undef TIGRE_ATB_LAUNCH
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+}  // namespace tigre

@@ -1,0 +1,154 @@
+// Internal definitions shared by the host core and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tigre_amd.h"
+
+namespace tigre {
+
+// ---------------------------------------------------------------------------
+// Error handling: every failure becomes a status code plus a thread-local
+// message; nothing exits the process.
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Status {
+    int code = TIGRE_OK;
+    bool ok() const { return code == TIGRE_OK; }
+};
+
+#define TIGRE_HIP_TRY(expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            ::tigre::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+            return TIGRE_ERR_HIP;                                                        \
+        }                                                                                \
+    } while (0)
+
+#define TIGRE_REQUIRE(cond, msg)                                                         \
+    do {                                                                                 \
+        if (!(cond)) {                                                                   \
+            ::tigre::set_error(msg);                                                     \
+            return TIGRE_ERR_INVALID_ARGUMENT;                                           \
+        }                                                                                \
+    } while (0)
+
+#define TIGRE_CHECK(call)                                                                \
+    do {                                                                                 \
+        int _s = (call);                                                                 \
+        if (_s != TIGRE_OK) return _s;                                                   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Per-view parameter records, computed on the host in FP64 from the float32
+// geometry and uploaded once per call.
+// ---------------------------------------------------------------------------
+
+// Forward projectors.  Positions in voxel-index units.  For the Siddon
+// projector the volume corner is at 0 (voxel i spans [i, i+1)); for the
+// interpolated projector voxel centres sit at integers.
+struct FwdView {
+    double src[3];     // source
+    double org[3];     // centre of detector pixel (u = 0, row counted from the top = 0)
+    double du[3];      // step per detector column u
+    double dv[3];      // step per row counted from the top (pv = nV - 1 - v)
+    double first;      // interpolated: first sample index floor(crop / accuracy)
+    double cap;        // interpolated: (2 DSO / dmin + crop) / accuracy
+    double pad[2];
+};
+static_assert(sizeof(FwdView) == 16 * sizeof(double), "FwdView layout");
+
+// Backprojectors.  x in mm, y and z in detector-pixel units (the frame of
+// the reference's computeDeltasCube).
+struct BwdView {
+    float org[3];      // voxel (0,0,0) centre minus source, with the COR shift in y
+    float dX[3];
+    float dY[3];
+    float dZ[3];
+    float src[3];      // source
+    float tnum;        // DSO - DSD - src.x
+    float uoff;        // nDetecU/2 - 0.5  (texture coordinate -> pixel index)
+    float voff;        // nDetecV/2 - 0.5
+    // FDK weight (voxel_backprojection.cu:243-249): W = 1/((DSO + ry sin - rx cos)/DSO)^2
+    float sina, cosa, DSO, DSD;
+    float realx0, realy0;   // rx = realx0 + ix*dVx, ry = realy0 + iy*dVy
+    // matched weight (voxel_backprojection2.cu:201-279)
+    float rvx0, rvy0, rvz0; // real position of voxel (0,0,0)
+    float Sx, Sy;           // real source position
+    float dax;              // -(DSD - DSO)
+    float day0, dz0;        // detector real-position offsets
+    float pad[8];
+};
+static_assert(sizeof(BwdView) == 40 * sizeof(float), "BwdView layout");
+
+// ---------------------------------------------------------------------------
+// Host-side geometry (FP64 copy of the caller's float32 geometry).
+// ---------------------------------------------------------------------------
+struct HostGeometry {
+    int nVoxel[3];          // X, Y, Z
+    double sVoxel[3];
+    double dVoxel[3];
+    int nDetec[2];          // U, V
+    double sDetec[2];
+    double dDetec[2];
+    double accuracy;
+    int nangles;
+    std::vector<double> angles;    // 3 per view
+    std::vector<double> DSO, DSD, COR;
+    std::vector<double> offOrig;   // 3 per view (X, Y, Z)
+    std::vector<double> offDetec;  // 2 per view (U, V)
+    std::vector<double> rot;       // 3 per view (roll, pitch, yaw)
+};
+
+int import_geometry(const TigreGeometry* geo, const float* angles, int nangles, HostGeometry* out);
+
+// Per-view records (geometry.cpp).
+void forward_views(const HostGeometry& g, bool voxel_centre_origin, std::vector<FwdView>* out);
+void backward_views(const HostGeometry& g, std::vector<BwdView>* out);
+bool backward_is_axial(const HostGeometry& g, const std::vector<BwdView>& v);
+
+// ---------------------------------------------------------------------------
+// Device helpers (runtime.cpp)
+// ---------------------------------------------------------------------------
+int select_device(const int* gpuids, int ngpus);
+int upload_async(void** d_ptr, const void* host, size_t bytes, hipStream_t stream);
+
+// Launchers (one per operator; all enqueue on `stream`).
+int launch_pad_volume(const float* vol, int nx, int ny, int nz, float* padded, hipStream_t stream);
+int launch_ax_siddon(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout,
+                     float* proj, hipStream_t stream);
+int launch_ax_interp(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout,
+                     float* proj, hipStream_t stream);
+int launch_pad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* padded,
+                           hipStream_t stream);
+int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d_views, bool matched,
+               bool axial, float* vol, hipStream_t stream);
+int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
+                hipStream_t stream);
+int run_tvdenoise(const float* src, float* dst, float lambda, const float* spacing, long nx, long ny,
+                  long nz, int maxiter, hipStream_t stream);
+
+// Operator drivers on device pointers (ops.cpp).
+int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int projection_type, int layout,
+              hipStream_t stream);
+int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int backprojection_type,
+               int layout, hipStream_t stream);
+
+// Clamp into [-1, hi]; NaN maps to -1 (fmaxf returns the non-NaN operand), so
+// a clamped coordinate always addresses the zero-padded border or the data.
+__device__ inline float clampf(float x, float hi) { return fminf(fmaxf(x, -1.0f), hi); }
+
+// Padded volume dimensions: one zero voxel below, two above on every axis.
+__host__ __device__ inline long padded_dim(int n) { return (long)n + 3; }
+// Padded projection dimensions: one zero pixel before, two after on both axes.
+__host__ __device__ inline long padded_det(int n) { return (long)n + 3; }
+
+}  // namespace tigre
