@@ -144,7 +144,7 @@ def main():
     if args.cpu_views > 0:
         import oracle
 
-        threads = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         oracle.set_threads(threads)
         idx = np.linspace(0, nviews, args.cpu_views, endpoint=False).astype(int)
         sub = angles[idx]

@@ -146,7 +146,7 @@ def test_matlab_layout(cuda, kind):
         assert oracle.rel_l2(out, ref) <= TOL
     else:
         proj = oracle.ax(img, gp, "Siddon", layout_matlab=True).astype(np.float32)
-        out = np.empty((32, 28, 24), dtype=np.float32)
+        out = np.empty((24, 28, 32), dtype=np.float32)  # (nZ, nY, nX)
         st = L.tigre_atb(proj.ctypes.data_as(ctypes.c_void_p), cg.ptr(), out.ctypes.data_as(ctypes.c_void_p),
                          ang.ctypes.data_as(ctypes.c_void_p), n, 0, _native.LAYOUT_MATLAB, None, 0)
         assert st == 0, _native.last_error()
