@@ -18,20 +18,37 @@ namespace tigre {
 
 namespace {
 
-constexpr int kTile = 16;
+// Lane -> (view, detector pixel).  A wavefront owns an 8 x 8 pixel tile of
+// one view; the four wavefronts of a workgroup take the same tile in four
+// consecutive views, whose rays cross nearly the same voxels.
+constexpr int kTileEdge = 8;
+constexpr int kViewsPerBlock = 4;
 
 struct RayCoords {
-    int u, v;
+    int u, v, view;
 };
 
-// Lane -> detector pixel: 2 x 2 wavefronts of 8 x 8 rays per 16 x 16 tile.
-__device__ inline RayCoords tile_pixel(int tile, int nU) {
-    const int tilesU = (nU + kTile - 1) / kTile;
-    const int tu = tile % tilesU, tv = tile / tilesU;
+// Workgroups are dispatched round-robin over the 8 XCDs; remap the hardware
+// id so that each XCD runs a contiguous range of logical blocks (neighbouring
+// tiles of the same views share that XCD's L2).
+__device__ inline int xcd_logical_block(int b, int nblk) {
+    const int q = nblk / 8, r = nblk % 8;
+    const int xcd = b % 8, local = b / 8;
+    return xcd * q + min(xcd, r) + local;
+}
+
+__device__ inline RayCoords ray_of_lane(int nU, int nV, int nviews) {
+    const int tilesU = (nU + kTileEdge - 1) / kTileEdge;
+    const int tilesV = (nV + kTileEdge - 1) / kTileEdge;
+    const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int tile = L % (tilesU * tilesV), vgroup = L / (tilesU * tilesV);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     RayCoords c;
-    c.u = tu * kTile + (wave & 1) * 8 + (lane & 7);
-    c.v = tv * kTile + (wave >> 1) * 8 + (lane >> 3);
+    c.u = (tile % tilesU) * kTileEdge + (lane & 7);
+    c.v = (tile / tilesU) * kTileEdge + (lane >> 3);
+    c.view = vgroup * kViewsPerBlock + wave;
+    (void)nV;
+    (void)nviews;
     return c;
 }
 
@@ -80,18 +97,23 @@ __device__ inline Axis axis_setup(double s, double r, double am) {
     return a;
 }
 
+// Offset of padded coordinate p along one axis of the 4^3-bricked volume.
+__device__ inline unsigned brick_off(int p, unsigned stride, unsigned unit) {
+    return (unsigned)(p >> 2) * stride + (unsigned)(p & 3) * unit;
+}
+
 }  // namespace
 
 // Siddon / Jacobs exact radiological path, Siddon_projection.cu:110-268.
+// vol: 4^3-bricked, zero-bordered volume (see launch_brick_volume).
 __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                        const FwdView* __restrict__ views, int nU, int nV,
-                                                        long long su, long long sv, long long sview,
-                                                        float* __restrict__ out, double dvx, double dvy,
-                                                        double dvz) {
-    const RayCoords c = tile_pixel(blockIdx.x, nU);
-    if (c.u >= nU || c.v >= nV) return;
-    const int view = blockIdx.y;
-    const FwdView& w = views[view];
+                                                        int nbx, int nby, const FwdView* __restrict__ views,
+                                                        int nviews, int nU, int nV, long long su, long long sv,
+                                                        long long sview, float* __restrict__ out, double dvx,
+                                                        double dvy, double dvz) {
+    const RayCoords c = ray_of_lane(nU, nV, nviews);
+    if (c.u >= nU || c.v >= nV || c.view >= nviews) return;
+    const FwdView& w = views[c.view];
     const double pv = (double)(nV - 1 - c.v);  // detector row flip, :141
     double rx = w.org[0] + c.u * w.du[0] + pv * w.dv[0] - w.src[0];
     double ry = w.org[1] + c.u * w.du[1] + pv * w.dv[1] - w.src[1];
@@ -100,7 +122,7 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     if (fabs(rx) < 1e-3) rx = 0.0;
     if (fabs(ry) < 1e-3) ry = 0.0;
     if (fabs(rz) < 1e-3) rz = 0.0;
-    float* dst = out + (long long)view * sview + (long long)c.u * su + (long long)c.v * sv;
+    float* dst = out + (long long)c.view * sview + (long long)c.u * su + (long long)c.v * sv;
 
     double am = -INFINITY, aM = INFINITY;
     bool hit = slab(w.src[0], rx, nx, am, aM);
@@ -115,9 +137,10 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     const Axis az = axis_setup(w.src[2], rz, am);
     const float T = (float)(aM - am);
 
-    const long long PX = padded_dim(nx), PXY = PX * padded_dim(ny);
-    long long addr = (long long)(az.idx + 1) * PXY + (long long)(ay.idx + 1) * PX + (ax.idx + 1);
-    const long long sx = ax.step, sy = ay.step * PX, sz = az.step * PXY;
+    // padded coordinates (index + 1) and their brick offsets
+    const unsigned SY = 64u * (unsigned)nbx, SZ = SY * (unsigned)nby;
+    int px = ax.idx + 1, py = ay.idx + 1, pz = az.idx + 1;
+    unsigned ox = brick_off(px, 64u, 1u), oy = brick_off(py, SY, 4u), oz = brick_off(pz, SZ, 16u);
 
     float tx = ax.t0, ty = ay.t0, tz = az.t0;
     float kx = 0.0f, ky = 0.0f, kz = 0.0f;
@@ -127,21 +150,24 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     for (int guard = nx + ny + nz + 4; guard > 0; --guard) {
         const float tm = fminf(tx, fminf(ty, tz));
         const float tn = fminf(tm, T);
-        sum = fmaf(tn - tc, vol[addr], sum);
+        sum = fmaf(tn - tc, vol[ox + oy + oz], sum);
         if (tm >= T) break;
         tc = tn;
         if (tx == tm) {
             kx += 1.0f;
             tx = fmaf(kx, ax.dt, ax.t0);
-            addr += sx;
+            px += ax.step;
+            ox = brick_off(px, 64u, 1u);
         } else if (ty == tm) {
             ky += 1.0f;
             ty = fmaf(ky, ay.dt, ay.t0);
-            addr += sy;
+            py += ay.step;
+            oy = brick_off(py, SY, 4u);
         } else {
             kz += 1.0f;
             tz = fmaf(kz, az.dt, az.t0);
-            addr += sz;
+            pz += az.step;
+            oz = brick_off(pz, SZ, 16u);
         }
     }
     // scale by the ray length in mm (:241, :267)
@@ -150,21 +176,24 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
 }
 
 // Uniform-step trilinear sampling, ray_interpolated_projection.cu:117-206.
+// vol: apron-bricked volume: cell (lower corner) c in padded coordinates lies
+// in brick c >> 2, and each brick stores the 5^3 samples 4b .. 4b + 4 of its
+// axis ranges, so the 8 taps of a trilinear sample are one brick base plus the
+// constant offsets {0, 1, 5, 6, 25, 26, 30, 31}.
 template <bool kNearest>
 __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                        const FwdView* __restrict__ views, int nU, int nV,
-                                                        long long su, long long sv, long long sview,
-                                                        float* __restrict__ out, double acc, double dvx,
-                                                        double dvy, double dvz) {
-    const RayCoords c = tile_pixel(blockIdx.x, nU);
-    if (c.u >= nU || c.v >= nV) return;
-    const int view = blockIdx.y;
-    const FwdView& w = views[view];
+                                                        int nbx, int nby, const FwdView* __restrict__ views,
+                                                        int nviews, int nU, int nV, long long su, long long sv,
+                                                        long long sview, float* __restrict__ out, double acc,
+                                                        double dvx, double dvy, double dvz) {
+    const RayCoords c = ray_of_lane(nU, nV, nviews);
+    if (c.u >= nU || c.v >= nV || c.view >= nviews) return;
+    const FwdView& w = views[c.view];
     const double pv = (double)(nV - 1 - c.v);
     const double dx = w.org[0] + c.u * w.du[0] + pv * w.dv[0] - w.src[0];
     const double dy = w.org[1] + c.u * w.du[1] + pv * w.dv[1] - w.src[1];
     const double dz = w.org[2] + c.u * w.du[2] + pv * w.dv[2] - w.src[2];
-    float* dst = out + (long long)view * sview + (long long)c.u * su + (long long)c.v * sv;
+    float* dst = out + (long long)c.view * sview + (long long)c.u * su + (long long)c.v * sv;
 
     // samples per ray and step (:164-169)
     const double n = ceil(sqrt(dx * dx + dy * dy + dz * dz) / acc);
@@ -199,30 +228,36 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
         *dst = 0.0f;
         return;
     }
-    // Positions relative to the first sample keep the FP32 loop accurate.
-    const float qx = (float)(w.src[0] + i0 * stx), qy = (float)(w.src[1] + i0 * sty),
-                qz = (float)(w.src[2] + i0 * stz);
+    // Positions relative to the first sample keep the FP32 loop accurate;
+    // +1 moves them to padded coordinates (index -1 -> 0).
+    const float qx = (float)(w.src[0] + i0 * stx + 1.0), qy = (float)(w.src[1] + i0 * sty + 1.0),
+                qz = (float)(w.src[2] + i0 * stz + 1.0);
     const float fx = (float)stx, fy = (float)sty, fz = (float)stz;
     const int count = (int)(i1 - i0) + 1;
 
-    const long long PX = padded_dim(nx), PXY = PX * padded_dim(ny);
-    const float hx = (float)nx, hy = (float)ny, hz = (float)nz;
+    const unsigned BY = 125u * (unsigned)nbx, BZ = BY * (unsigned)nby;
+    // padded coordinates stay in [0, n + 1] so the cell and its +1 taps exist
+    const float hx = (float)(nx + 1), hy = (float)(ny + 1), hz = (float)(nz + 1);
     float sum = 0.0f;
     for (int m = 0; m < count; ++m) {
         const float fm = (float)m;
-        float x = fmaf(fm, fx, qx), y = fmaf(fm, fy, qy), z = fmaf(fm, fz, qz);
-        x = clampf(x, hx);
-        y = clampf(y, hy);
-        z = clampf(z, hz);
+        const float x = __builtin_amdgcn_fmed3f(fmaf(fm, fx, qx), 0.0f, hx);
+        const float y = __builtin_amdgcn_fmed3f(fmaf(fm, fy, qy), 0.0f, hy);
+        const float z = __builtin_amdgcn_fmed3f(fmaf(fm, fz, qz), 0.0f, hz);
         if (kNearest) {
             const int ix = (int)floorf(x + 0.5f), iy = (int)floorf(y + 0.5f), iz = (int)floorf(z + 0.5f);
-            sum += vol[(long long)(iz + 1) * PXY + (long long)(iy + 1) * PX + (ix + 1)];
+            const unsigned o = (unsigned)(ix >> 2) * 125u + (unsigned)(iy >> 2) * BY + (unsigned)(iz >> 2) * BZ +
+                               (unsigned)(ix & 3) + (unsigned)(iy & 3) * 5u + (unsigned)(iz & 3) * 25u;
+            sum += vol[o];
         } else {
             const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
             const float wx = x - flx, wy = y - fly, wz = z - flz;
-            const float* p = vol + ((long long)((int)flz + 1) * PXY + (long long)((int)fly + 1) * PX + ((int)flx + 1));
-            const float c000 = p[0], c100 = p[1], c010 = p[PX], c110 = p[PX + 1];
-            const float c001 = p[PXY], c101 = p[PXY + 1], c011 = p[PXY + PX], c111 = p[PXY + PX + 1];
+            const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+            const unsigned o = (unsigned)(ix >> 2) * 125u + (unsigned)(iy >> 2) * BY + (unsigned)(iz >> 2) * BZ +
+                               (unsigned)(ix & 3) + (unsigned)(iy & 3) * 5u + (unsigned)(iz & 3) * 25u;
+            const float* p = vol + o;
+            const float c000 = p[0], c100 = p[1], c010 = p[5], c110 = p[6];
+            const float c001 = p[25], c101 = p[26], c011 = p[30], c111 = p[31];
             const float c00 = fmaf(wx, c100 - c000, c000), c10 = fmaf(wx, c110 - c010, c010);
             const float c01 = fmaf(wx, c101 - c001, c001), c11 = fmaf(wx, c111 - c011, c011);
             const float c0 = fmaf(wy, c10 - c00, c00), c1 = fmaf(wy, c11 - c01, c01);
@@ -232,30 +267,46 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     *dst = (float)((double)sum * dl);
 }
 
-// Zero-padded copy of a volume (padding 1 below, 2 above on every axis).
-__global__ __launch_bounds__(256) void pad_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                         float* __restrict__ padded) {
-    const long long PX = padded_dim(nx), PY = padded_dim(ny), PZ = padded_dim(nz);
-    const long long total = PX * PY * PZ;
+// 4^3 bricks over padded coordinates [0, 4 nb): voxel index = padded - 1,
+// zero outside the volume.  Output element q: brick q >> 6, local q & 63.
+__global__ __launch_bounds__(256) void brick_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
+                                                           int nbx, int nby, int nbz, float* __restrict__ bricked) {
+    const long long total = (long long)nbx * nby * nbz * 64;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
-        const long long i = q % PX - 1, j = (q / PX) % PY - 1, k = q / (PX * PY) - 1;
+        const long long b = q >> 6;
+        const int l = (int)(q & 63);
+        const long long bx = b % nbx, by = (b / nbx) % nby, bz = b / ((long long)nbx * nby);
+        const long long i = bx * 4 + (l & 3) - 1, j = by * 4 + ((l >> 2) & 3) - 1, k = bz * 4 + (l >> 4) - 1;
         float v = 0.0f;
         if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
-        padded[q] = v;
+        bricked[q] = v;
     }
 }
 
-int launch_pad_volume(const float* vol, int nx, int ny, int nz, float* padded, hipStream_t stream) {
-    const long long total = padded_dim(nx) * padded_dim(ny) * padded_dim(nz);
-    long long blocks = (total + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(pad_volume_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, vol, nx, ny, nz, padded);
-    TIGRE_HIP_TRY(hipGetLastError());
-    return TIGRE_OK;
+// Apron bricks: brick b stores padded coordinates 4b .. 4b + 4 on each axis
+// (5^3 values, x fastest), so neighbouring bricks share one face.
+__global__ __launch_bounds__(256) void apron_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
+                                                           int nbx, int nby, int nbz, float* __restrict__ bricked) {
+    const long long total = (long long)nbx * nby * nbz * 125;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long b = q / 125;
+        const int l = (int)(q % 125);
+        const long long bx = b % nbx, by = (b / nbx) % nby, bz = b / ((long long)nbx * nby);
+        const long long i = bx * 4 + (l % 5) - 1, j = by * 4 + ((l / 5) % 5) - 1, k = bz * 4 + (l / 25) - 1;
+        float v = 0.0f;
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
+        bricked[q] = v;
+    }
 }
 
 namespace {
+long long grid_stride_blocks(long long total) {
+    long long blocks = (total + 255) / 256;
+    return blocks > 65536 ? 65536 : blocks;
+}
+
 void out_strides(const HostGeometry& g, int layout, long long* su, long long* sv, long long* sview) {
     const long long nU = g.nDetec[0], nV = g.nDetec[1];
     *sview = nU * nV;
@@ -267,38 +318,64 @@ void out_strides(const HostGeometry& g, int layout, long long* su, long long* sv
         *sv = nU;
     }
 }
+
+int ray_blocks(const HostGeometry& g) {
+    const long long tiles = (long long)((g.nDetec[0] + kTileEdge - 1) / kTileEdge) *
+                            ((g.nDetec[1] + kTileEdge - 1) / kTileEdge);
+    const long long groups = (g.nangles + kViewsPerBlock - 1) / kViewsPerBlock;
+    return (int)(tiles * groups);
+}
 }  // namespace
 
-int launch_ax_siddon(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout, float* proj,
-                     hipStream_t stream) {
-    long long su, sv, sview;
-    out_strides(g, layout, &su, &sv, &sview);
-    const int nU = g.nDetec[0], nV = g.nDetec[1];
-    const int tiles = ((nU + kTile - 1) / kTile) * ((nV + kTile - 1) / kTile);
-    TIGRE_REQUIRE(g.nangles <= 65535, "at most 65535 projections per call");
-    hipLaunchKernelGGL(ax_siddon_kernel, dim3(tiles, g.nangles), dim3(256), 0, stream, padded, g.nVoxel[0],
-                       g.nVoxel[1], g.nVoxel[2], d_views, nU, nV, su, sv, sview, proj, g.dVoxel[0], g.dVoxel[1],
-                       g.dVoxel[2]);
+BrickGrid siddon_bricks(const HostGeometry& g) {
+    // padded coordinates 0 .. n + 2
+    return BrickGrid{(g.nVoxel[0] + 2) / 4 + 1, (g.nVoxel[1] + 2) / 4 + 1, (g.nVoxel[2] + 2) / 4 + 1, 64};
+}
+
+BrickGrid interp_bricks(const HostGeometry& g) {
+    // cells (lower taps) 0 .. n + 1
+    return BrickGrid{(g.nVoxel[0] + 1) / 4 + 1, (g.nVoxel[1] + 1) / 4 + 1, (g.nVoxel[2] + 1) / 4 + 1, 125};
+}
+
+int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
+                        hipStream_t stream) {
+    const long long total = b.elements();
+    if (b.per_brick == 64)
+        hipLaunchKernelGGL(brick_volume_kernel, dim3((unsigned)grid_stride_blocks(total)), dim3(256), 0, stream, vol,
+                           g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
+    else
+        hipLaunchKernelGGL(apron_volume_kernel, dim3((unsigned)grid_stride_blocks(total)), dim3(256), 0, stream, vol,
+                           g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }
 
-int launch_ax_interp(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout, float* proj,
-                     hipStream_t stream) {
+int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream) {
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
-    const int nU = g.nDetec[0], nV = g.nDetec[1];
-    const int tiles = ((nU + kTile - 1) / kTile) * ((nV + kTile - 1) / kTile);
-    TIGRE_REQUIRE(g.nangles <= 65535, "at most 65535 projections per call");
+    TIGRE_REQUIRE(b.elements() < (1LL << 32), "volume too large for 32-bit brick offsets");
+    hipLaunchKernelGGL(ax_siddon_kernel, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
+                       g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview, proj,
+                       g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream) {
+    long long su, sv, sview;
+    out_strides(g, layout, &su, &sv, &sview);
+    TIGRE_REQUIRE(b.elements() < (1LL << 32), "volume too large for 32-bit brick offsets");
     // point filtering when accuracy > 1 (ray_interpolated_projection.cu:589-595)
     if (g.accuracy > 1.0)
-        hipLaunchKernelGGL(ax_interp_kernel<true>, dim3(tiles, g.nangles), dim3(256), 0, stream, padded, g.nVoxel[0],
-                           g.nVoxel[1], g.nVoxel[2], d_views, nU, nV, su, sv, sview, proj, g.accuracy, g.dVoxel[0],
-                           g.dVoxel[1], g.dVoxel[2]);
+        hipLaunchKernelGGL(ax_interp_kernel<true>, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0],
+                           g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su,
+                           sv, sview, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     else
-        hipLaunchKernelGGL(ax_interp_kernel<false>, dim3(tiles, g.nangles), dim3(256), 0, stream, padded,
-                           g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], d_views, nU, nV, su, sv, sview, proj, g.accuracy,
-                           g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+        hipLaunchKernelGGL(ax_interp_kernel<false>, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0],
+                           g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su,
+                           sv, sview, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

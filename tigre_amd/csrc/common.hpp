@@ -121,12 +121,22 @@ bool backward_is_axial(const HostGeometry& g, const std::vector<BwdView>& v);
 int select_device(const int* gpuids, int ngpus);
 int upload_async(void** d_ptr, const void* host, size_t bytes, hipStream_t stream);
 
+// Bricked copies of a volume for the forward projectors.
+struct BrickGrid {
+    int nbx, nby, nbz;   // bricks per axis
+    int per_brick;       // 64 (4^3 bricks) or 125 (5^3 apron bricks)
+    long long elements() const { return (long long)nbx * nby * nbz * per_brick; }
+};
+BrickGrid siddon_bricks(const HostGeometry& g);
+BrickGrid interp_bricks(const HostGeometry& g);
+
 // Launchers (one per operator; all enqueue on `stream`).
-int launch_pad_volume(const float* vol, int nx, int ny, int nz, float* padded, hipStream_t stream);
-int launch_ax_siddon(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout,
-                     float* proj, hipStream_t stream);
-int launch_ax_interp(const float* padded, const HostGeometry& g, const FwdView* d_views, int layout,
-                     float* proj, hipStream_t stream);
+int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
+                        hipStream_t stream);
+int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream);
+int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream);
 int launch_pad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* padded,
                            hipStream_t stream);
 int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d_views, bool matched,
@@ -146,8 +156,6 @@ int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int bac
 // a clamped coordinate always addresses the zero-padded border or the data.
 __device__ inline float clampf(float x, float hi) { return fminf(fmaxf(x, -1.0f), hi); }
 
-// Padded volume dimensions: one zero voxel below, two above on every axis.
-__host__ __device__ inline long padded_dim(int n) { return (long)n + 3; }
 // Padded projection dimensions: one zero pixel before, two after on both axes.
 __host__ __device__ inline long padded_det(int n) { return (long)n + 3; }
 

@@ -15,16 +15,15 @@ int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int proj
     forward_views(g, interp, &views);
     void* d_views = nullptr;
     TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
-    const size_t padded_bytes =
-        (size_t)padded_dim(g.nVoxel[0]) * padded_dim(g.nVoxel[1]) * padded_dim(g.nVoxel[2]) * sizeof(float);
-    float* padded = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&padded, padded_bytes, stream));
-    TIGRE_CHECK(launch_pad_volume(d_img, g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], padded, stream));
+    const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
+    float* bricked = nullptr;
+    TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
+    TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
     if (interp)
-        TIGRE_CHECK(launch_ax_interp(padded, g, (const FwdView*)d_views, layout, d_proj, stream));
+        TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
     else
-        TIGRE_CHECK(launch_ax_siddon(padded, g, (const FwdView*)d_views, layout, d_proj, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(padded, stream));
+        TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
+    TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
     return TIGRE_OK;
 }
