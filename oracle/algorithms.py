@@ -1,0 +1,205 @@
+"""TEST INFRASTRUCTURE ONLY: FP64 NumPy restatements of the reference's
+reconstruction loops (the callers of Ax/Atb/minimizeTV), driven by the FP64
+oracle operators.  Used to check tigre_amd.algorithms on small problems.
+
+Followed files (Python/tigre/):
+  utilities/filtering.py:15-87          ramp/Shepp-Logan/... filtering
+  algorithms/single_pass_algorithms.py:12-202   FDK
+  utilities/order_subsets.py:31-96      ordered subsets
+  algorithms/iterative_recon_alg.py:214-401     W, V, OS-SART update
+  algorithms/krylov_subspace_algorithms.py:22-97 CGLS
+  algorithms/pocs_algorithms.py:105-169 ASD-POCS
+"""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+from scipy.fft import fft, ifft
+
+from . import atb, ax, pocs_tv
+
+
+def prep(geo, angles):
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    g.cast_to_single()
+    return g
+
+
+def sub_geometry(geo, idx):
+    """Geometry restricted to the views idx (iterative_recon_alg.py:335-341)."""
+    g = copy.deepcopy(geo)
+    for a in ("DSD", "DSO", "COR"):
+        setattr(g, a, np.asarray(getattr(geo, a))[idx])
+    for a in ("offOrigin", "offDetector", "rotDetector"):
+        setattr(g, a, np.asarray(getattr(geo, a))[idx])
+    return g
+
+
+# -- FDK ---------------------------------------------------------------------------
+def ramp_filter(geo, filter_name=None):
+    """Frequency response used by filtering (filtering.py:18-23, :56-87)."""
+    n = int(geo.nDetector[1])
+    i = 1
+    while 2 ** i <= 2 * n:
+        i += 1
+    filt_len = max(64, 2 ** i)
+    nn = np.arange(-filt_len / 2, filt_len / 2)
+    h = np.zeros(nn.shape, dtype=np.float32)
+    h[int(filt_len / 2)] = 0.25
+    odd = nn % 2 == 1
+    h[odd] = -1 / (np.pi * nn[odd]) ** 2
+    f = np.abs(np.fft.fft(h)) * 2
+    filt = f[: int(filt_len / 2 + 1)]
+    w = 2 * np.pi * np.arange(len(filt)) / filt_len
+    d = 1
+    if filter_name == "shepp_logan":
+        filt[1:] *= np.sin(w[1:] / (2 * d)) / (w[1:] / (2 * d))
+    elif filter_name == "cosine":
+        filt[1:] *= np.cos(w[1:] / (2 * d))
+    elif filter_name == "hamming":
+        filt[1:] *= 0.54 + 0.46 * np.cos(w[1:] / d)
+    elif filter_name == "hann":
+        filt[1:] *= (1 + np.cos(w[1:] / d)) / 2
+    elif filter_name not in ("ram_lak", None):
+        raise ValueError("filter not recognized: " + str(filter_name))
+    filt[w > np.pi * d] = 0
+    return np.hstack((filt, filt[1:-1][::-1])).astype(np.float32), filt_len
+
+
+def fdk(proj, geo, angles, filter_name=None):
+    g = prep(geo, angles)
+    nV, nU = int(g.nDetector[0]), int(g.nDetector[1])
+    dV, dU = float(g.dDetector[0]), float(g.dDetector[1])
+    DSD0, DSO0 = float(np.asarray(g.DSD)[0]), float(np.asarray(g.DSO)[0])
+    xv = (np.arange(nU) - nU / 2 + 0.5) * dU
+    yv = (np.arange(nV) - nV / 2 + 0.5) * dV
+    xx, yy = np.meshgrid(xv, yv)
+    w = DSD0 / np.sqrt(DSD0 ** 2 + xx ** 2 + yy ** 2)
+    p = np.asarray(proj, dtype=np.float64) * w
+    filt, L = ramp_filter(g, filter_name)
+    pad = (L - nU) // 2
+    scale = (DSD0 / DSO0) * (2 * np.pi / len(g.angles)) / (4 * dU)
+    buf = np.zeros((p.shape[0], nV, L))
+    buf[:, :, pad:pad + nU] = p
+    out = np.real(ifft(fft(buf, axis=2) * filt.astype(np.float64), axis=2))[:, :, pad:pad + nU] * scale
+    return atb(out.astype(np.float32), g, "FDK"), out
+
+
+# -- OS-SART -----------------------------------------------------------------------
+def ordered_blocks(n, blocksize):
+    nb = (n + blocksize - 1) // blocksize
+    return [np.arange(i * blocksize, min(n, (i + 1) * blocksize)) for i in range(nb)]
+
+
+def weights_w(geo, angles):
+    """W = 1 / Ax(ones(2^3)) over an enlarged volume (iterative_recon_alg.py:214-230)."""
+    g = copy.deepcopy(geo)
+    g.sVoxel = np.asarray(g.sVoxel, dtype=np.float64).copy()
+    g.sVoxel[1:] = g.sVoxel[1:] * 1.1
+    g.sVoxel[0] = max(g.sDetector[0], g.sVoxel[0])
+    g.nVoxel = np.array([2, 2, 2])
+    g.dVoxel = g.sVoxel / g.nVoxel
+    W = ax(np.ones((2, 2, 2), dtype=np.float32), prep(g, angles), "Siddon")
+    W[W <= min(np.asarray(geo.dVoxel) / 2)] = np.inf
+    return 1.0 / W
+
+
+def weights_v(geo, angles, blocks):
+    """V per block: mean over z of Atb(ones) over a shrunk volume (iterative_recon_alg.py:232-270)."""
+    out = []
+    for idx in blocks:
+        g = sub_geometry(geo, idx)
+        s = np.asarray(g.sVoxel, dtype=np.float64)
+        g.sVoxel = s * np.max(s[1:] / np.linalg.norm(s[1:])) * 0.9
+        g.dVoxel = g.sVoxel / np.asarray(g.nVoxel)
+        ones = np.ones((len(idx), int(g.nDetector[0]), int(g.nDetector[1])), dtype=np.float32)
+        out.append(atb(ones, prep(g, angles[idx]), "FDK").mean(axis=0))
+    V = np.stack(out)
+    V[V == 0.0] = np.inf
+    return V
+
+
+def os_sart(proj, geo, angles, niter, blocksize=20, lmbda=1.0, noneg=True, init=None):
+    """iterative_recon_alg.py:326-401 with ordered subsets."""
+    geo = copy.deepcopy(geo)
+    geo.check_geo(angles)
+    ang = geo.angles
+    blocks = ordered_blocks(len(ang), blocksize)
+    W = weights_w(geo, ang)
+    V = weights_v(geo, ang, blocks)
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel)) if init is None else np.array(init, dtype=np.float64)
+    proj = np.asarray(proj, dtype=np.float64)
+    for _ in range(niter):
+        for j, idx in enumerate(blocks):
+            g = prep(sub_geometry(geo, idx), ang[idx])
+            r = W[idx] * (proj[idx] - ax(res.astype(np.float32), g, "Siddon"))
+            res = res + lmbda / V[j] * atb(r.astype(np.float32), g, "FDK")
+            if noneg:
+                res = np.maximum(res, 0)
+    return res
+
+
+# -- CGLS --------------------------------------------------------------------------
+def cgls(proj, geo, angles, niter):
+    """krylov_subspace_algorithms.py:45-97 (no restart branch), FP64 norms."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    x = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    r = proj - ax(x.astype(np.float32), g, "Siddon")
+    p = atb(r.astype(np.float32), g, "matched")
+    gamma = float(np.sum(p * p))
+    res_norms = []
+    for _ in range(niter):
+        q = ax(p.astype(np.float32), g, "Siddon")
+        alpha = gamma / float(np.sum(q * q))
+        x = x + alpha * p
+        res_norms.append(float(np.linalg.norm(proj - ax(x.astype(np.float32), g, "Siddon"))))
+        r = r - alpha * q
+        s = atb(r.astype(np.float32), g, "matched")
+        gamma1 = float(np.sum(s * s))
+        p = s + (gamma1 / gamma) * p
+        gamma = gamma1
+    return x, res_norms
+
+
+# -- ASD-POCS ----------------------------------------------------------------------
+def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, tviter=20, lmbda=1.0,
+             lmbda_red=1.0, epsilon=None):
+    """pocs_algorithms.py:105-169 with blocksize 1 (one view per SART step)."""
+    geo = copy.deepcopy(geo)
+    geo.check_geo(angles)
+    ang = geo.angles
+    gfull = prep(geo, ang)
+    proj = np.asarray(proj, dtype=np.float64)
+    blocks = ordered_blocks(len(ang), 1)
+    W = weights_w(geo, ang)
+    V = weights_v(geo, ang, blocks)
+    if epsilon is None:
+        epsilon = 0.2 * np.linalg.norm(ax(fdk(proj, geo, angles)[0].astype(np.float32), gfull) - proj)
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    beta, n_iter = lmbda, 0
+    dtvg = None
+    while True:
+        res_prev = res.copy()
+        for j, idx in enumerate(blocks):
+            g = prep(sub_geometry(geo, idx), ang[idx])
+            r = W[idx] * (proj[idx] - ax(res.astype(np.float32), g, "Siddon"))
+            res = np.maximum(res + lmbda / V[j] * atb(r.astype(np.float32), g, "FDK"), 0)
+        n_iter += 1
+        dd = np.linalg.norm(ax(res.astype(np.float32), gfull) - proj)
+        dp_vec = res - res_prev
+        dp = np.linalg.norm(dp_vec)
+        if n_iter == 1:
+            dtvg = alpha * dp
+        res_prev = res.copy()
+        res = pocs_tv(res, dtvg, tviter)
+        dg_vec = res - res_prev
+        dg = np.linalg.norm(dg_vec)
+        if dg > rmax * dp and dd > epsilon:
+            dtvg *= alpha_red
+        beta *= lmbda_red
+        c = float(np.dot(dg_vec.ravel(), dp_vec.ravel())) / max(dg * dp, 1e-6)
+        if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter >= niter:
+            return res
