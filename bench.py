@@ -39,6 +39,7 @@ def main():
     ap.add_argument("--n", type=int, default=512, help="volume edge / detector edge")
     ap.add_argument("--views", type=int, default=360)
     ap.add_argument("--cpu-views", type=int, default=8, help="views in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -105,8 +106,11 @@ def main():
         torch.cuda.synchronize()
         return s.elapsed_time(e) / reps
 
-    interp_ms = timed(lambda: tigre_amd.Ax(img, geo, angles, "interpolated"))
-    matched_ms = timed(lambda: tigre_amd.Atb(proj, geo, angles, "matched"))
+    if args.no_extra:
+        interp_ms = matched_ms = float("nan")
+    else:
+        interp_ms = timed(lambda: tigre_amd.Ax(img, geo, angles, "interpolated"))
+        matched_ms = timed(lambda: tigre_amd.Atb(proj, geo, angles, "matched"))
     log("Ax interpolated %.2f ms  Atb matched %.2f ms" % (interp_ms, matched_ms))
 
     b_ax = nviews * (V + P) * 4.0

@@ -8,9 +8,12 @@
 // hit a few cache lines.  Per-view parameters are uniform across the
 // workgroup and are fetched with scalar loads.
 //
-// Projections are read from a zero-padded copy (one pixel before, two after
-// on both axes), which replaces the border-addressed linear texture of the
-// reference: taps outside the detector read zeros without bounds checks.
+// Projections are read from a "quad" copy: element (u, v) holds the four
+// bilinear taps p(u,v), p(u+1,v), p(u,v+1), p(u+1,v+1) (zero outside the
+// detector) for u in [-1, nU], v in [-1, nV].  One 16-byte load per voxel
+// update replaces four scattered taps (the texture-address unit, not the ALU,
+// bounds this kernel), and the zero border replaces the border-addressed
+// linear texture of the reference without bounds checks.
 //
 // kAxial: when no view tilts the rotation axis (dZ has no x/y component, the
 // case of every planar orbit without detector pitch/yaw) the column projects
@@ -23,7 +26,10 @@
 namespace tigre {
 
 namespace {
-constexpr int kZ = 8;
+constexpr int kZ = 16;
+
+// 1/x with the hardware reciprocal (1 ulp), instead of a correctly rounded division.
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 }
 
 template <bool kMatched, bool kAxial>
@@ -37,8 +43,9 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
     const int z0 = blockIdx.z * kZ;
     if (ix >= nx || iy >= ny) return;
 
-    const long long PU = padded_det(nU);
-    const long long PP = PU * padded_det(nV);
+    const long long QU = quad_det(nU);
+    const long long QP = QU * quad_det(nV);
+    const float4* __restrict__ quads = reinterpret_cast<const float4*>(proj);
     const float hu = (float)nU, hv = (float)nV;
     const float fx = (float)ix, fy = (float)iy;
 
@@ -48,7 +55,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
 
     for (int p = 0; p < nviews; ++p) {
         const BwdView& w = views[p];
-        const float* pj = proj + (long long)p * PP;
+        const float4* pj = quads + (long long)p * QP;
         // source -> voxel (ix, iy, z0) vector, frame of computeDeltasCube
         const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
         const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
@@ -57,8 +64,8 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         float wfdk = 0.0f;
         if (!kMatched) {
             const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
-            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) / w.DSO;
-            wfdk = 1.0f / (ww * ww);
+            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) * w.invDSO;
+            wfdk = rcp(ww * ww);
         }
         // matched weight: squared in-plane source-voxel distance, constant along z
         float Axy = 0.0f;
@@ -68,13 +75,12 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             Axy = ex * ex + ey * ey;
         }
         if (kAxial) {
-            const float inv = 1.0f / bx;
-            const float t = w.tnum * inv;
+            const float t = w.tnum * rcp(bx);
             const float pu_raw = fmaf(by, t, w.uoff);
             const float pu = clampf(pu_raw, hu);
             const float flu = floorf(pu);
             const float wu = pu - flu;
-            const float* col = pj + ((int)flu + 1);
+            const float4* col = pj + ((int)flu + 1);
             const float dpv = w.dZ[2] * t;
             const float pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
             float Day2 = 0.0f;
@@ -88,16 +94,16 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                 const float pv = clampf(pv_raw, hv);
                 const float flv = floorf(pv);
                 const float wv = pv - flv;
-                const float* q = col + (long long)((int)flv + 1) * PU;
-                const float a0 = fmaf(wu, q[1] - q[0], q[0]);
-                const float a1 = fmaf(wu, q[PU + 1] - q[PU], q[PU]);
+                const float4 q = col[(long long)((int)flv + 1) * QU];
+                const float a0 = fmaf(wu, q.y - q.x, q.x);
+                const float a1 = fmaf(wu, q.w - q.z, q.z);
                 const float s = fmaf(wv, a1 - a0, a0);
                 if (kMatched) {
                     const float Dz = fmaf(pv_raw, dV, w.dz0);
                     const float L2 = fmaf(Dz, Dz, Day2);
                     const float rvz = fmaf((float)(z0 + k), dvz, w.rvz0);
                     const float lsq = fmaf(rvz, rvz, Axy);
-                    acc[k] = fmaf(s, L2 * sqrtf(L2) / (w.DSD * lsq), acc[k]);
+                    acc[k] = fmaf(s, L2 * __builtin_amdgcn_sqrtf(L2) * rcp(w.DSD * lsq), acc[k]);
                 } else {
                     acc[k] = fmaf(s, wfdk, acc[k]);
                 }
@@ -107,15 +113,15 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             for (int k = 0; k < kZ; ++k) {
                 const float fz = (float)(z0 + k);
                 const float vx = fmaf(fz, w.dZ[0], bx), vy = fmaf(fz, w.dZ[1], by), vz = fmaf(fz, w.dZ[2], bz);
-                const float t = w.tnum / vx;
+                const float t = w.tnum * rcp(vx);
                 const float pu_raw = fmaf(vy, t, w.uoff), pv_raw = fmaf(vz, t, w.voff);
                 const float pu = clampf(pu_raw, hu);
                 const float pv = clampf(pv_raw, hv);
                 const float flu = floorf(pu), flv = floorf(pv);
                 const float wu = pu - flu, wv = pv - flv;
-                const float* q = pj + ((long long)((int)flv + 1) * PU + ((int)flu + 1));
-                const float a0 = fmaf(wu, q[1] - q[0], q[0]);
-                const float a1 = fmaf(wu, q[PU + 1] - q[PU], q[PU]);
+                const float4 q = pj[(long long)((int)flv + 1) * QU + ((int)flu + 1)];
+                const float a0 = fmaf(wu, q.y - q.x, q.x);
+                const float a1 = fmaf(wu, q.w - q.z, q.z);
                 const float s = fmaf(wv, a1 - a0, a0);
                 if (kMatched) {
                     const float Day = fmaf(pu_raw, dU, w.day0);
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                     const float L2 = fmaf(Dz, Dz, fmaf(Day, Day, w.DSD * w.DSD));
                     const float rvz = fmaf(fz, dvz, w.rvz0);
                     const float lsq = fmaf(rvz, rvz, Axy);
-                    acc[k] = fmaf(s, L2 * sqrtf(L2) / (w.DSD * lsq), acc[k]);
+                    acc[k] = fmaf(s, L2 * __builtin_amdgcn_sqrtf(L2) * rcp(w.DSD * lsq), acc[k]);
                 } else {
                     acc[k] = fmaf(s, wfdk, acc[k]);
                 }
@@ -137,37 +143,41 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
 }
 
-// Zero-padded, internally laid-out copy of the projections: view-major,
-// rows of padded_det(nU) pixels, one zero pixel before and two after on
-// both axes.  Reads either detector layout.
-__global__ __launch_bounds__(256) void pad_projections_kernel(const float* __restrict__ proj, int nU, int nV,
-                                                              int nviews, int layout, float* __restrict__ padded) {
-    const long long PU = padded_det(nU), PV = padded_det(nV);
-    const long long total = PU * PV * nviews;
+// Quad copy of the projections (see above), view-major, rows of quad_det(nU)
+// elements.  Reads either detector layout.
+__global__ __launch_bounds__(256) void quad_projections_kernel(const float* __restrict__ proj, int nU, int nV,
+                                                               int nviews, int layout, float4* __restrict__ quads) {
+    const long long QU = quad_det(nU), QV = quad_det(nV);
+    const long long total = QU * QV * nviews;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
-        const long long u = q % PU - 1, v = (q / PU) % PV - 1, p = q / (PU * PV);
-        float val = 0.0f;
-        if (u >= 0 && u < nU && v >= 0 && v < nV) {
-            const long long base = p * (long long)nU * nV;
-            val = (layout == TIGRE_LAYOUT_MATLAB) ? proj[base + u * nV + v] : proj[base + v * nU + u];
+        const long long u = q % QU - 1, v = (q / QU) % QV - 1, p = q / (QU * QV);
+        const long long base = p * (long long)nU * nV;
+        float t[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const long long uu = u + (k & 1), vv = v + (k >> 1);
+            float val = 0.0f;
+            if (uu >= 0 && uu < nU && vv >= 0 && vv < nV)
+                val = (layout == TIGRE_LAYOUT_MATLAB) ? proj[base + uu * nV + vv] : proj[base + vv * nU + uu];
+            t[k] = val;
         }
-        padded[q] = val;
+        quads[q] = make_float4(t[0], t[1], t[2], t[3]);
     }
 }
 
-int launch_pad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* padded,
-                           hipStream_t stream) {
-    const long long total = padded_det(nU) * padded_det(nV) * (long long)nviews;
+int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,
+                            hipStream_t stream) {
+    const long long total = quad_det(nU) * quad_det(nV) * (long long)nviews;
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(pad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, nU, nV, nviews,
-                       layout, padded);
+    hipLaunchKernelGGL(quad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, nU, nV, nviews,
+                       layout, reinterpret_cast<float4*>(quads));
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }
 
-int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d_views, bool matched, bool axial,
+int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched, bool axial,
                float* vol, hipStream_t stream) {
     const int nx = g.nVoxel[0], ny = g.nVoxel[1], nz = g.nVoxel[2];
     dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + kZ - 1) / kZ);
@@ -177,7 +187,7 @@ int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d
     const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
                         (float)g.dDetec[1], scale, 0.0f};
 #define TIGRE_ATB_LAUNCH(M, A)                                                                                   \
-    hipLaunchKernelGGL((atb_kernel<M, A>), grid, dim3(256), 0, stream, padded_proj, g.nDetec[0], g.nDetec[1],  \
+    hipLaunchKernelGGL((atb_kernel<M, A>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],        \
                        g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5])
     if (matched) {
         if (axial) TIGRE_ATB_LAUNCH(true, true);

@@ -7,9 +7,15 @@
 // exit, first crossings) runs in FP64 -- on CDNA4 FP64 FMA has the same vector
 // rate as non-packed FP32 -- and the inner loop runs in FP32.
 //
-// The volume is read from a zero-padded copy (one voxel below, two above on
-// every axis), which replaces the reference's border-addressed texture: taps
-// that fall outside the volume read zeros without bounds checks.
+// The volume is read from re-laid-out, zero-bordered copies (one voxel below,
+// two above on every axis), which replace the reference's border-addressed
+// texture: taps that fall outside the volume read zeros without bounds checks.
+//  - Siddon: "line" layouts.  A 128-byte cache line holds an 8 x 4 patch of
+//    voxels across the ray direction, so the 64 rays of a wavefront touch a
+//    handful of lines per step.  Views whose rays run mostly along x read the
+//    x-major copy (patch in y-z), the others the y-major copy (patch in x-z);
+//    the per-step offset along the major axis is then a single add.
+//  - interpolated: 5^3 "apron" bricks, so the 8 trilinear taps share one base.
 #include <cmath>
 
 #include "common.hpp"
@@ -97,20 +103,51 @@ __device__ inline Axis axis_setup(double s, double r, double am) {
     return a;
 }
 
-// Offset of padded coordinate p along one axis of the 4^3-bricked volume.
+// Offset of padded coordinate p along one axis of the 4^3-bricked volume
+// (p >= 0; stride < 2^24, so the 24-bit multiplier suffices).
 __device__ inline unsigned brick_off(int p, unsigned stride, unsigned unit) {
-    return (unsigned)(p >> 2) * stride + (unsigned)(p & 3) * unit;
+    return __umul24((unsigned)p >> 2, stride) + ((unsigned)p & 3u) * unit;
 }
+
+// Volume reads.  Below 4 GiB the volume is read through a raw buffer
+// descriptor whose range is the volume: an element index outside it reads 0
+// instead of touching memory.  Larger volumes use plain global loads.
+template <bool kBuffer>
+struct VolumeReader {
+    __amdgpu_buffer_rsrc_t rsrc;
+    const float* base;
+    __device__ VolumeReader(const float* p, unsigned long long elements) : base(p) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0,
+                                                 kBuffer ? (int)(unsigned)(elements * 4ull) : 0, 0x00020000);
+    }
+    __device__ float operator()(unsigned index) const {
+        if (kBuffer) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, index * 4u, 0, 0));
+        return base[index];
+    }
+};
 
 }  // namespace
 
+// Line-layout offsets.  Padded coordinate p (= voxel index + 1) along the
+// major axis: p * 32; along the other in-plane axis: (p >> 3) * SB + (p & 7);
+// along z: (p >> 2) * SZ + (p & 3) * 8.
+struct LineGrid {
+    unsigned SB, SZ;
+};
+__device__ inline unsigned line_off_b(int p, unsigned SB) { return __umul24((unsigned)p >> 3, SB) + ((unsigned)p & 7u); }
+__device__ inline unsigned line_off_z(int p, unsigned SZ) {
+    return __umul24((unsigned)p >> 2, SZ) + (((unsigned)p & 3u) << 3);
+}
+
 // Siddon / Jacobs exact radiological path, Siddon_projection.cu:110-268.
-// vol: 4^3-bricked, zero-bordered volume (see launch_brick_volume).
-__global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                        int nbx, int nby, const FwdView* __restrict__ views,
-                                                        int nviews, int nU, int nV, long long su, long long sv,
-                                                        long long sview, float* __restrict__ out, double dvx,
-                                                        double dvy, double dvz) {
+// vol: line-layout copy (x-major when !kYMajor).  views: the views of this
+// launch; pad[0] of each record holds its projection index in `out`.
+template <bool kBuffer, bool kYMajor>
+__global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict__ vol, unsigned long long elements,
+                                                        int nx, int ny, int nz, unsigned SB, unsigned SZ,
+                                                        const FwdView* __restrict__ views, int nviews, int nU, int nV,
+                                                        long long su, long long sv, long long sview,
+                                                        float* __restrict__ out, double dvx, double dvy, double dvz) {
     const RayCoords c = ray_of_lane(nU, nV, nviews);
     if (c.u >= nU || c.v >= nV || c.view >= nviews) return;
     const FwdView& w = views[c.view];
@@ -122,7 +159,7 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     if (fabs(rx) < 1e-3) rx = 0.0;
     if (fabs(ry) < 1e-3) ry = 0.0;
     if (fabs(rz) < 1e-3) rz = 0.0;
-    float* dst = out + (long long)c.view * sview + (long long)c.u * su + (long long)c.v * sv;
+    float* dst = out + (long long)w.pad[0] * sview + (long long)c.u * su + (long long)c.v * sv;
 
     double am = -INFINITY, aM = INFINITY;
     bool hit = slab(w.src[0], rx, nx, am, aM);
@@ -137,39 +174,72 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     const Axis az = axis_setup(w.src[2], rz, am);
     const float T = (float)(aM - am);
 
-    // padded coordinates (index + 1) and their brick offsets
-    const unsigned SY = 64u * (unsigned)nbx, SZ = SY * (unsigned)nby;
+    const VolumeReader<kBuffer> vload(vol, elements);
     int px = ax.idx + 1, py = ay.idx + 1, pz = az.idx + 1;
-    unsigned ox = brick_off(px, 64u, 1u), oy = brick_off(py, SY, 4u), oz = brick_off(pz, SZ, 16u);
+    unsigned ox, oy;
+    if (kYMajor) {
+        ox = line_off_b(px, SB);
+        oy = (unsigned)py << 5;
+    } else {
+        ox = (unsigned)px << 5;
+        oy = line_off_b(py, SB);
+    }
+    unsigned oz = line_off_z(pz, SZ);
+    const int sx32 = ax.step * 32, sy32 = ay.step * 32;
 
     float tx = ax.t0, ty = ay.t0, tz = az.t0;
     float kx = 0.0f, ky = 0.0f, kz = 0.0f;
     float tc = 0.0f, sum = 0.0f;
-    // Every iteration consumes one plane crossing; the crossings inside the
-    // volume number at most nx + ny + nz + 3, so the guard always ends the walk.
-    for (int guard = nx + ny + nz + 4; guard > 0; --guard) {
-        const float tm = fminf(tx, fminf(ty, tz));
-        const float tn = fminf(tm, T);
-        sum = fmaf(tn - tc, vol[ox + oy + oz], sum);
-        if (tm >= T) break;
-        tc = tn;
-        if (tx == tm) {
-            kx += 1.0f;
-            tx = fmaf(kx, ax.dt, ax.t0);
-            px += ax.step;
-            ox = brick_off(px, 64u, 1u);
-        } else if (ty == tm) {
-            ky += 1.0f;
-            ty = fmaf(ky, ay.dt, ay.t0);
-            py += ay.step;
-            oy = brick_off(py, SY, 4u);
-        } else {
-            kz += 1.0f;
-            tz = fmaf(kz, az.dt, az.t0);
-            pz += az.step;
-            oz = brick_off(pz, SZ, 16u);
-        }
+    // Every step consumes one plane crossing; the crossings inside the volume
+    // number at most nx + ny + nz + 3, so the guard always ends the walk.
+    // Software pipeline: step k fetches the voxel of segment k + 1 and
+    // accumulates segment k - 1, so every load has two steps to arrive; the
+    // loop is unrolled three times so the three voxel registers rotate.  The
+    // last segment, cut at the exit T, is added when the walk ends.
+#define TIGRE_SIDDON_STEP(PREV, CUR, NEXT)                                 \
+    {                                                                      \
+        const float tm = fminf(tx, fminf(ty, tz));                         \
+        if (tm >= T) {                                                     \
+            sum = fmaf(lp, PREV, sum);                                     \
+            sum = fmaf(T - tc, CUR, sum);                                  \
+            break;                                                         \
+        }                                                                  \
+        if (tx == tm) {                                                    \
+            kx += 1.0f;                                                    \
+            tx = fmaf(kx, ax.dt, ax.t0);                                   \
+            if (kYMajor) {                                                 \
+                px += ax.step;                                             \
+                ox = line_off_b(px, SB);                                   \
+            } else {                                                       \
+                ox += sx32;                                                \
+            }                                                              \
+        } else if (ty == tm) {                                             \
+            ky += 1.0f;                                                    \
+            ty = fmaf(ky, ay.dt, ay.t0);                                   \
+            if (kYMajor) {                                                 \
+                oy += sy32;                                                \
+            } else {                                                       \
+                py += ay.step;                                             \
+                oy = line_off_b(py, SB);                                   \
+            }                                                              \
+        } else {                                                           \
+            kz += 1.0f;                                                    \
+            tz = fmaf(kz, az.dt, az.t0);                                   \
+            pz += az.step;                                                 \
+            oz = line_off_z(pz, SZ);                                       \
+        }                                                                  \
+        NEXT = vload(ox + oy + oz);                                        \
+        sum = fmaf(lp, PREV, sum);                                         \
+        lp = tm - tc;                                                      \
+        tc = tm;                                                           \
     }
+    float r0 = 0.0f, r1 = vload(ox + oy + oz), r2 = 0.0f, lp = 0.0f;
+    for (int guard = (nx + ny + nz + 9) / 3; guard > 0; --guard) {
+        TIGRE_SIDDON_STEP(r0, r1, r2)
+        TIGRE_SIDDON_STEP(r1, r2, r0)
+        TIGRE_SIDDON_STEP(r2, r0, r1)
+    }
+#undef TIGRE_SIDDON_STEP
     // scale by the ray length in mm (:241, :267)
     const double len = sqrt(rx * rx * dvx * dvx + ry * ry * dvy * dvy + rz * rz * dvz * dvz);
     *dst = (float)((double)sum * len);
@@ -180,9 +250,9 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
 // in brick c >> 2, and each brick stores the 5^3 samples 4b .. 4b + 4 of its
 // axis ranges, so the 8 taps of a trilinear sample are one brick base plus the
 // constant offsets {0, 1, 5, 6, 25, 26, 30, 31}.
-template <bool kNearest>
+template <bool kNearest, bool kBuffer>
 __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                        int nbx, int nby, const FwdView* __restrict__ views,
+                                                        int nbx, int nby, int nbz, const FwdView* __restrict__ views,
                                                         int nviews, int nU, int nV, long long su, long long sv,
                                                         long long sview, float* __restrict__ out, double acc,
                                                         double dvx, double dvy, double dvz) {
@@ -193,7 +263,7 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     const double dx = w.org[0] + c.u * w.du[0] + pv * w.dv[0] - w.src[0];
     const double dy = w.org[1] + c.u * w.du[1] + pv * w.dv[1] - w.src[1];
     const double dz = w.org[2] + c.u * w.du[2] + pv * w.dv[2] - w.src[2];
-    float* dst = out + (long long)c.view * sview + (long long)c.u * su + (long long)c.v * sv;
+    float* dst = out + (long long)w.pad[0] * sview + (long long)c.u * su + (long long)c.v * sv;
 
     // samples per ray and step (:164-169)
     const double n = ceil(sqrt(dx * dx + dy * dy + dz * dz) / acc);
@@ -236,9 +306,11 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     const int count = (int)(i1 - i0) + 1;
 
     const unsigned BY = 125u * (unsigned)nbx, BZ = BY * (unsigned)nby;
+    const VolumeReader<kBuffer> vload(vol, (unsigned long long)BZ * (unsigned)nbz);
     // padded coordinates stay in [0, n + 1] so the cell and its +1 taps exist
     const float hx = (float)(nx + 1), hy = (float)(ny + 1), hz = (float)(nz + 1);
     float sum = 0.0f;
+#pragma unroll 2
     for (int m = 0; m < count; ++m) {
         const float fm = (float)m;
         const float x = __builtin_amdgcn_fmed3f(fmaf(fm, fx, qx), 0.0f, hx);
@@ -246,18 +318,21 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
         const float z = __builtin_amdgcn_fmed3f(fmaf(fm, fz, qz), 0.0f, hz);
         if (kNearest) {
             const int ix = (int)floorf(x + 0.5f), iy = (int)floorf(y + 0.5f), iz = (int)floorf(z + 0.5f);
-            const unsigned o = (unsigned)(ix >> 2) * 125u + (unsigned)(iy >> 2) * BY + (unsigned)(iz >> 2) * BZ +
-                               (unsigned)(ix & 3) + (unsigned)(iy & 3) * 5u + (unsigned)(iz & 3) * 25u;
-            sum += vol[o];
+            const unsigned o = __umul24((unsigned)ix >> 2, 125u) + __umul24((unsigned)iy >> 2, BY) +
+                               __umul24((unsigned)iz >> 2, BZ) + ((unsigned)ix & 3u) +
+                               __umul24((unsigned)iy & 3u, 5u) + __umul24((unsigned)iz & 3u, 25u);
+            sum += vload(o);
         } else {
             const float flx = floorf(x), fly = floorf(y), flz = floorf(z);
             const float wx = x - flx, wy = y - fly, wz = z - flz;
             const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-            const unsigned o = (unsigned)(ix >> 2) * 125u + (unsigned)(iy >> 2) * BY + (unsigned)(iz >> 2) * BZ +
-                               (unsigned)(ix & 3) + (unsigned)(iy & 3) * 5u + (unsigned)(iz & 3) * 25u;
-            const float* p = vol + o;
-            const float c000 = p[0], c100 = p[1], c010 = p[5], c110 = p[6];
-            const float c001 = p[25], c101 = p[26], c011 = p[30], c111 = p[31];
+            const unsigned o = __umul24((unsigned)ix >> 2, 125u) + __umul24((unsigned)iy >> 2, BY) +
+                               __umul24((unsigned)iz >> 2, BZ) + ((unsigned)ix & 3u) +
+                               __umul24((unsigned)iy & 3u, 5u) + __umul24((unsigned)iz & 3u, 25u);
+            const float c000 = vload(o), c100 = vload(o + 1), c010 = vload(o + 5),
+                        c110 = vload(o + 6);
+            const float c001 = vload(o + 25), c101 = vload(o + 26), c011 = vload(o + 30),
+                        c111 = vload(o + 31);
             const float c00 = fmaf(wx, c100 - c000, c000), c10 = fmaf(wx, c110 - c010, c010);
             const float c01 = fmaf(wx, c101 - c001, c001), c11 = fmaf(wx, c111 - c011, c011);
             const float c0 = fmaf(wy, c10 - c00, c00), c1 = fmaf(wy, c11 - c01, c01);
@@ -301,6 +376,25 @@ __global__ __launch_bounds__(256) void apron_volume_kernel(const float* __restri
     }
 }
 
+// Line layout (see LineGrid): lines of 32 floats = 8 (in-plane minor axis)
+// x 4 (z) voxels at one major-axis position; lines ordered major axis fastest.
+template <bool kYMajor>
+__global__ __launch_bounds__(256) void line_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
+                                                          int pmajor, int nb8, int nz4, float* __restrict__ lines) {
+    const long long total = (long long)pmajor * nb8 * nz4 * 32;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long line = q >> 5;
+        const int l = (int)(q & 31);
+        const long long pa = line % pmajor, gb = (line / pmajor) % nb8, gz = line / ((long long)pmajor * nb8);
+        const long long pb = gb * 8 + (l & 7), pzz = gz * 4 + (l >> 3);
+        const long long i = (kYMajor ? pb : pa) - 1, j = (kYMajor ? pa : pb) - 1, k = pzz - 1;
+        float v = 0.0f;
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
+        lines[q] = v;
+    }
+}
+
 namespace {
 long long grid_stride_blocks(long long total) {
     long long blocks = (total + 255) / 256;
@@ -327,11 +421,6 @@ int ray_blocks(const HostGeometry& g) {
 }
 }  // namespace
 
-BrickGrid siddon_bricks(const HostGeometry& g) {
-    // padded coordinates 0 .. n + 2
-    return BrickGrid{(g.nVoxel[0] + 2) / 4 + 1, (g.nVoxel[1] + 2) / 4 + 1, (g.nVoxel[2] + 2) / 4 + 1, 64};
-}
-
 BrickGrid interp_bricks(const HostGeometry& g) {
     // cells (lower taps) 0 .. n + 1
     return BrickGrid{(g.nVoxel[0] + 1) / 4 + 1, (g.nVoxel[1] + 1) / 4 + 1, (g.nVoxel[2] + 1) / 4 + 1, 125};
@@ -350,14 +439,44 @@ int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid
     return TIGRE_OK;
 }
 
-int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream) {
+SiddonLayout siddon_layout(const HostGeometry& g, bool ymajor) {
+    SiddonLayout L;
+    const int na = ymajor ? g.nVoxel[1] : g.nVoxel[0], nb = ymajor ? g.nVoxel[0] : g.nVoxel[1];
+    L.ymajor = ymajor;
+    L.pmajor = na + 3;
+    L.nb8 = (nb + 3 + 7) / 8;
+    L.nz4 = (g.nVoxel[2] + 3 + 3) / 4;
+    L.SB = 32ull * (unsigned long long)L.pmajor;
+    L.SZ = L.SB * (unsigned long long)L.nb8;
+    L.elements = L.SZ * (unsigned long long)L.nz4;
+    return L;
+}
+
+int launch_line_volume(const float* vol, const HostGeometry& g, const SiddonLayout& L, float* lines,
+                       hipStream_t stream) {
+    auto kern = L.ymajor ? line_volume_kernel<true> : line_volume_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid_stride_blocks((long long)L.elements)), dim3(256), 0, stream, vol,
+                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], L.pmajor, L.nb8, L.nz4, lines);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_ax_siddon(const float* lines, const HostGeometry& g, const SiddonLayout& L, const FwdView* d_views,
+                     int nviews, int layout, float* proj, hipStream_t stream) {
+    if (nviews == 0) return TIGRE_OK;
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
-    TIGRE_REQUIRE(b.elements() < (1LL << 32), "volume too large for 32-bit brick offsets");
-    hipLaunchKernelGGL(ax_siddon_kernel, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
-                       g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview, proj,
-                       g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+    // 24-bit multiplier operands and 32-bit offsets
+    TIGRE_REQUIRE(L.SZ < (1ull << 24) && L.elements < (1ull << 32), "volume too large for 32-bit line offsets");
+    const bool buf = L.elements * 4 < (1ull << 32);
+    auto kern = L.ymajor ? (buf ? ax_siddon_kernel<true, true> : ax_siddon_kernel<false, true>)
+                         : (buf ? ax_siddon_kernel<true, false> : ax_siddon_kernel<false, false>);
+    const long long tiles = (long long)((g.nDetec[0] + kTileEdge - 1) / kTileEdge) *
+                            ((g.nDetec[1] + kTileEdge - 1) / kTileEdge);
+    const long long blocks = tiles * ((nviews + kViewsPerBlock - 1) / kViewsPerBlock);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, lines, L.elements, g.nVoxel[0],
+                       g.nVoxel[1], g.nVoxel[2], (unsigned)L.SB, (unsigned)L.SZ, d_views, nviews, g.nDetec[0],
+                       g.nDetec[1], su, sv, sview, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }
@@ -366,16 +485,15 @@ int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGri
                      int layout, float* proj, hipStream_t stream) {
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
-    TIGRE_REQUIRE(b.elements() < (1LL << 32), "volume too large for 32-bit brick offsets");
+    TIGRE_REQUIRE(b.elements() < (1LL << 32) && (long long)b.nbx * b.nby * 125 < (1LL << 24),
+                  "volume too large for 32-bit brick offsets");
     // point filtering when accuracy > 1 (ray_interpolated_projection.cu:589-595)
-    if (g.accuracy > 1.0)
-        hipLaunchKernelGGL(ax_interp_kernel<true>, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0],
-                           g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su,
-                           sv, sview, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
-    else
-        hipLaunchKernelGGL(ax_interp_kernel<false>, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0],
-                           g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su,
-                           sv, sview, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+    const bool buf = b.elements() * 4 < (1LL << 32);
+    auto kern = (g.accuracy > 1.0) ? (buf ? ax_interp_kernel<true, true> : ax_interp_kernel<true, false>)
+                                   : (buf ? ax_interp_kernel<false, true> : ax_interp_kernel<false, false>);
+    hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
+                       g.nVoxel[2], b.nbx, b.nby, b.nbz, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview,
+                       proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

@@ -62,7 +62,7 @@ struct FwdView {
     double dv[3];      // step per row counted from the top (pv = nV - 1 - v)
     double first;      // interpolated: first sample index floor(crop / accuracy)
     double cap;        // interpolated: (2 DSO / dmin + crop) / accuracy
-    double pad[2];
+    double pad[2];     // pad[0]: index of this view's projection in the output
 };
 static_assert(sizeof(FwdView) == 16 * sizeof(double), "FwdView layout");
 
@@ -78,14 +78,14 @@ struct BwdView {
     float uoff;        // nDetecU/2 - 0.5  (texture coordinate -> pixel index)
     float voff;        // nDetecV/2 - 0.5
     // FDK weight (voxel_backprojection.cu:243-249): W = 1/((DSO + ry sin - rx cos)/DSO)^2
-    float sina, cosa, DSO, DSD;
+    float sina, cosa, DSO, DSD, invDSO;
     float realx0, realy0;   // rx = realx0 + ix*dVx, ry = realy0 + iy*dVy
     // matched weight (voxel_backprojection2.cu:201-279)
     float rvx0, rvy0, rvz0; // real position of voxel (0,0,0)
     float Sx, Sy;           // real source position
     float dax;              // -(DSD - DSO)
     float day0, dz0;        // detector real-position offsets
-    float pad[8];
+    float pad[7];
 };
 static_assert(sizeof(BwdView) == 40 * sizeof(float), "BwdView layout");
 
@@ -127,19 +127,28 @@ struct BrickGrid {
     int per_brick;       // 64 (4^3 bricks) or 125 (5^3 apron bricks)
     long long elements() const { return (long long)nbx * nby * nbz * per_brick; }
 };
-BrickGrid siddon_bricks(const HostGeometry& g);
 BrickGrid interp_bricks(const HostGeometry& g);
+
+// Line layouts of a volume for the Siddon projector (x-major or y-major).
+struct SiddonLayout {
+    bool ymajor;
+    int pmajor, nb8, nz4;
+    unsigned long long SB, SZ, elements;
+};
+SiddonLayout siddon_layout(const HostGeometry& g, bool ymajor);
+int launch_line_volume(const float* vol, const HostGeometry& g, const SiddonLayout& L, float* lines,
+                       hipStream_t stream);
 
 // Launchers (one per operator; all enqueue on `stream`).
 int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream);
-int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream);
+int launch_ax_siddon(const float* lines, const HostGeometry& g, const SiddonLayout& L, const FwdView* d_views,
+                     int nviews, int layout, float* proj, hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int layout, float* proj, hipStream_t stream);
-int launch_pad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* padded,
-                           hipStream_t stream);
-int launch_atb(const float* padded_proj, const HostGeometry& g, const BwdView* d_views, bool matched,
+int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,
+                            hipStream_t stream);
+int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched,
                bool axial, float* vol, hipStream_t stream);
 int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
                 hipStream_t stream);
@@ -156,7 +165,7 @@ int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int bac
 // a clamped coordinate always addresses the zero-padded border or the data.
 __device__ inline float clampf(float x, float hi) { return fminf(fmaxf(x, -1.0f), hi); }
 
-// Padded projection dimensions: one zero pixel before, two after on both axes.
-__host__ __device__ inline long padded_det(int n) { return (long)n + 3; }
+// Quad projection dimensions: tap origins -1 .. n on each detector axis.
+__host__ __device__ inline long quad_det(int n) { return (long)n + 2; }
 
 }  // namespace tigre

@@ -184,7 +184,8 @@ void forward_views(const HostGeometry& g, bool centre, std::vector<FwdView>* out
                                    : std::fmin(std::fmin(g.dVoxel[0], g.dVoxel[1]), g.dVoxel[2]);
         r.first = std::floor(crop / g.accuracy);
         r.cap = (2.0 * DSO / dmin + crop) / g.accuracy;
-        r.pad[0] = r.pad[1] = 0.0;
+        r.pad[0] = (double)i;
+        r.pad[1] = 0.0;
     }
 }
 
@@ -241,6 +242,7 @@ void backward_views(const HostGeometry& g, std::vector<BwdView>* out) {
         r.cosa = (float)ca;
         r.DSO = (float)DSO;
         r.DSD = (float)DSD;
+        r.invDSO = (float)(1.0 / DSO);
         r.realx0 = (float)(-(g.sVoxel[0] - g.dVoxel[0]) * 0.5 + off[0]);
         r.realy0 = (float)(-(g.sVoxel[1] - g.dVoxel[1]) * 0.5 + off[1] + COR);
         r.rvx0 = (float)(-g.sVoxel[0] / 2 + g.dVoxel[0] / 2 + off[0]);
