@@ -10,11 +10,8 @@
 // The volume is read from re-laid-out, zero-bordered copies (one voxel below,
 // two above on every axis), which replace the reference's border-addressed
 // texture: taps that fall outside the volume read zeros without bounds checks.
-//  - Siddon: "line" layouts.  A 128-byte cache line holds an 8 x 4 patch of
-//    voxels across the ray direction, so the 64 rays of a wavefront touch a
-//    handful of lines per step.  Views whose rays run mostly along x read the
-//    x-major copy (patch in y-z), the others the y-major copy (patch in x-z);
-//    the per-step offset along the major axis is then a single add.
+//  - Siddon: 4^3 bricks (256 bytes), so the nearly parallel rays of a
+//    wavefront share bricks whatever their direction.
 //  - interpolated: 5^3 "apron" bricks, so the 8 trilinear taps share one base.
 #include <cmath>
 
@@ -128,23 +125,13 @@ struct VolumeReader {
 
 }  // namespace
 
-// Line-layout offsets.  Padded coordinate p (= voxel index + 1) along the
-// major axis: p * 32; along the other in-plane axis: (p >> 3) * SB + (p & 7);
-// along z: (p >> 2) * SZ + (p & 3) * 8.
-struct LineGrid {
-    unsigned SB, SZ;
-};
-__device__ inline unsigned line_off_b(int p, unsigned SB) { return __umul24((unsigned)p >> 3, SB) + ((unsigned)p & 7u); }
-__device__ inline unsigned line_off_z(int p, unsigned SZ) {
-    return __umul24((unsigned)p >> 2, SZ) + (((unsigned)p & 3u) << 3);
-}
 
 // Siddon / Jacobs exact radiological path, Siddon_projection.cu:110-268.
-// vol: line-layout copy (x-major when !kYMajor).  views: the views of this
-// launch; pad[0] of each record holds its projection index in `out`.
-template <bool kBuffer, bool kYMajor>
+// vol: 4^3-bricked copy (see brick_volume_kernel); pad[0] of each view record
+// holds its projection index in `out`.
+template <bool kBuffer>
 __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict__ vol, unsigned long long elements,
-                                                        int nx, int ny, int nz, unsigned SB, unsigned SZ,
+                                                        int nx, int ny, int nz, int nbx, int nby,
                                                         const FwdView* __restrict__ views, int nviews, int nU, int nV,
                                                         long long su, long long sv, long long sview,
                                                         float* __restrict__ out, double dvx, double dvy, double dvz) {
@@ -174,18 +161,11 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     const Axis az = axis_setup(w.src[2], rz, am);
     const float T = (float)(aM - am);
 
+    // padded coordinates (index + 1) and their brick offsets
+    const unsigned SY = 64u * (unsigned)nbx, SZ = SY * (unsigned)nby;
     const VolumeReader<kBuffer> vload(vol, elements);
     int px = ax.idx + 1, py = ay.idx + 1, pz = az.idx + 1;
-    unsigned ox, oy;
-    if (kYMajor) {
-        ox = line_off_b(px, SB);
-        oy = (unsigned)py << 5;
-    } else {
-        ox = (unsigned)px << 5;
-        oy = line_off_b(py, SB);
-    }
-    unsigned oz = line_off_z(pz, SZ);
-    const int sx32 = ax.step * 32, sy32 = ay.step * 32;
+    unsigned ox = brick_off(px, 64u, 1u), oy = brick_off(py, SY, 4u), oz = brick_off(pz, SZ, 16u);
 
     float tx = ax.t0, ty = ay.t0, tz = az.t0;
     float kx = 0.0f, ky = 0.0f, kz = 0.0f;
@@ -207,26 +187,18 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
         if (tx == tm) {                                                    \
             kx += 1.0f;                                                    \
             tx = fmaf(kx, ax.dt, ax.t0);                                   \
-            if (kYMajor) {                                                 \
-                px += ax.step;                                             \
-                ox = line_off_b(px, SB);                                   \
-            } else {                                                       \
-                ox += sx32;                                                \
-            }                                                              \
+            px += ax.step;                                                 \
+            ox = brick_off(px, 64u, 1u);                                   \
         } else if (ty == tm) {                                             \
             ky += 1.0f;                                                    \
             ty = fmaf(ky, ay.dt, ay.t0);                                   \
-            if (kYMajor) {                                                 \
-                oy += sy32;                                                \
-            } else {                                                       \
-                py += ay.step;                                             \
-                oy = line_off_b(py, SB);                                   \
-            }                                                              \
+            py += ay.step;                                                 \
+            oy = brick_off(py, SY, 4u);                                    \
         } else {                                                           \
             kz += 1.0f;                                                    \
             tz = fmaf(kz, az.dt, az.t0);                                   \
             pz += az.step;                                                 \
-            oz = line_off_z(pz, SZ);                                       \
+            oz = brick_off(pz, SZ, 16u);                                   \
         }                                                                  \
         NEXT = vload(ox + oy + oz);                                        \
         sum = fmaf(lp, PREV, sum);                                         \
@@ -376,25 +348,6 @@ __global__ __launch_bounds__(256) void apron_volume_kernel(const float* __restri
     }
 }
 
-// Line layout (see LineGrid): lines of 32 floats = 8 (in-plane minor axis)
-// x 4 (z) voxels at one major-axis position; lines ordered major axis fastest.
-template <bool kYMajor>
-__global__ __launch_bounds__(256) void line_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                          int pmajor, int nb8, int nz4, float* __restrict__ lines) {
-    const long long total = (long long)pmajor * nb8 * nz4 * 32;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (long long)gridDim.x * blockDim.x) {
-        const long long line = q >> 5;
-        const int l = (int)(q & 31);
-        const long long pa = line % pmajor, gb = (line / pmajor) % nb8, gz = line / ((long long)pmajor * nb8);
-        const long long pb = gb * 8 + (l & 7), pzz = gz * 4 + (l >> 3);
-        const long long i = (kYMajor ? pb : pa) - 1, j = (kYMajor ? pa : pb) - 1, k = pzz - 1;
-        float v = 0.0f;
-        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
-        lines[q] = v;
-    }
-}
-
 namespace {
 long long grid_stride_blocks(long long total) {
     long long blocks = (total + 255) / 256;
@@ -439,43 +392,22 @@ int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid
     return TIGRE_OK;
 }
 
-SiddonLayout siddon_layout(const HostGeometry& g, bool ymajor) {
-    SiddonLayout L;
-    const int na = ymajor ? g.nVoxel[1] : g.nVoxel[0], nb = ymajor ? g.nVoxel[0] : g.nVoxel[1];
-    L.ymajor = ymajor;
-    L.pmajor = na + 3;
-    L.nb8 = (nb + 3 + 7) / 8;
-    L.nz4 = (g.nVoxel[2] + 3 + 3) / 4;
-    L.SB = 32ull * (unsigned long long)L.pmajor;
-    L.SZ = L.SB * (unsigned long long)L.nb8;
-    L.elements = L.SZ * (unsigned long long)L.nz4;
-    return L;
+BrickGrid siddon_bricks(const HostGeometry& g) {
+    // padded coordinates 0 .. n + 2
+    return BrickGrid{(g.nVoxel[0] + 2) / 4 + 1, (g.nVoxel[1] + 2) / 4 + 1, (g.nVoxel[2] + 2) / 4 + 1, 64};
 }
 
-int launch_line_volume(const float* vol, const HostGeometry& g, const SiddonLayout& L, float* lines,
-                       hipStream_t stream) {
-    auto kern = L.ymajor ? line_volume_kernel<true> : line_volume_kernel<false>;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid_stride_blocks((long long)L.elements)), dim3(256), 0, stream, vol,
-                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], L.pmajor, L.nb8, L.nz4, lines);
-    TIGRE_HIP_TRY(hipGetLastError());
-    return TIGRE_OK;
-}
-
-int launch_ax_siddon(const float* lines, const HostGeometry& g, const SiddonLayout& L, const FwdView* d_views,
-                     int nviews, int layout, float* proj, hipStream_t stream) {
-    if (nviews == 0) return TIGRE_OK;
+int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream) {
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
     // 24-bit multiplier operands and 32-bit offsets
-    TIGRE_REQUIRE(L.SZ < (1ull << 24) && L.elements < (1ull << 32), "volume too large for 32-bit line offsets");
-    const bool buf = L.elements * 4 < (1ull << 32);
-    auto kern = L.ymajor ? (buf ? ax_siddon_kernel<true, true> : ax_siddon_kernel<false, true>)
-                         : (buf ? ax_siddon_kernel<true, false> : ax_siddon_kernel<false, false>);
-    const long long tiles = (long long)((g.nDetec[0] + kTileEdge - 1) / kTileEdge) *
-                            ((g.nDetec[1] + kTileEdge - 1) / kTileEdge);
-    const long long blocks = tiles * ((nviews + kViewsPerBlock - 1) / kViewsPerBlock);
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, lines, L.elements, g.nVoxel[0],
-                       g.nVoxel[1], g.nVoxel[2], (unsigned)L.SB, (unsigned)L.SZ, d_views, nviews, g.nDetec[0],
+    TIGRE_REQUIRE(b.elements() < (1LL << 32) && (long long)b.nbx * b.nby * 64 < (1LL << 24),
+                  "volume too large for 32-bit brick offsets");
+    const bool buf = b.elements() * 4 < (1LL << 32);
+    auto kern = buf ? ax_siddon_kernel<true> : ax_siddon_kernel<false>;
+    hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, (unsigned long long)b.elements(),
+                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0],
                        g.nDetec[1], su, sv, sview, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;

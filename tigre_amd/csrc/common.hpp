@@ -124,26 +124,17 @@ int upload_async(void** d_ptr, const void* host, size_t bytes, hipStream_t strea
 // Bricked copies of a volume for the forward projectors.
 struct BrickGrid {
     int nbx, nby, nbz;   // bricks per axis
-    int per_brick;       // 64 (4^3 bricks) or 125 (5^3 apron bricks)
+    int per_brick;       // floats per brick: 64 (4^3 bricks) or 125 (5^3 apron bricks)
     long long elements() const { return (long long)nbx * nby * nbz * per_brick; }
 };
+BrickGrid siddon_bricks(const HostGeometry& g);
 BrickGrid interp_bricks(const HostGeometry& g);
-
-// Line layouts of a volume for the Siddon projector (x-major or y-major).
-struct SiddonLayout {
-    bool ymajor;
-    int pmajor, nb8, nz4;
-    unsigned long long SB, SZ, elements;
-};
-SiddonLayout siddon_layout(const HostGeometry& g, bool ymajor);
-int launch_line_volume(const float* vol, const HostGeometry& g, const SiddonLayout& L, float* lines,
-                       hipStream_t stream);
 
 // Launchers (one per operator; all enqueue on `stream`).
 int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream);
-int launch_ax_siddon(const float* lines, const HostGeometry& g, const SiddonLayout& L, const FwdView* d_views,
-                     int nviews, int layout, float* proj, hipStream_t stream);
+int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
+                     int layout, float* proj, hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int layout, float* proj, hipStream_t stream);
 int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,

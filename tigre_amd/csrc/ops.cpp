@@ -1,8 +1,6 @@
 // Operator drivers on device-resident data: build the per-view records,
 // upload them, stage the padded operand and launch the kernel, all on the
 // caller's stream with stream-ordered scratch allocations.
-#include <cmath>
-
 #include "common.hpp"
 
 namespace tigre {
@@ -15,44 +13,17 @@ int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int proj
     const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
     std::vector<FwdView> views;
     forward_views(g, interp, &views);
-    if (interp) {
-        void* d_views = nullptr;
-        TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
-        const BrickGrid b = interp_bricks(g);
-        float* bricked = nullptr;
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
-        TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
-        TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
-        return TIGRE_OK;
-    }
-    // Siddon: views whose central ray runs mostly along y read the y-major
-    // copy of the volume, the others the x-major copy.
-    std::vector<FwdView> order;
-    order.reserve(views.size());
-    int n_x = 0;
-    for (int pass = 0; pass < 2; ++pass)
-        for (const FwdView& w : views) {
-            const double cx = w.org[0] + 0.5 * (g.nDetec[0] - 1) * w.du[0] + 0.5 * (g.nDetec[1] - 1) * w.dv[0];
-            const double cy = w.org[1] + 0.5 * (g.nDetec[0] - 1) * w.du[1] + 0.5 * (g.nDetec[1] - 1) * w.dv[1];
-            const bool ymaj = std::fabs(cy - w.src[1]) > std::fabs(cx - w.src[0]);
-            if ((pass == 0) == !ymaj) order.push_back(w);
-            if (pass == 0 && !ymaj) ++n_x;
-        }
     void* d_views = nullptr;
-    TIGRE_CHECK(upload_async(&d_views, order.data(), order.size() * sizeof(FwdView), stream));
-    const FwdView* dv = (const FwdView*)d_views;
-    const int counts[2] = {n_x, g.nangles - n_x};
-    for (int ymaj = 0; ymaj < 2; ++ymaj) {
-        if (counts[ymaj] == 0) continue;
-        const SiddonLayout L = siddon_layout(g, ymaj == 1);
-        float* lines = nullptr;
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&lines, (size_t)L.elements * sizeof(float), stream));
-        TIGRE_CHECK(launch_line_volume(d_img, g, L, lines, stream));
-        TIGRE_CHECK(launch_ax_siddon(lines, g, L, dv + (ymaj ? n_x : 0), counts[ymaj], layout, d_proj, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(lines, stream));
-    }
+    TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
+    const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
+    float* bricked = nullptr;
+    TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
+    TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
+    if (interp)
+        TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
+    else
+        TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
+    TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
     return TIGRE_OK;
 }
