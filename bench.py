@@ -25,6 +25,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+# (scaled by a copy-kernel calibration); parsed from this summary when present.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_512cube_360views.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 2 and cells[0] == kernel:
+                    return float(cells[-1])
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def log(*a):
@@ -38,8 +54,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512, help="volume edge / detector edge")
     ap.add_argument("--views", type=int, default=360)
-    ap.add_argument("--cpu-views", type=int, default=8, help="views in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-views", type=int, default=64,
+                    help="views in the CPU-baseline sample, indices floor(k * views / cpu_views) (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the timed region, copy a buffer of this size once (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -49,7 +68,10 @@ def main():
     import numpy as np
     import torch
 
+    import ctypes
+
     import tigre_amd
+    from tigre_amd import _native
     from tigre_amd.geometry import cone_geometry, prepared
     from tigre_amd.phantom import ellipsoid_phantom_torch
 
@@ -77,6 +99,18 @@ def main():
         atb()
     torch.cuda.synchronize()
 
+    lib = _native.lib()
+
+    def kernel_ms(kernel_id):
+        """Average duration of the launches of one kernel recorded since the last call."""
+        total, count = ctypes.c_double(0.0), ctypes.c_int(0)
+        _native.check(lib.tigre_kernel_time(kernel_id, ctypes.byref(total), ctypes.byref(count)), "kernel_time")
+        return total.value / count.value if count.value else float("nan")
+
+    for k in range(4):
+        kernel_ms(k)
+    lib.tigre_kernel_timing(1)
+
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -90,10 +124,13 @@ def main():
     wall = time.perf_counter() - t0
     ax_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
     atb_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    k_siddon = kernel_ms(0)
+    k_fdk = kernel_ms(2)
     ms_per_step = wall * 1e3 / args.steps
     updates = 2.0 * V * nviews
     value = updates * args.steps / wall / 1e9
-    log("Ax Siddon %.2f ms  Atb FDK %.2f ms  step %.2f ms" % (ax_ms, atb_ms, ms_per_step))
+    log("Ax Siddon %.2f ms (kernel %.2f)  Atb FDK %.2f ms (kernel %.2f)  step %.2f ms" % (
+        ax_ms, k_siddon, atb_ms, k_fdk, ms_per_step))
 
     # the other two operators of the pair, timed the same way
     def timed(fn, reps=3):
@@ -107,19 +144,36 @@ def main():
         return s.elapsed_time(e) / reps
 
     if args.no_extra:
-        interp_ms = matched_ms = float("nan")
+        interp_ms = matched_ms = k_interp = k_matched = float("nan")
     else:
         interp_ms = timed(lambda: tigre_amd.Ax(img, geo, angles, "interpolated"))
         matched_ms = timed(lambda: tigre_amd.Atb(proj, geo, angles, "matched"))
-    log("Ax interpolated %.2f ms  Atb matched %.2f ms" % (interp_ms, matched_ms))
+        k_interp = kernel_ms(1)
+        k_matched = kernel_ms(3)
+    lib.tigre_kernel_timing(0)
+    kernel_ms(4)
+    log("Ax interpolated %.2f ms (kernel %.2f)  Atb matched %.2f ms (kernel %.2f)" % (
+        interp_ms, k_interp, matched_ms, k_matched))
+
+    if args.calib_copy_mib > 0:
+        # one streaming elementwise kernel of known size (reads and writes calib_copy_mib MiB each),
+        # for scaling the FETCH_SIZE / WRITE_SIZE counters
+        n_el = args.calib_copy_mib * 262144
+        a = torch.ones(n_el, device=dev)
+        b = torch.empty_like(a)
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB" % args.calib_copy_mib)
+        del a, b
 
     b_ax = nviews * (V + P) * 4.0
     b_atb = nviews * (2 * V + P) * 4.0
-    if atb_ms >= ax_ms:
-        dom, dom_ms, dom_b = "atb_fdk", atb_ms, b_atb
+    if k_fdk >= k_siddon:
+        dom, dom_ms, dom_b = "atb_fdk", k_fdk, b_atb
     else:
-        dom, dom_ms, dom_b = "ax_siddon", ax_ms, b_ax
+        dom, dom_ms, dom_b = "ax_siddon", k_siddon, b_ax
     achieved = dom_b / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom)
 
     result = {
         "metric": "Ax+Atb Gupdates/s (Siddon + FDK, cone beam)",
@@ -138,10 +192,12 @@ def main():
             n, nviews, n, n), "nVoxel": [n, n, n], "nViews": nviews, "nDetector": [n, n],
             "DSO_mm": 1000.0, "DSD_mm": 1536.0},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "model_bytes_per_launch": dom_b},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "model_bytes_per_launch": dom_b, "kernel_ms": round(dom_ms, 3)},
         "ops_ms": {"ax_siddon": round(ax_ms, 3), "atb_fdk": round(atb_ms, 3), "ax_interpolated": round(interp_ms, 3),
                    "atb_matched": round(matched_ms, 3)},
+        "kernel_ms": {"ax_siddon": round(k_siddon, 3), "atb_fdk": round(k_fdk, 3),
+                      "ax_interpolated": round(k_interp, 3), "atb_matched": round(k_matched, 3)},
         "pair_ms_model_60pct": round((b_ax + b_atb) / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 2),
     }
 
@@ -150,7 +206,7 @@ def main():
 
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
         oracle.set_threads(threads)
-        idx = np.linspace(0, nviews, args.cpu_views, endpoint=False).astype(int)
+        idx = np.array([k * nviews // args.cpu_views for k in range(args.cpu_views)])
         sub = angles[idx]
         gp = prepared(geo, sub)
         img_h = img.cpu().numpy()

@@ -134,6 +134,22 @@ int tigre_tvdenoising_device(const float* d_src, float* d_dst, float lambda, con
 int tigre_get_gpu_count(void);
 int tigre_get_gpu_name(int device, char* name, int len);
 
+/* Kernel timing.  No counterpart in the reference: a measurement hook for
+ * benchmarks.  While enabled, every launch of the kernels below is bracketed
+ * by HIP events on the stream it is launched on; tigre_kernel_time waits for
+ * the recorded launches of one kernel, returns the summed duration and the
+ * launch count, and forgets them. */
+#define TIGRE_KERNEL_AX_SIDDON       0
+#define TIGRE_KERNEL_AX_INTERPOLATED 1
+#define TIGRE_KERNEL_ATB_FDK         2
+#define TIGRE_KERNEL_ATB_MATCHED     3
+#define TIGRE_KERNEL_RELAYOUT        4   /* volume bricking / projection quads */
+#define TIGRE_KERNEL_POCS_TV         5   /* both passes of one TV iteration */
+#define TIGRE_KERNEL_TVDENOISE       6
+#define TIGRE_KERNEL_COUNT           7
+int tigre_kernel_timing(int enable);
+int tigre_kernel_time(int kernel, double* total_ms, int* launches);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* tigre_last_error(void);
 

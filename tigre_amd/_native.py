@@ -72,6 +72,9 @@ SIGNATURES = {
                                                 ctypes.c_void_p]),
     "tigre_get_gpu_count": (ctypes.c_int, []),
     "tigre_get_gpu_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "tigre_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
+    "tigre_kernel_time": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int)]),
     "tigre_last_error": (ctypes.c_char_p, []),
     "tigre_version": (ctypes.c_char_p, []),
 }

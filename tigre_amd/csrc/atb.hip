@@ -171,6 +171,7 @@ int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int l
     const long long total = quad_det(nU) * quad_det(nV) * (long long)nviews;
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
     hipLaunchKernelGGL(quad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, nU, nV, nviews,
                        layout, reinterpret_cast<float4*>(quads));
     TIGRE_HIP_TRY(hipGetLastError());
@@ -186,6 +187,7 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
     const float scale = matched ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
     const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
                         (float)g.dDetec[1], scale, 0.0f};
+    KernelTimerScope timer(matched ? TIGRE_KERNEL_ATB_MATCHED : TIGRE_KERNEL_ATB_FDK, stream);
 #define TIGRE_ATB_LAUNCH(M, A)                                                                                   \
     hipLaunchKernelGGL((atb_kernel<M, A>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],        \
                        g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5])

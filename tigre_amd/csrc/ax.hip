@@ -382,6 +382,7 @@ BrickGrid interp_bricks(const HostGeometry& g) {
 int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream) {
     const long long total = b.elements();
+    KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
     if (b.per_brick == 64)
         hipLaunchKernelGGL(brick_volume_kernel, dim3((unsigned)grid_stride_blocks(total)), dim3(256), 0, stream, vol,
                            g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
@@ -406,6 +407,7 @@ int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGri
                   "volume too large for 32-bit brick offsets");
     const bool buf = b.elements() * 4 < (1LL << 32);
     auto kern = buf ? ax_siddon_kernel<true> : ax_siddon_kernel<false>;
+    KernelTimerScope timer(TIGRE_KERNEL_AX_SIDDON, stream);
     hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, (unsigned long long)b.elements(),
                        g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0],
                        g.nDetec[1], su, sv, sview, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
@@ -423,6 +425,7 @@ int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGri
     const bool buf = b.elements() * 4 < (1LL << 32);
     auto kern = (g.accuracy > 1.0) ? (buf ? ax_interp_kernel<true, true> : ax_interp_kernel<true, false>)
                                    : (buf ? ax_interp_kernel<false, true> : ax_interp_kernel<false, false>);
+    KernelTimerScope timer(TIGRE_KERNEL_AX_INTERPOLATED, stream);
     hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
                        g.nVoxel[2], b.nbx, b.nby, b.nbz, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview,
                        proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);

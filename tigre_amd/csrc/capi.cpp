@@ -178,6 +178,12 @@ int tigre_get_gpu_name(int device, char* name, int len) {
     return TIGRE_OK;
 }
 
+int tigre_kernel_timing(int enable) { return kernel_timing_enable(enable); }
+
+int tigre_kernel_time(int kernel, double* total_ms, int* launches) {
+    return kernel_time_collect(kernel, total_ms, launches);
+}
+
 const char* tigre_last_error(void) { return last_error(); }
 
 const char* tigre_version(void) { return "tigre_amd 0.1.0 (gfx950)"; }

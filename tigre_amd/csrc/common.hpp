@@ -121,6 +121,24 @@ bool backward_is_axial(const HostGeometry& g, const std::vector<BwdView>& v);
 int select_device(const int* gpuids, int ngpus);
 int upload_async(void** d_ptr, const void* host, size_t bytes, hipStream_t stream);
 
+// Optional HIP-event timing of kernel launches (TIGRE_KERNEL_* ids).  The
+// scope records a start event on construction and a stop event on
+// destruction; both are no-ops while timing is disabled.
+class KernelTimerScope {
+  public:
+    KernelTimerScope(int kernel, hipStream_t stream);
+    ~KernelTimerScope();
+    KernelTimerScope(const KernelTimerScope&) = delete;
+    KernelTimerScope& operator=(const KernelTimerScope&) = delete;
+
+  private:
+    int kernel_;
+    hipStream_t stream_;
+    hipEvent_t start_ = nullptr;
+};
+int kernel_timing_enable(int on);
+int kernel_time_collect(int kernel, double* total_ms, int* launches);
+
 // Bricked copies of a volume for the forward projectors.
 struct BrickGrid {
     int nbx, nby, nbz;   // bricks per axis

@@ -44,7 +44,76 @@ int current_state(DeviceState** st) {
     *st = &s;
     return TIGRE_OK;
 }
+
+struct TimedLaunch {
+    hipEvent_t start, stop;
+};
+std::mutex g_timer_mu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_timed[TIGRE_KERNEL_COUNT];
 }  // namespace
+
+KernelTimerScope::KernelTimerScope(int kernel, hipStream_t stream) : kernel_(kernel), stream_(stream) {
+    {
+        std::lock_guard<std::mutex> lock(g_timer_mu);
+        if (!g_timing || kernel < 0 || kernel >= TIGRE_KERNEL_COUNT) return;
+    }
+    if (hipEventCreate(&start_) != hipSuccess) {
+        start_ = nullptr;
+        return;
+    }
+    if (hipEventRecord(start_, stream_) != hipSuccess) {
+        (void)hipEventDestroy(start_);
+        start_ = nullptr;
+    }
+}
+
+KernelTimerScope::~KernelTimerScope() {
+    if (start_ == nullptr) return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream_) != hipSuccess) {
+        if (stop) (void)hipEventDestroy(stop);
+        (void)hipEventDestroy(start_);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_timer_mu);
+    g_timed[kernel_].push_back({start_, stop});
+}
+
+int kernel_timing_enable(int on) {
+    std::lock_guard<std::mutex> lock(g_timer_mu);
+    g_timing = on != 0;
+    return TIGRE_OK;
+}
+
+int kernel_time_collect(int kernel, double* total_ms, int* launches) {
+    TIGRE_REQUIRE(kernel >= 0 && kernel < TIGRE_KERNEL_COUNT, "unknown kernel id");
+    TIGRE_REQUIRE(total_ms != nullptr && launches != nullptr, "null output pointer");
+    std::vector<TimedLaunch> recorded;
+    {
+        std::lock_guard<std::mutex> lock(g_timer_mu);
+        recorded.swap(g_timed[kernel]);
+    }
+    double sum = 0.0;
+    int status = TIGRE_OK;
+    for (const TimedLaunch& t : recorded) {
+        float ms = 0.0f;
+        if (status == TIGRE_OK) {
+            hipError_t e = hipEventSynchronize(t.stop);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t.start, t.stop);
+            if (e != hipSuccess) {
+                set_error(std::string("kernel timing: ") + hipGetErrorString(e));
+                status = TIGRE_ERR_HIP;
+            }
+        }
+        sum += ms;
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    *total_ms = sum;
+    *launches = (int)recorded.size();
+    return status;
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }

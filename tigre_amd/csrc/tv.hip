@@ -113,6 +113,7 @@ int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, lon
     int cur = 0;
     for (int it = 0; it < maxiter; ++it) {
         float* o = (it == maxiter - 1 && dst != img) ? dst : buf[cur];
+        KernelTimerScope timer(TIGRE_KERNEL_POCS_TV, stream);
         hipLaunchKernelGGL(tv_norm_kernel, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, nx, ny, nz, partial);
         hipLaunchKernelGGL(tv_step_kernel, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, o, nx, ny, nz,
                            partial, kTvBlocks, alpha);
@@ -201,6 +202,7 @@ int run_tvdenoise(const float* src, float* dst, float lambda, const float* spaci
         // step sizes of tv_proximal.cu:478-479, evaluated in float as there
         const float tau2 = 0.3f + 0.02f * (float)it;
         const float tau1 = (1.f / tau2) * ((1.f / 6.f) - (5.f / (15.f + (float)it)));
+        KernelTimerScope timer(TIGRE_KERNEL_TVDENOISE, stream);
         hipLaunchKernelGGL(tvd_update_u_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, f, pz, py, px, u, tau1,
                            1.0f / lambda, D, ny, nx, dz, dy, dx);
         hipLaunchKernelGGL(tvd_update_p_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, u, pz, py, px, tau2, D,
