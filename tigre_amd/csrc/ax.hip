@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "ray.hpp"
 
 namespace tigre {
 
@@ -31,15 +32,6 @@ struct RayCoords {
     int u, v, view;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap the hardware
-// id so that each XCD runs a contiguous range of logical blocks (neighbouring
-// tiles of the same views share that XCD's L2).
-__device__ inline int xcd_logical_block(int b, int nblk) {
-    const int q = nblk / 8, r = nblk % 8;
-    const int xcd = b % 8, local = b / 8;
-    return xcd * q + min(xcd, r) + local;
-}
-
 __device__ inline RayCoords ray_of_lane(int nU, int nV, int nviews) {
     const int tilesU = (nU + kTileEdge - 1) / kTileEdge;
     const int tilesV = (nV + kTileEdge - 1) / kTileEdge;
@@ -53,51 +45,6 @@ __device__ inline RayCoords ray_of_lane(int nU, int nV, int nviews) {
     (void)nV;
     (void)nviews;
     return c;
-}
-
-// Slab [0, n] limits of the parametric ray s + a r (IEEE semantics of
-// Siddon_projection.cu:168-176: a ray parallel to the slab and strictly inside
-// is unconstrained, otherwise it misses).
-__device__ inline bool slab(double s, double r, int n, double& am, double& aM) {
-    if (r == 0.0) return s > 0.0 && s < (double)n;
-    const double a0 = (0.0 - s) / r, a1 = ((double)n - s) / r;
-    am = fmax(am, fmin(a0, a1));
-    aM = fmin(aM, fmax(a0, a1));
-    return true;
-}
-
-struct Axis {
-    float t0, dt;    // first crossing (relative to the entry) and crossing spacing
-    int idx;         // voxel index before the first crossing
-    int step;        // +1 / -1 (0 if the ray is parallel to the axis)
-};
-
-// First plane crossed after the entry a = am and the voxel index before it.
-// The index is derived from that plane, so index and crossing list can never
-// disagree (the reference derives it from a midpoint instead).
-__device__ inline Axis axis_setup(double s, double r, double am) {
-    Axis a;
-    if (r == 0.0) {
-        a.t0 = INFINITY;
-        a.dt = INFINITY;
-        a.idx = (int)floor(s);
-        a.step = 0;
-        return a;
-    }
-    const double p = s + am * r;
-    double plane;
-    if (r > 0.0) {
-        plane = floor(p) + 1.0;
-        a.idx = (int)plane - 1;
-        a.step = 1;
-    } else {
-        plane = ceil(p) - 1.0;
-        a.idx = (int)plane;
-        a.step = -1;
-    }
-    a.t0 = (float)((plane - s) / r - am);
-    a.dt = (float)(1.0 / fabs(r));
-    return a;
 }
 
 // Offset of padded coordinate p along one axis of the 4^3-bricked volume
@@ -366,10 +313,10 @@ void out_strides(const HostGeometry& g, int layout, long long* su, long long* sv
     }
 }
 
-int ray_blocks(const HostGeometry& g) {
+int ray_blocks(const HostGeometry& g, int nviews) {
     const long long tiles = (long long)((g.nDetec[0] + kTileEdge - 1) / kTileEdge) *
                             ((g.nDetec[1] + kTileEdge - 1) / kTileEdge);
-    const long long groups = (g.nangles + kViewsPerBlock - 1) / kViewsPerBlock;
+    const long long groups = (nviews + kViewsPerBlock - 1) / kViewsPerBlock;
     return (int)(tiles * groups);
 }
 }  // namespace
@@ -399,7 +346,7 @@ BrickGrid siddon_bricks(const HostGeometry& g) {
 }
 
 int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream) {
+                     int nviews, int layout, float* proj, hipStream_t stream) {
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
     // 24-bit multiplier operands and 32-bit offsets
@@ -408,8 +355,9 @@ int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGri
     const bool buf = b.elements() * 4 < (1LL << 32);
     auto kern = buf ? ax_siddon_kernel<true> : ax_siddon_kernel<false>;
     KernelTimerScope timer(TIGRE_KERNEL_AX_SIDDON, stream);
-    hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, (unsigned long long)b.elements(),
-                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views, g.nangles, g.nDetec[0],
+    hipLaunchKernelGGL(kern, dim3(ray_blocks(g, nviews)), dim3(256), 0, stream, bricked,
+                       (unsigned long long)b.elements(), g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, d_views,
+                       nviews, g.nDetec[0],
                        g.nDetec[1], su, sv, sview, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
@@ -426,7 +374,7 @@ int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGri
     auto kern = (g.accuracy > 1.0) ? (buf ? ax_interp_kernel<true, true> : ax_interp_kernel<true, false>)
                                    : (buf ? ax_interp_kernel<false, true> : ax_interp_kernel<false, false>);
     KernelTimerScope timer(TIGRE_KERNEL_AX_INTERPOLATED, stream);
-    hipLaunchKernelGGL(kern, dim3(ray_blocks(g)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
+    hipLaunchKernelGGL(kern, dim3(ray_blocks(g, g.nangles)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
                        g.nVoxel[2], b.nbx, b.nby, b.nbz, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview,
                        proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());

@@ -1,0 +1,483 @@
+// Beam-staged Siddon forward projector for gfx950.
+//
+// A 1024-thread workgroup traces the rays of a 16 x 16 detector tile in up to
+// four views that share a principal axis P (the horizontal axis, x or y, along
+// which their rays advance fastest) and its direction.  Their rays form a thin
+// beam.  The workgroup sweeps the volume along P in slabs of a few planes; for
+// each slab it copies the box of voxels the beam crosses into LDS (16-byte
+// loads of rows of a re-laid-out copy of the volume), and every ray walks its
+// Siddon crossings inside the slab reading voxels from LDS.  The gathers of
+// the walk thus never reach the texture/L1 path, which bounds a gather-driven
+// walk; the slab copies are coalesced and are shared by the 1024 rays.
+//
+// The staging copy ("stage") of the volume is [p][z][a], a = the other
+// horizontal axis fastest, with a zero border: p in -1 .. nP, z in -1 .. nz,
+// a in -4 .. round_up(nA, 4) + 3.  Every voxel a walk can reach (indices -1 ..
+// n) therefore exists, and rows start on 16-byte boundaries.
+//
+// Planning: every wavefront evaluates the 16 corner rays of the tile (four
+// corners in each view) and computes the box origins of a share of the slabs;
+// the box extents, maxed over the slabs, are the same for the whole sweep, so
+// a ray's LDS offset carries from one slab to the next by a uniform shift.
+// Slabs are double-buffered: slab s + 1 streams into LDS by direct
+// global-to-LDS loads while the rays walk slab s, and one barrier ends each
+// slab.  A tile whose box does not fit the LDS buffer even with one plane per
+// slab walks the staging copy in global memory instead (same arithmetic).
+#include <cmath>
+
+#include "common.hpp"
+#include "ray.hpp"
+
+namespace tigre {
+
+namespace {
+
+constexpr int kBeamTile = 16;       // detector pixels per tile edge
+constexpr int kBeamViews = 4;       // views per workgroup
+constexpr int kBeamThreads = 1024;  // kBeamViews * kBeamTile * kBeamTile
+constexpr int kSlabCap = 9216;      // floats per LDS slab buffer (two per workgroup)
+constexpr int kSlabPlanes = 8;      // principal planes per slab, at most
+
+// Minimum / maximum over lanes 0..15 (the 16 corner rays; the other rows of
+// the wavefront hold copies), by four DPP row shifts, returned as a uniform.
+template <int kCtrl>
+__device__ __forceinline__ float row_shr(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), kCtrl, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float corner_min(float x) {
+    x = fminf(x, row_shr<0x111>(x));
+    x = fminf(x, row_shr<0x112>(x));
+    x = fminf(x, row_shr<0x114>(x));
+    x = fminf(x, row_shr<0x118>(x));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 15));
+}
+__device__ __forceinline__ float corner_max(float x) {
+    x = fmaxf(x, row_shr<0x111>(x));
+    x = fmaxf(x, row_shr<0x112>(x));
+    x = fmaxf(x, row_shr<0x114>(x));
+    x = fmaxf(x, row_shr<0x118>(x));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 15));
+}
+
+// Extent of the beam over principal planes lo .. hi: the beam's lines are
+// A(p) = cA + kA p, z(p) = cz + kz p (corner ray c in lane c; the other rows of
+// the wavefront hold copies), the voxels of the planes span P coordinates
+// [lo, hi + 1], and a margin of one voxel covers rounding.  Clipped to -1 .. n
+// (the walk never leaves it); a0 is aligned down to 4 for 16-byte rows.
+struct BoxExtent {
+    int a0, a1, z0, z1;
+};
+__device__ inline BoxExtent beam_extent(int lo, int hi, float cA, float kA, float cz, float kz, int nA, int nz) {
+    const float p0 = (float)lo, p1 = (float)(hi + 1);
+    const float A0 = fmaf(kA, p0, cA), A1 = fmaf(kA, p1, cA);
+    const float Z0 = fmaf(kz, p0, cz), Z1 = fmaf(kz, p1, cz);
+    float amin = corner_min(fminf(A0, A1)), amax = corner_max(fmaxf(A0, A1));
+    float zmin = corner_min(fminf(Z0, Z1)), zmax = corner_max(fmaxf(Z0, Z1));
+    amin = fminf(fmaxf(amin, -2.0f), (float)nA + 1.0f);
+    amax = fminf(fmaxf(amax, -2.0f), (float)nA + 1.0f);
+    zmin = fminf(fmaxf(zmin, -2.0f), (float)nz + 1.0f);
+    zmax = fminf(fmaxf(zmax, -2.0f), (float)nz + 1.0f);
+    BoxExtent e;
+    e.a0 = max((int)floorf(amin) - 1, -1) & ~3;
+    e.a1 = min((int)floorf(amax) + 1, nA);
+    e.z0 = max((int)floorf(zmin) - 1, -1);
+    e.z1 = min((int)floorf(zmax) + 1, nz);
+    return e;
+}
+
+// Slab s of a sweep with `planes` planes per slab along direction dir over
+// principal indices -1 .. nP.
+__device__ __forceinline__ void slab_range(int s, int planes, int dir, int nP, int& lo, int& hi) {
+    if (dir > 0) {
+        lo = -1 + s * planes;
+        hi = min(lo + planes - 1, nP);
+    } else {
+        hi = nP - s * planes;
+        lo = max(hi - planes + 1, -1);
+    }
+}
+
+struct StageGeom {
+    int NZ, NA;  // bordered extents of the staging copy (z, a)
+};
+
+__device__ __forceinline__ unsigned long long stage_index(const StageGeom& s, int p, int z, int a) {
+    return ((unsigned long long)(p + 1) * (unsigned)s.NZ + (unsigned)(z + 1)) * (unsigned)s.NA + (unsigned)(a + 4);
+}
+
+// Per-ray Siddon state (Siddon_projection.cu:110-268 restated: exact path
+// through the voxel grid, crossings taken in increasing parameter order).
+// The crossing counts k per axis give the next crossing on each axis,
+// t0 + k dt (recomputed exactly, never accumulated).
+struct RayWalk {
+    float tx, ty, tz, t0x, t0y, t0z, dtx, dty, dtz;
+    int kx, ky, kz;
+    unsigned off;          // voxel offset in the current source (LDS box or staging copy)
+    unsigned dX, dY, dZ;   // offset steps of one crossing on each axis
+    int kexit;             // principal crossing count at which the ray leaves the slab
+    float T, tc, sum;
+};
+
+// Walks crossings until the ray leaves the volume (returns true) or its
+// principal crossing count reaches kexit (returns false).  The axis of each
+// crossing is selected without branches; every step reads one voxel, issued
+// one step before its value is needed.
+template <int kP>
+__device__ inline bool walk(RayWalk& r, const float* __restrict__ src, int guard) {
+    // locals only: a conditional between struct members would be compiled as
+    // a select of their addresses and force the struct into scratch memory
+    float tx = r.tx, ty = r.ty, tz = r.tz, tc = r.tc, sum = r.sum;
+    int kx = r.kx, ky = r.ky, kz = r.kz;
+    unsigned off = r.off;
+    const unsigned dX = r.dX, dY = r.dY, dZ = r.dZ;
+    const float T = r.T, t0x = r.t0x, t0y = r.t0y, t0z = r.t0z, dtx = r.dtx, dty = r.dty, dtz = r.dtz;
+    const int kexit = r.kexit;
+    bool fin = false;
+    float va = src[off], vb = 0.0f;
+#define TIGRE_BEAM_STEP(CUR, NEXT)                                       \
+    {                                                                    \
+        const float tm = fminf(tx, fminf(ty, tz));                       \
+        fin = tm >= T;                                                   \
+        sum = fmaf((fin ? T : tm) - tc, CUR, sum);                       \
+        tc = tm;                                                         \
+        const bool bx = tx == tm, by = !bx && ty == tm, bz = !bx && !by; \
+        kx += bx ? 1 : 0;                                                \
+        ky += by ? 1 : 0;                                                \
+        kz += bz ? 1 : 0;                                                \
+        tx = fmaf((float)kx, dtx, t0x);                                  \
+        ty = fmaf((float)ky, dty, t0y);                                  \
+        tz = fmaf((float)kz, dtz, t0z);                                  \
+        off += bx ? dX : (by ? dY : dZ);                                 \
+        if (fin || (kP == 0 ? kx : ky) == kexit) break;                  \
+        NEXT = src[off];                                                 \
+    }
+    for (; guard > 0; --guard) {
+        TIGRE_BEAM_STEP(va, vb)
+        TIGRE_BEAM_STEP(vb, va)
+    }
+#undef TIGRE_BEAM_STEP
+    r.tx = tx;
+    r.ty = ty;
+    r.tz = tz;
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.off = off;
+    r.tc = tc;
+    r.sum = sum;
+    return fin;
+}
+
+// Workgroup table entry: views [first, first + count) of the ordered view
+// records share principal axis `principal` and direction `dir`.
+struct BeamGroup {
+    int first, count, principal, dir;
+};
+
+constexpr int kMaxSlabs = 272;
+
+struct SweepShared {
+    float buf[2 * kSlabCap];
+    int2 origin[kMaxSlabs];  // (a0, z0) of each slab box
+    int ext[2];              // box extents na (multiple of 4) and nzb, maxed over the slabs
+};
+
+template <int kP>
+__device__ inline void beam_sweep(const float* __restrict__ stage, const StageGeom& sg, int nx, int ny, int nz,
+                                  const FwdView* __restrict__ views, const BeamGroup& grp, int tile_u, int tile_v,
+                                  int nU, int nV, long long su, long long sv, long long sview,
+                                  float* __restrict__ out, double dvx, double dvy, double dvz, SweepShared& sh) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int nP = kP == 0 ? nx : ny, nA = kP == 0 ? ny : nx;
+    const int dir = grp.dir;
+
+    // ---- the lane's ray (FP64 set-up) ----
+    const int slot = wave >> 2;
+    const int u = tile_u * kBeamTile + (lane & 15);
+    const int v = tile_v * kBeamTile + (wave & 3) * 4 + (lane >> 4);
+    const bool active = slot < grp.count && u < nU && v < nV;
+    const FwdView& w = views[grp.first + min(slot, grp.count - 1)];
+    RayWalk r;
+    r.sum = 0.0f;
+    int state = 0;  // 0: no ray or finished, 1: waiting for the slab of its entry, 2: walking
+    int ip0 = 0, ia0 = 0, iz0 = 0, sx = 0, sy = 0, sz = 0;
+    double len = 0.0;
+    if (active) {
+        const double pv = (double)(nV - 1 - v);  // detector row flip, Siddon_projection.cu:141
+        double rx = w.org[0] + u * w.du[0] + pv * w.dv[0] - w.src[0];
+        double ry = w.org[1] + u * w.du[1] + pv * w.dv[1] - w.src[1];
+        double rz = w.org[2] + u * w.du[2] + pv * w.dv[2] - w.src[2];
+        // components below 1e-3 voxel are treated as exactly zero (:155-159)
+        if (fabs(rx) < 1e-3) rx = 0.0;
+        if (fabs(ry) < 1e-3) ry = 0.0;
+        if (fabs(rz) < 1e-3) rz = 0.0;
+        len = sqrt(rx * rx * dvx * dvx + ry * ry * dvy * dvy + rz * rz * dvz * dvz);  // mm (:241, :267)
+        double am = -INFINITY, aM = INFINITY;
+        bool hit = slab(w.src[0], rx, nx, am, aM);
+        hit = hit && slab(w.src[1], ry, ny, am, aM);
+        hit = hit && slab(w.src[2], rz, nz, am, aM);
+        if (hit && am < aM) {  // a ray that misses the volume projects to 0 (:179-180)
+            const Axis ax = axis_setup(w.src[0], rx, am);
+            const Axis ay = axis_setup(w.src[1], ry, am);
+            const Axis az = axis_setup(w.src[2], rz, am);
+            r.tx = r.t0x = ax.t0;
+            r.ty = r.t0y = ay.t0;
+            r.tz = r.t0z = az.t0;
+            r.dtx = ax.dt;
+            r.dty = ay.dt;
+            r.dtz = az.dt;
+            r.kx = r.ky = r.kz = 0;
+            r.T = (float)(aM - am);
+            r.tc = 0.0f;
+            ip0 = kP == 0 ? ax.idx : ay.idx;
+            ia0 = kP == 0 ? ay.idx : ax.idx;
+            iz0 = az.idx;
+            sx = ax.step;
+            sy = ay.step;
+            sz = az.step;
+            state = 1;
+        }
+    }
+
+    // ---- corner rays of the tile (lanes 0..15 of every wavefront) ----
+    float cA, kA, cz, kz;
+    {
+        const int c = lane & 15;
+        const int cs = min(c >> 2, grp.count - 1);
+        const FwdView& cw = views[grp.first + cs];
+        const int cu = min(tile_u * kBeamTile + ((c & 1) ? kBeamTile - 1 : 0), nU - 1);
+        const int cvv = min(tile_v * kBeamTile + ((c & 2) ? kBeamTile - 1 : 0), nV - 1);
+        const double pv = (double)(nV - 1 - cvv);
+        const double dP = cw.org[kP] + cu * cw.du[kP] + pv * cw.dv[kP] - cw.src[kP];
+        const double dA = cw.org[1 - kP] + cu * cw.du[1 - kP] + pv * cw.dv[1 - kP] - cw.src[1 - kP];
+        const double dZ = cw.org[2] + cu * cw.du[2] + pv * cw.dv[2] - cw.src[2];
+        const double ka = dA / dP, kzz = dZ / dP;
+        kA = (float)ka;
+        cA = (float)(cw.src[1 - kP] - cw.src[kP] * ka);
+        kz = (float)kzz;
+        cz = (float)(cw.src[2] - cw.src[kP] * kzz);
+    }
+
+    // ---- plan: box origins of every slab (in parallel over the wavefronts)
+    // and the extents shared by all of them; fewer planes per slab until the
+    // box fits an LDS buffer ----
+    int planes = 0, nslabs = 0, na = 0, nzb = 0;
+    bool staged = false;
+    for (int attempt = 0; attempt < 5 && !staged; ++attempt) {
+        const int pl_try = attempt == 0 ? kSlabPlanes : (attempt == 1 ? 6 : (attempt == 2 ? 4 : (attempt == 3 ? 2 : 1)));
+        const int ns = (nP + 2 + pl_try - 1) / pl_try;
+        if (ns > kMaxSlabs) continue;
+        if (tid == 0) sh.ext[0] = sh.ext[1] = 0;
+        __syncthreads();
+        for (int s = wave; s < ns; s += kBeamThreads / 64) {
+            int lo, hi;
+            slab_range(s, pl_try, dir, nP, lo, hi);
+            const BoxExtent e = beam_extent(lo, hi, cA, kA, cz, kz, nA, nz);
+            if (lane == 0) {
+                sh.origin[s] = make_int2(e.a0, e.z0);
+                atomicMax(&sh.ext[0], (e.a1 - e.a0 + 4) & ~3);
+                atomicMax(&sh.ext[1], e.z1 - e.z0 + 1);
+            }
+        }
+        __syncthreads();
+        na = sh.ext[0];
+        nzb = sh.ext[1];
+        if (na * nzb * pl_try <= kSlabCap) {
+            staged = true;
+            planes = pl_try;
+            nslabs = ns;
+        }
+        __syncthreads();  // ext is reset by the next attempt
+    }
+
+    if (!staged) {
+        // The beam's box does not fit an LDS buffer: walk from the staging copy.
+        if (state == 1) {
+            r.off = (unsigned)stage_index(sg, ip0, iz0, ia0);
+            r.dX = (unsigned)(sx * (kP == 0 ? sg.NZ * sg.NA : 1));
+            r.dY = (unsigned)(sy * (kP == 1 ? sg.NZ * sg.NA : 1));
+            r.dZ = (unsigned)(sz * sg.NA);
+            r.kexit = -1;  // never reached
+            walk<kP>(r, stage, (nx + ny + nz + 8) / 2);
+        }
+    } else {
+        const int SP = nzb * na;
+        // offset steps inside a box
+        r.dX = (unsigned)(sx * (kP == 0 ? SP : 1));
+        r.dY = (unsigned)(sy * (kP == 1 ? SP : 1));
+        r.dZ = (unsigned)(sz * na);
+        // this thread's quads of a box: row (plane * nzb + z) and 4-float column
+        const int nq4 = na >> 2, nq = planes * nzb * nq4;
+        unsigned long long qoff[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int q = min(tid + j * kBeamThreads, nq - 1);
+            const int row = q / nq4, aq = q - row * nq4;
+            const int pl = row / nzb, zr = row - pl * nzb;
+            qoff[j] = ((unsigned long long)pl * (unsigned)sg.NZ + (unsigned)zr) * (unsigned)sg.NA + 4u * aq;
+        }
+        const int wave_first = tid & ~63;
+        // Direct global-to-LDS copies (16 bytes per lane) of slab s: quad q lands
+        // at buf[4 q]; the LDS base of each instruction is uniform per wavefront.
+        auto fetch = [&](int s, float* dst) {
+            int lo, hi;
+            slab_range(s, planes, dir, nP, lo, hi);
+            const int2 o = sh.origin[s];
+            const unsigned long long base = stage_index(sg, lo, o.y, o.x);
+            const int nqs = (hi - lo + 1) * nzb * nq4;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                if (wave_first + j * kBeamThreads < nqs) {
+                    if (tid + j * kBeamThreads < nqs)
+                        __builtin_amdgcn_global_load_lds(stage + base + qoff[j], dst + 4 * (wave_first + j * kBeamThreads),
+                                                         16, 0, 0);
+                }
+            }
+        };
+        fetch(0, sh.buf);
+        __builtin_amdgcn_s_waitcnt(0);  // the direct LDS loads have landed
+        __syncthreads();
+        int plo = 0, pa0 = 0, pz0 = 0;
+        for (int s = 0; s < nslabs; ++s) {
+            if (s + 1 < nslabs) fetch(s + 1, sh.buf + ((s + 1) & 1) * kSlabCap);
+            int lo, hi;
+            slab_range(s, planes, dir, nP, lo, hi);
+            const int2 o = sh.origin[s];
+            if (state == 2) {
+                // paused at the first plane of this slab: re-base the offset
+                r.off += (unsigned)(-((lo - plo) * SP + (o.y - pz0) * na + (o.x - pa0)));
+                r.kexit += planes;
+            } else if (state == 1 && ip0 >= lo && ip0 <= hi) {
+                r.off = (unsigned)(((ip0 - lo) * nzb + (iz0 - o.y)) * na + (ia0 - o.x));
+                r.kexit = dir > 0 ? hi + 1 - ip0 : ip0 - lo + 1;
+                state = 2;
+            }
+            if (state == 2) {
+                // crossings inside one slab: its planes plus the box's a and z planes
+                if (walk<kP>(r, sh.buf + (s & 1) * kSlabCap, (planes + na + nzb + 5) / 2)) state = 0;
+            }
+            plo = lo;
+            pa0 = o.x;
+            pz0 = o.y;
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+    }
+
+    if (active) {
+        float* dst = out + (long long)w.pad[0] * sview + (long long)u * su + (long long)v * sv;
+        *dst = (float)((double)r.sum * len);
+    }
+}
+
+// Two workgroups per CU (8 wavefronts per SIMD): the register budget is held
+// to that occupancy, which the slab barriers need to keep the SIMDs busy.
+__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_siddon_beam_kernel(
+    const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
+    const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
+    long long su, long long sv, long long sview, float* __restrict__ out, double dvx, double dvy, double dvz) {
+    __shared__ SweepShared sh;
+    const int tilesU = (nU + kBeamTile - 1) / kBeamTile, tilesV = (nV + kBeamTile - 1) / kBeamTile;
+    const int tiles = tilesU * tilesV;
+    const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int gi = L / tiles, tile = L % tiles;
+    if (gi >= ngroups) return;  // uniform per workgroup
+    const BeamGroup grp = groups[gi];
+    StageGeom sg;
+    sg.NZ = nz + 2;
+    if (grp.principal == 0) {
+        sg.NA = na_x + 8;
+        beam_sweep<0>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview, out,
+                      dvx, dvy, dvz, sh);
+    } else {
+        sg.NA = na_y + 8;
+        beam_sweep<1>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview, out,
+                      dvx, dvy, dvz, sh);
+    }
+}
+
+// Staging copy for principal axis P: out[p][z][a] over the bordered ranges
+// p -1 .. nP, z -1 .. nz, a -4 .. na_pad + 3 (zero outside the volume), from the
+// x-fastest volume.  64 x 64 tiles of (p, a) per z row go through LDS, so both
+// the reads (x fastest) and the writes (a fastest) are coalesced whichever of
+// p, a is x.
+__global__ __launch_bounds__(256) void stage_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
+                                                           int principal, int na_pad, float* __restrict__ out) {
+    __shared__ float tile[64][65];
+    const int NP = (principal == 0 ? nx : ny) + 2, NZ = nz + 2, NA = na_pad + 8;
+    const int a_base = blockIdx.x * 64 - 4, p_base = blockIdx.y * 64 - 1, z = (int)blockIdx.z - 1;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    // read: x fastest across threads
+    for (int r = ty; r < 64; r += 4) {
+        int x, y;
+        if (principal == 0) {  // p = x, a = y: rows of the tile are a, columns p
+            x = p_base + tx;
+            y = a_base + r;
+        } else {               // p = y, a = x
+            x = a_base + tx;
+            y = p_base + r;
+        }
+        float val = 0.0f;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+            val = vol[((long long)z * ny + y) * nx + x];
+        if (principal == 0)
+            tile[r][tx] = val;   // tile[a][p]
+        else
+            tile[tx][r] = val;   // tile[a][p]
+    }
+    __syncthreads();
+    // write: a fastest across threads
+    for (int r = ty; r < 64; r += 4) {
+        const int p = p_base + r, a = a_base + tx;
+        if (p + 1 < NP && a + 4 < NA)
+            out[((long long)(p + 1) * NZ + (z + 1)) * NA + (a + 4)] = tile[tx][r];
+    }
+}
+
+}  // namespace
+
+int stage_pad(int n) { return (n + 3) & ~3; }
+
+long long stage_elements(const HostGeometry& g, int principal) {
+    const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
+    const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
+    return (long long)(nP + 2) * (g.nVoxel[2] + 2) * (stage_pad(nA) + 8);
+}
+
+int launch_stage_volume(const float* vol, const HostGeometry& g, int principal, float* stage, hipStream_t stream) {
+    const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
+    const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
+    const int NA = stage_pad(nA) + 8;
+    dim3 grid((unsigned)((NA + 63) / 64), (unsigned)((nP + 2 + 63) / 64), (unsigned)(g.nVoxel[2] + 2));
+    TIGRE_REQUIRE(grid.y <= 65535u && grid.z <= 65535u, "volume too large for the staging copy");
+    KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
+    hipLaunchKernelGGL(stage_volume_kernel, grid, dim3(256), 0, stream, vol, g.nVoxel[0], g.nVoxel[1], g.nVoxel[2],
+                       principal, stage_pad(nA), stage);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_ax_siddon_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
+                          const int* d_groups, int ngroups, int layout, float* proj, hipStream_t stream) {
+    const long long nU = g.nDetec[0], nV = g.nDetec[1];
+    long long su = 1, sv = nU;
+    if (layout == TIGRE_LAYOUT_MATLAB) {
+        su = nV;
+        sv = 1;
+    }
+    const long long tiles = ((nU + kBeamTile - 1) / kBeamTile) * ((nV + kBeamTile - 1) / kBeamTile);
+    const long long blocks = tiles * ngroups;
+    TIGRE_REQUIRE(blocks < (1LL << 31), "too many detector tiles for one launch");
+    // LDS offsets are 32-bit; global element offsets of the staging copy too
+    TIGRE_REQUIRE(stage_elements(g, 0) < (1LL << 32) && stage_elements(g, 1) < (1LL << 32),
+                  "volume too large for 32-bit staging offsets");
+    KernelTimerScope timer(TIGRE_KERNEL_AX_SIDDON, stream);
+    hipLaunchKernelGGL(ax_siddon_beam_kernel, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y,
+                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
+                       reinterpret_cast<const BeamGroup*>(d_groups), ngroups, g.nDetec[0], g.nDetec[1], su, sv,
+                       nU * nV, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+}  // namespace tigre

@@ -152,7 +152,14 @@ BrickGrid interp_bricks(const HostGeometry& g);
 int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream);
 int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream);
+                     int nviews, int layout, float* proj, hipStream_t stream);
+// Beam-staged Siddon (ax_beam.hip).  Staging copy for principal axis P
+// (0 = x, 1 = y): [p][z][a] with a zero border, a = the other horizontal axis.
+long long stage_elements(const HostGeometry& g, int principal);
+int launch_stage_volume(const float* vol, const HostGeometry& g, int principal, float* stage, hipStream_t stream);
+// d_groups: ngroups records {first view, view count (1..4), principal, direction}.
+int launch_ax_siddon_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
+                          const int* d_groups, int ngroups, int layout, float* proj, hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int layout, float* proj, hipStream_t stream);
 int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,

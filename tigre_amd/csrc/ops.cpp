@@ -1,9 +1,44 @@
 // Operator drivers on device-resident data: build the per-view records,
 // upload them, stage the padded operand and launch the kernel, all on the
 // caller's stream with stream-ordered scratch allocations.
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
 
 namespace tigre {
+
+namespace {
+
+// Beam eligibility of a view for the beam-staged Siddon projector: every ray
+// of the view must advance along the principal axis P in one direction.  The
+// ray direction is linear in the pixel coordinates, so checking the four
+// detector corners covers the whole detector.
+struct BeamClass {
+    int principal, dir;
+    bool ok;
+};
+
+BeamClass beam_class(const HostGeometry& g, const FwdView& w) {
+    const int nU = g.nDetec[0], nV = g.nDetec[1];
+    auto ray = [&](double u, double pv, int c) { return w.org[c] + u * w.du[c] + pv * w.dv[c] - w.src[c]; };
+    const double cu = 0.5 * (nU - 1), cv = 0.5 * (nV - 1);
+    BeamClass k;
+    k.principal = std::fabs(ray(cu, cv, 0)) >= std::fabs(ray(cu, cv, 1)) ? 0 : 1;
+    k.dir = ray(cu, cv, k.principal) >= 0.0 ? 1 : -1;
+    k.ok = true;
+    for (int cu_i = 0; cu_i < 2; ++cu_i)
+        for (int cv_i = 0; cv_i < 2; ++cv_i) {
+            const double u = cu_i ? nU - 1 : 0, pv = cv_i ? nV - 1 : 0;
+            const double d[3] = {ray(u, pv, 0), ray(u, pv, 1), ray(u, pv, 2)};
+            const double n = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            // a ray that barely advances along P would make the slab boxes huge
+            if (!(d[k.principal] * k.dir > 0.1 * n)) k.ok = false;
+        }
+    return k;
+}
+
+}  // namespace
 
 int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int projection_type, int layout,
               hipStream_t stream) {
@@ -13,17 +48,95 @@ int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int proj
     const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
     std::vector<FwdView> views;
     forward_views(g, interp, &views);
-    void* d_views = nullptr;
-    TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
-    const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
-    float* bricked = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
-    TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
-    if (interp)
+    if (interp) {
+        void* d_views = nullptr;
+        TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
+        const BrickGrid b = interp_bricks(g);
+        float* bricked = nullptr;
+        TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
+        TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
         TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
-    else
-        TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
+        TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
+        TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
+        return TIGRE_OK;
+    }
+
+    // Siddon: beam-eligible views grouped by (principal, direction), in
+    // rotation-angle order inside a class, four per workgroup; the others go
+    // through the gather-driven kernel.  pad[0] keeps each view's output index.
+    std::vector<int> cls(views.size());
+    std::vector<int> beam, rest;
+    for (size_t i = 0; i < views.size(); ++i) {
+        const BeamClass k = beam_class(g, views[i]);
+        cls[i] = k.ok ? k.principal * 2 + (k.dir > 0 ? 1 : 0) : -1;
+        (k.ok ? beam : rest).push_back((int)i);
+    }
+    auto angle = [&](int i) {
+        const double a = std::fmod(g.angles[3 * i], 2.0 * M_PI);
+        return a < 0.0 ? a + 2.0 * M_PI : a;
+    };
+    std::stable_sort(beam.begin(), beam.end(), [&](int a, int b) {
+        if (cls[a] != cls[b]) return cls[a] < cls[b];
+        return angle(a) < angle(b);
+    });
+    // Views of one workgroup must be close in angle, or their beams spread
+    // apart and the slab boxes grow: a group spans at most 3.5 typical view
+    // spacings (a class can wrap around 0 and leave a gap inside its list).
+    std::vector<double> sorted_angles;
+    for (int i : beam) sorted_angles.push_back(angle(i));
+    std::sort(sorted_angles.begin(), sorted_angles.end());
+    std::vector<double> gaps;
+    for (size_t i = 1; i < sorted_angles.size(); ++i) gaps.push_back(sorted_angles[i] - sorted_angles[i - 1]);
+    double spacing = 0.0;
+    if (!gaps.empty()) {
+        std::nth_element(gaps.begin(), gaps.begin() + gaps.size() / 2, gaps.end());
+        spacing = gaps[gaps.size() / 2];
+    }
+    const double max_spread = std::max(3.5 * spacing, 0.05);
+    std::vector<FwdView> ordered;
+    ordered.reserve(views.size());
+    std::vector<int> groups;  // {first, count, principal, dir} per workgroup row
+    for (size_t k = 0; k < beam.size();) {
+        const int c = cls[beam[k]];
+        size_t e = k;
+        while (e < beam.size() && e - k < 4 && cls[beam[e]] == c && angle(beam[e]) - angle(beam[k]) <= max_spread)
+            ++e;
+        groups.insert(groups.end(), {(int)ordered.size(), (int)(e - k), c / 2, (c & 1) ? 1 : -1});
+        for (size_t j = k; j < e; ++j) ordered.push_back(views[beam[j]]);
+        k = e;
+    }
+    const int nbeam = (int)ordered.size();
+    for (int i : rest) ordered.push_back(views[i]);
+    const int ngroups = (int)groups.size() / 4;
+
+    void* d_views = nullptr;
+    TIGRE_CHECK(upload_async(&d_views, ordered.data(), ordered.size() * sizeof(FwdView), stream));
+    if (ngroups > 0) {
+        void* d_groups = nullptr;
+        TIGRE_CHECK(upload_async(&d_groups, groups.data(), groups.size() * sizeof(int), stream));
+        float* stage[2] = {nullptr, nullptr};
+        for (int a = 0; a < 2; ++a) {
+            bool used = false;
+            for (int gi = 0; gi < ngroups; ++gi) used = used || groups[4 * gi + 2] == a;
+            if (!used) continue;
+            TIGRE_HIP_TRY(hipMallocAsync((void**)&stage[a], (size_t)stage_elements(g, a) * sizeof(float), stream));
+            TIGRE_CHECK(launch_stage_volume(d_img, g, a, stage[a], stream));
+        }
+        TIGRE_CHECK(launch_ax_siddon_beam(stage[0], stage[1], g, (const FwdView*)d_views, (const int*)d_groups,
+                                          ngroups, layout, d_proj, stream));
+        for (int a = 0; a < 2; ++a)
+            if (stage[a]) TIGRE_HIP_TRY(hipFreeAsync(stage[a], stream));
+        TIGRE_HIP_TRY(hipFreeAsync(d_groups, stream));
+    }
+    if (nbeam < (int)ordered.size()) {
+        const BrickGrid b = siddon_bricks(g);
+        float* bricked = nullptr;
+        TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
+        TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
+        TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views + nbeam, (int)ordered.size() - nbeam,
+                                     layout, d_proj, stream));
+        TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
+    }
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
     return TIGRE_OK;
 }
