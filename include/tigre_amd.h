@@ -42,7 +42,9 @@ extern "C" {
 #define TIGRE_BACKPROJ_MATCHED  1
 
 /* Storage types of volumes and projections (accumulation is always FP32). */
-#define TIGRE_DTYPE_F32 0
+#define TIGRE_DTYPE_F32  0   /* float */
+#define TIGRE_DTYPE_BF16 1   /* bfloat16 (upper half of an IEEE float) */
+#define TIGRE_DTYPE_F16  2   /* IEEE binary16 */
 
 /*
  * Per-projection geometry.  Field-for-field and byte-for-byte the layout of
@@ -98,6 +100,16 @@ int tigre_atb(const float* projections, const TigreGeometry* geo, float* result,
 int tigre_pocs_tv(const float* img, float* dst, float alpha, const long* image_size,
                   int maxIter, const int* gpuids, int ngpus);
 
+/* Replaces aw_pocs_tv (Common/CUDA/GD_AwTV.hpp:58): adaptive-weighted TV
+ * steps, difference weights exp(-(d / delta)^2). */
+int tigre_aw_pocs_tv(const float* img, float* dst, float alpha, const long* image_size,
+                     int maxIter, float delta, const int* gpuids, int ngpus);
+
+/* Replaces piccs_tv (Common/CUDA/PICCS.hpp:58): steps along
+ * (1 - ratio) grad TV(img) + ratio grad TV(img - prior). */
+int tigre_piccs_tv(const float* img, const float* prior, float* dst, float alpha, float ratio,
+                   const long* image_size, int maxIter, const int* gpuids, int ngpus);
+
 /* Replaces tvdenoising (Common/CUDA/tv_proximal.hpp:52).  spacing = {dx, dy, dz}. */
 int tigre_tvdenoising(const float* src, float* dst, float lambda, const float* spacing,
                       const long* image_size, int maxIter, const int* gpuids, int ngpus);
@@ -117,10 +129,28 @@ int tigre_atb_device(const float* d_proj, const TigreGeometry* geo, float* d_vol
                      const float* angles, int nangles, int backprojection_type, int layout,
                      void* stream);
 
+/* The two operators above on volumes and projections stored as `dtype`
+ * (TIGRE_DTYPE_*; both operands share it).  Values are widened to float on
+ * load, all arithmetic and accumulation is float, results are rounded to
+ * `dtype` (round to nearest even) on store.  No counterpart in the reference,
+ * which stores float only. */
+int tigre_ax_device_typed(const void* d_img, const TigreGeometry* geo, void* d_proj,
+                          const float* angles, int nangles, int projection_type, int layout,
+                          int dtype, void* stream);
+int tigre_atb_device_typed(const void* d_proj, const TigreGeometry* geo, void* d_vol,
+                           const float* angles, int nangles, int backprojection_type, int layout,
+                           int dtype, void* stream);
+
 /* maxIter TV steepest-descent steps on a device volume (X fastest); d_dst may
  * equal d_img.  image_size = {X, Y, Z}. */
 int tigre_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size,
                          int maxIter, void* stream);
+
+/* Device-pointer forms of tigre_aw_pocs_tv and tigre_piccs_tv. */
+int tigre_aw_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size,
+                            int maxIter, float delta, void* stream);
+int tigre_piccs_tv_device(const float* d_img, const float* d_prior, float* d_dst, float alpha,
+                          float ratio, const long* image_size, int maxIter, void* stream);
 
 /* ROF TV denoise of a device volume (d_dst may equal d_src). */
 int tigre_tvdenoising_device(const float* d_src, float* d_dst, float lambda, const float* spacing,

@@ -65,6 +65,12 @@ def lib():
         L.og_maxdistance_cuboid.restype = ctypes.c_double
         L.og_pocs_tv.argtypes = [vp, vp, ctypes.c_double, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int]
         L.og_tv_gradient.argtypes = [vp, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long]
+        L.og_tv_gradient_ex.argtypes = [vp, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double]
+        L.og_aw_pocs_tv.argtypes = [vp, vp, ctypes.c_double, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                    ctypes.c_int, ctypes.c_double]
+        L.og_piccs_tv.argtypes = [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_long, ctypes.c_long,
+                                  ctypes.c_long, ctypes.c_int]
         L.og_tvdenoise.argtypes = [vp, vp, ctypes.c_double, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                    ctypes.c_int]
         L.og_set_threads.argtypes = [ctypes.c_int]
@@ -176,6 +182,32 @@ def tv_gradient(img):
     out = np.empty_like(img)
     nz, ny, nx = img.shape
     lib().og_tv_gradient(_vp(img), _vp(out), nx, ny, nz)
+    return out
+
+
+def tv_gradient_ex(img, pad_z=True, delta=None, eps=1e-8):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.empty_like(img)
+    nz, ny, nx = img.shape
+    lib().og_tv_gradient_ex(_vp(img), _vp(out), nx, ny, nz, int(bool(pad_z)),
+                            0.0 if delta is None else 1.0 / float(delta), float(eps))
+    return out
+
+
+def aw_pocs_tv(img, alpha, maxiter, delta):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.empty_like(img)
+    nz, ny, nx = img.shape
+    lib().og_aw_pocs_tv(_vp(img), _vp(out), float(alpha), nx, ny, nz, int(maxiter), float(delta))
+    return out
+
+
+def piccs_tv(img, prior, alpha, ratio, maxiter):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    prior = np.ascontiguousarray(prior, dtype=np.float64)
+    out = np.empty_like(img)
+    nz, ny, nx = img.shape
+    lib().og_piccs_tv(_vp(img), _vp(prior), _vp(out), float(alpha), float(ratio), nx, ny, nz, int(maxiter))
     return out
 
 

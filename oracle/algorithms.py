@@ -68,7 +68,44 @@ def ramp_filter(geo, filter_name=None):
     return np.hstack((filt, filt[1:-1][::-1])).astype(np.float32), filt_len
 
 
-def fdk(proj, geo, angles, filter_name=None):
+def wang_pad(proj, geo):
+    """Offset-detector zero padding (single_pass_algorithms.py:78-111)."""
+    g = copy.deepcopy(geo)
+    g.offDetector = np.array(g.offDetector, dtype=np.float64)
+    g.nDetector = np.array(g.nDetector)
+    g.sDetector = np.array(g.sDetector, dtype=np.float64)
+    off = float(g.offDetector[1])
+    dU = float(geo.dDetector[1])
+    width = int(2 * off / dU)
+    g.offDetector[1] = off - width / 2 * dU
+    g.nDetector[1] = abs(width) + int(geo.nDetector[1])
+    g.sDetector[1] = g.nDetector[1] * dU
+    theta = (float(geo.sDetector[1]) / 2 - abs(off)) * np.sign(off)
+    z = np.zeros((proj.shape[0], proj.shape[1], abs(width)))
+    out = np.concatenate((z, proj), axis=2) if off > 0 else np.concatenate((proj, z), axis=2)
+    return out, g, theta
+
+
+def wang_weight(geo, theta):
+    """Wang redundancy weight over u (single_pass_algorithms.py:113-163)."""
+    DSO, DSD = float(np.ravel(geo.DSO)[0]), float(np.ravel(geo.DSD)[0])
+    offset = float(geo.offDetector[1]) + DSD / DSO * float(np.ravel(getattr(geo, "COR", 0.0))[0])
+    n = int(geo.nDetector[1])
+    w = np.ones(n)
+    lim = abs(theta * DSO / DSD)
+    for k in range(n):
+        t = ((k - n / 2 + 0.5) * float(geo.dDetector[1]) + abs(offset)) * DSO / DSD
+        if abs(t) <= lim:
+            w[k] = 0.5 * (np.sin(np.pi / 2 * np.arctan(t / DSO) / np.arctan(lim / DSO)) + 1)
+        if t < -lim:
+            w[k] = 0.0
+    return w[::-1] if theta < 0 else w
+
+
+def fdk(proj, geo, angles, filter_name=None, dowang=True):
+    if dowang and np.any(np.asarray(geo.offDetector)):
+        proj, geo, theta = wang_pad(np.asarray(proj, dtype=np.float64), geo)
+        proj = proj * (2 * wang_weight(geo, theta))
     g = prep(geo, angles)
     nV, nU = int(g.nDetector[0]), int(g.nDetector[1])
     dV, dU = float(g.dDetector[0]), float(g.dDetector[1])

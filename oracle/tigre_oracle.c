@@ -528,6 +528,96 @@ void og_tv_gradient(const double* f, double* g, long nx, long ny, long nz) {
             }
 }
 
+/* Backward differences without z padding (PICCS.cu:109-128): the z term is
+ * dropped at the low edge like the y and x terms; (z, y, x) inside the volume. */
+static inline void bdiff_unpadded(const double* f, long nx, long ny, long z, long y, long x, double d[3]) {
+    const long i = (z * ny + y) * nx + x;
+    d[0] = (z >= 1) ? f[i] - f[i - nx * ny] : 0.0;
+    d[1] = (y >= 1) ? f[i] - f[i - nx] : 0.0;
+    d[2] = (x >= 1) ? f[i] - f[i - 1] : 0.0;
+}
+
+/* Generalised sub-gradient: pad_z selects the GD_TV / GD_AwTV z padding (else
+ * PICCS's unpadded volume); inv_delta != 0 weights each difference d by
+ * exp(-(d inv_delta)^2) (GD_AwTV.cu:117-170); eps as in each reference kernel. */
+void og_tv_gradient_ex(const double* f, double* g, long nx, long ny, long nz, int pad_z, double inv_delta,
+                       double eps) {
+#pragma omp parallel for schedule(static)
+    for (long z = 0; z < nz; ++z)
+        for (long y = 0; y < ny; ++y)
+            for (long x = 0; x < nx; ++x) {
+                double df[3], di[3] = {0, 0, 0}, dj[3] = {0, 0, 0}, dk[3] = {0, 0, 0};
+                if (pad_z) {
+                    bdiff(f, nx, ny, nz, z, y, x, df);
+                    if (x + 1 < nx) bdiff(f, nx, ny, nz, z, y, x + 1, di);
+                    if (y + 1 < ny) bdiff(f, nx, ny, nz, z, y + 1, x, dj);
+                    bdiff(f, nx, ny, nz, z + 1, y, x, dk);
+                } else {
+                    bdiff_unpadded(f, nx, ny, z, y, x, df);
+                    if (x + 1 < nx) bdiff_unpadded(f, nx, ny, z, y, x + 1, di);
+                    if (y + 1 < ny) bdiff_unpadded(f, nx, ny, z, y + 1, x, dj);
+                    if (z + 1 < nz) bdiff_unpadded(f, nx, ny, z + 1, y, x, dk);
+                }
+                double w[12];
+                const double* all[4] = {df, di, dj, dk};
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 3; ++c) {
+                        const double t = all[a][c] * inv_delta;
+                        w[3 * a + c] = inv_delta != 0.0 ? exp(-t * t) : 1.0;
+                    }
+                double v = (w[0] * df[0] + w[1] * df[1] + w[2] * df[2]) /
+                           (sqrt(w[0] * df[0] * df[0] + w[1] * df[1] * df[1] + w[2] * df[2] * df[2]) + eps);
+                v -= w[5] * di[2] / (sqrt(w[3] * di[0] * di[0] + w[4] * di[1] * di[1] + w[5] * di[2] * di[2]) + eps);
+                v -= w[7] * dj[1] / (sqrt(w[6] * dj[0] * dj[0] + w[7] * dj[1] * dj[1] + w[8] * dj[2] * dj[2]) + eps);
+                v -= w[9] * dk[0] / (sqrt(w[9] * dk[0] * dk[0] + w[10] * dk[1] * dk[1] + w[11] * dk[2] * dk[2]) + eps);
+                g[(z * ny + y) * nx + x] = v;
+            }
+}
+
+/* aw_pocs_tv (GD_AwTV.cu:276-): pocs_tv with the weighted gradient. */
+void og_aw_pocs_tv(const double* img, double* dst, double alpha, long nx, long ny, long nz, int maxiter,
+                   double delta) {
+    size_t n = (size_t)nx * ny * nz;
+    double* g = (double*)malloc(n * sizeof(double));
+    memcpy(dst, img, n * sizeof(double));
+    for (int it = 0; it < maxiter; ++it) {
+        og_tv_gradient_ex(dst, g, nx, ny, nz, 1, 1.0 / delta, 1e-8);
+        double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+        for (long q = 0; q < (long)n; ++q) s += g[q] * g[q];
+        double scale = alpha / sqrt(s + 1e-7);
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < (long)n; ++q) dst[q] -= g[q] * scale;
+    }
+    free(g);
+}
+
+/* piccs_tv (PICCS.cu:274-380): g = (1 - ratio) grad TV(img) + ratio grad TV(img -
+ * prior), both unpadded with eps 1e-6; img -= alpha g / sqrt(sum g^2). */
+void og_piccs_tv(const double* img, const double* prior, double* dst, double alpha, double ratio, long nx, long ny,
+                 long nz, int maxiter) {
+    size_t n = (size_t)nx * ny * nz;
+    double* g = (double*)malloc(n * sizeof(double));
+    double* h = (double*)malloc(n * sizeof(double));
+    double* d = (double*)malloc(n * sizeof(double));
+    memcpy(dst, img, n * sizeof(double));
+    for (int it = 0; it < maxiter; ++it) {
+        og_tv_gradient_ex(dst, g, nx, ny, nz, 0, 0.0, 1e-6);
+        for (size_t q = 0; q < n; ++q) d[q] = dst[q] - prior[q];
+        og_tv_gradient_ex(d, h, nx, ny, nz, 0, 0.0, 1e-6);
+        double s = 0.0;
+        for (size_t q = 0; q < n; ++q) {
+            g[q] = (1.0 - ratio) * g[q] + ratio * h[q];
+            s += g[q] * g[q];
+        }
+        double scale = alpha / sqrt(s);
+        for (size_t q = 0; q < n; ++q) dst[q] -= g[q] * scale;
+    }
+    free(g);
+    free(h);
+    free(d);
+}
+
 /* pocs_tv single-device iteration, GD_TV.cu:489-597:
  * g = grad TV(img); img -= alpha * g / sqrt(sum(g^2) + 1e-7). */
 void og_pocs_tv(const double* img, double* dst, double alpha, long nx, long ny, long nz, int maxiter) {

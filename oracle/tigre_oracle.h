@@ -65,6 +65,17 @@ void og_pocs_tv(const double* img, double* dst, double alpha,
 /* One TV sub-gradient evaluation (exposed for the closed-form tests). */
 void og_tv_gradient(const double* img, double* grad, long nx, long ny, long nz);
 
+/* Generalised TV sub-gradient (GD_TV / GD_AwTV / PICCS kernels): pad_z = 1
+ * for the zero-padded z of pocs_tv / aw_pocs_tv, inv_delta != 0 for the AwTV
+ * weights, eps 1e-8 (TV, AwTV) or 1e-6 (PICCS). */
+void og_tv_gradient_ex(const double* img, double* grad, long nx, long ny, long nz, int pad_z,
+                       double inv_delta, double eps);
+/* aw_pocs_tv (GD_AwTV.cu) and piccs_tv (PICCS.cu), FP64. */
+void og_aw_pocs_tv(const double* img, double* dst, double alpha, long nx, long ny, long nz,
+                   int maxiter, double delta);
+void og_piccs_tv(const double* img, const double* prior, double* dst, double alpha, double ratio,
+                 long nx, long ny, long nz, int maxiter);
+
 /* ROF primal-dual denoise (tv_proximal.cu tvdenoising), FP64. */
 void og_tvdenoise(const double* src, double* dst, double lambda,
                   const double spacing[3], long nx, long ny, long nz, int maxiter);

@@ -216,3 +216,29 @@ def test_tvdenoise(cuda):
     err = oracle.rel_l2(got, oracle.tvdenoise(img, 15.0, 25))
     print("tvdenoise rel-L2 %.3e" % err)
     assert err <= TOL
+
+
+def test_aw_minTV(cuda):
+    import torch
+
+    img = ellipsoid_phantom((36, 40, 44), (36.0, 40.0, 44.0), 6, seed=12)
+    img += np.random.default_rng(6).normal(0, 0.05, img.shape).astype(np.float32)
+    got = tigre_amd.AwminTV(torch.from_numpy(img).to(cuda), 0.02, 15, delta=-0.05).cpu().numpy()
+    err = oracle.rel_l2(got, oracle.aw_pocs_tv(img, 0.02, 15, -0.05))
+    print("AwminTV rel-L2 %.3e" % err)
+    assert err <= TOL
+    assert np.array_equal(tigre_amd.AwminTV(img, 0.02, 15, delta=-0.05), got)
+
+
+def test_minPICCS(cuda):
+    import torch
+
+    img = ellipsoid_phantom((36, 40, 44), (36.0, 40.0, 44.0), 6, seed=13)
+    prior = ellipsoid_phantom((36, 40, 44), (36.0, 40.0, 44.0), 6, seed=14)
+    img += np.random.default_rng(7).normal(0, 0.05, img.shape).astype(np.float32)
+    got = tigre_amd.minPICCS(torch.from_numpy(img).to(cuda), torch.from_numpy(prior).to(cuda), 0.02, 15,
+                             ratio=0.6).cpu().numpy()
+    err = oracle.rel_l2(got, oracle.piccs_tv(img, prior, 0.02, 0.6, 15))
+    print("minPICCS rel-L2 %.3e" % err)
+    assert err <= TOL
+    assert np.array_equal(tigre_amd.minPICCS(img, prior, 0.02, 15, ratio=0.6), got)

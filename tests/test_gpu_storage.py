@@ -1,0 +1,76 @@
+"""Narrow storage on the GPU: volumes and projections in bfloat16 / float16,
+float32 arithmetic.  Reference: the FP64 oracle applied to the same rounded
+inputs, so the measured error is the operator's plus the output rounding.
+
+Tolerance (north-star precision rule): relative L2 <= 3e-3 for BF16 storage;
+FP16 (11-bit significand) is held to the same bound.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import algorithms as oalg
+import tigre_amd
+from tigre_amd import algorithms as algs
+from tigre_amd.geometry import cone_geometry, prepared
+from tigre_amd.phantom import ellipsoid_phantom
+
+pytestmark = pytest.mark.gpu
+TOL = 3e-3
+
+
+def _case():
+    geo = cone_geometry((64, 64, 64), (64.0, 64.0, 64.0), (64, 80), (1.6, 1.6), 1000.0, 1536.0)
+    angles = np.linspace(0, 2 * np.pi, 48, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 8, seed=31)
+    return geo, angles, img, prepared(geo, angles)
+
+
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+@pytest.mark.parametrize("ptype", ["Siddon", "interpolated"])
+def test_ax_storage(cuda, dtype, ptype):
+    import torch
+
+    geo, angles, img, gp = _case()
+    t = torch.from_numpy(img).to(cuda).to(getattr(torch, dtype))
+    got = tigre_amd.Ax(t, geo, angles, ptype)
+    assert got.dtype == t.dtype
+    ref = oracle.ax(t.float().cpu().numpy(), gp, ptype)
+    err = oracle.rel_l2(got.float().cpu().numpy(), ref)
+    print("Ax %s %s rel-L2 %.3e" % (ptype, dtype, err))
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+@pytest.mark.parametrize("btype", ["FDK", "matched"])
+def test_atb_storage(cuda, dtype, btype):
+    import torch
+
+    geo, angles, img, gp = _case()
+    proj = oracle.ax(img, gp).astype(np.float32)
+    t = torch.from_numpy(proj).to(cuda).to(getattr(torch, dtype))
+    got = tigre_amd.Atb(t, geo, angles, btype)
+    assert got.dtype == t.dtype
+    ref = oracle.atb(t.float().cpu().numpy(), gp, btype)
+    err = oracle.rel_l2(got.float().cpu().numpy(), ref)
+    print("Atb %s %s rel-L2 %.3e" % (btype, dtype, err))
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
+def test_cgls_storage(cuda, dtype):
+    """CGLS with every vector stored narrow: the iterate against the FP64 oracle
+    CGLS, beside the float32-storage run (reported; bounded loosely, since the
+    rounding of each update accumulates over the iterations)."""
+    import torch
+
+    geo, angles, img, gp = _case()
+    proj = oracle.ax(img, gp).astype(np.float32)
+    ref, _ = oalg.cgls(proj, geo, angles, 8)
+    x32 = algs.cgls(proj, geo, angles, 8)
+    xn = algs.cgls(proj, geo, angles, 8, storage=getattr(torch, dtype))
+    e32 = oracle.rel_l2(x32, ref)
+    en = oracle.rel_l2(xn, ref)
+    print("CGLS 8 it: float32 storage rel-L2 %.3e, %s storage rel-L2 %.3e" % (e32, dtype, en))
+    assert e32 <= 1e-4
+    assert en <= 3e-2

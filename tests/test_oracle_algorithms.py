@@ -53,3 +53,36 @@ def test_cgls_residual_decreases(problem):
     x, r = oalg.cgls(proj, geo, angles, 6)
     assert all(b < a for a, b in zip(r, r[1:]))
     assert nrmse(x, img) < 0.2
+
+
+def test_wang_weights_redundancy():
+    """Detector-offset weights: a ray measured twice (positions t and -t of the
+    central plane inside the overlap) gets total weight 1; beyond the overlap the
+    short side gets 0 and the long side 1; the product and the oracle agree."""
+    from oracle import algorithms as oalg
+    from tigre_amd.algorithms import wang_weights
+    from tigre_amd.geometry import cone_geometry
+
+    geo = cone_geometry((32, 32, 32), (32.0, 32.0, 32.0), (16, 64), (1.0, 1.0), 500.0, 1000.0)
+    for off in (10.0, -13.0):
+        geo.offDetector = np.array([0.0, off])
+        proj = np.zeros((2, 16, 64))
+        _, zgeo, theta = oalg.wang_pad(proj, geo)
+        w = oalg.wang_weight(zgeo, theta)
+        n = int(zgeo.nDetector[1])
+        DSO, DSD = 500.0, 1000.0
+        offset = float(zgeo.offDetector[1])
+        t = ((np.arange(n) - n / 2 + 0.5) * 1.0 + abs(offset)) * DSO / DSD
+        lim = abs(theta) * DSO / DSD
+        ww = w[::-1] if theta < 0 else w
+        inside = np.abs(t) <= lim
+        # mirror pairs t_k = -t_j inside the overlap
+        pairs = 0
+        for k in np.nonzero(inside)[0]:
+            j = np.argmin(np.abs(t + t[k]))
+            if abs(t[j] + t[k]) < 1e-9:
+                assert abs(ww[k] + ww[j] - 1.0) < 1e-12
+                pairs += 1
+        assert pairs >= 8
+        assert np.all(ww[t < -lim] == 0.0) and np.all(ww[t > lim] == 1.0)
+        assert np.allclose(wang_weights(zgeo, theta), w, rtol=0, atol=1e-12)

@@ -115,3 +115,81 @@ def test_tvdenoise_reaches_rof_minimiser():
     u_star = rof_chambolle(fe, lam)[1:-1]
     got = oracle.tvdenoise(f, lam, 3000)
     assert np.max(np.abs(got - u_star)) < 1e-4
+
+
+def _tv_functional(f, pad_z, w=None):
+    """Sum over voxels of sqrt(sum_k w_k d_k^2) with the reference's backward
+    differences (x/y dropped at the low edge; z padded with zeros or dropped)."""
+    nz, ny, nx = f.shape
+    if pad_z:
+        g = np.zeros((nz + 2, ny, nx))
+        g[1:-1] = f
+        zs = range(1, nz + 2)
+    else:
+        g = f
+        zs = range(nz)
+    total = 0.0
+    for z in zs:
+        for y in range(ny):
+            for x in range(nx):
+                d0 = g[z, y, x] - g[z - 1, y, x] if (z >= 1) else 0.0
+                d1 = g[z, y, x] - g[z, y - 1, x] if y >= 1 else 0.0
+                d2 = g[z, y, x] - g[z, y, x - 1] if x >= 1 else 0.0
+                ww = (1, 1, 1) if w is None else w(z, y, x)
+                total += np.sqrt(ww[0] * d0 * d0 + ww[1] * d1 * d1 + ww[2] * d2 * d2)
+    return total
+
+
+def test_aw_gradient_is_derivative_with_frozen_weights():
+    """GD_AwTV's gradient is the derivative of sum sqrt(sum_k w_k d_k^2) with the
+    weights w_k = exp(-(d_k/delta)^2) held at their current values."""
+    rng = np.random.default_rng(3)
+    f = rng.normal(0, 1, (4, 5, 6))
+    delta = 0.9
+    g = oracle.tv_gradient_ex(f, pad_z=True, delta=delta, eps=1e-12)
+    nz, ny, nx = f.shape
+    fp = np.zeros((nz + 2, ny, nx))
+    fp[1:-1] = f
+    W = {}
+    for z in range(1, nz + 2):
+        for y in range(ny):
+            for x in range(nx):
+                d = [fp[z, y, x] - fp[z - 1, y, x], fp[z, y, x] - fp[z, y - 1, x] if y else 0.0,
+                     fp[z, y, x] - fp[z, y, x - 1] if x else 0.0]
+                W[(z, y, x)] = [np.exp(-(v / delta) ** 2) for v in d]
+    h = 1e-6
+    for (z, y, x) in [(0, 0, 0), (1, 2, 3), (3, 4, 5), (2, 0, 4)]:
+        fa, fb = f.copy(), f.copy()
+        fa[z, y, x] += h
+        fb[z, y, x] -= h
+        num = (_tv_functional(fa, True, lambda *k: W[k]) - _tv_functional(fb, True, lambda *k: W[k])) / (2 * h)
+        assert abs(num - g[z, y, x]) < 1e-6
+
+
+def test_aw_tends_to_tv_for_large_delta():
+    rng = np.random.default_rng(4)
+    f = rng.normal(0, 1, (5, 6, 7))
+    a = oracle.aw_pocs_tv(f, 0.1, 4, 1e9)
+    b = oracle.pocs_tv(f, 0.1, 4)
+    assert np.max(np.abs(a - b)) < 1e-12
+
+
+def test_piccs_gradient_unpadded_and_prior_cases():
+    rng = np.random.default_rng(5)
+    f = rng.normal(0, 1, (4, 5, 6))
+    g = oracle.tv_gradient_ex(f, pad_z=False, eps=1e-12)
+    h = 1e-6
+    for (z, y, x) in [(0, 0, 0), (3, 4, 5), (1, 2, 3)]:
+        fa, fb = f.copy(), f.copy()
+        fa[z, y, x] += h
+        fb[z, y, x] -= h
+        num = (_tv_functional(fa, False) - _tv_functional(fb, False)) / (2 * h)
+        assert abs(num - g[z, y, x]) < 1e-6
+    # prior equal to the image: the prior term vanishes and the normalised step
+    # is the plain one
+    a = oracle.piccs_tv(f, f, 0.05, 0.5, 1)
+    b = oracle.piccs_tv(f, rng.normal(0, 1, f.shape), 0.05, 0.0, 1)
+    assert np.max(np.abs(a - b)) < 1e-12
+    # every step moves the image by alpha in L2
+    c = oracle.piccs_tv(f, rng.normal(0, 1, f.shape), 0.05, 0.3, 1)
+    assert abs(np.linalg.norm(c - f) - 0.05) < 1e-12

@@ -7,6 +7,6 @@ Public names mirror the reference package (Python/tigre/__init__.py):
 from .geometry import ConeGeometryDefault, Geometry, cone_geometry, geometry  # noqa: F401
 from .geometry import ConeGeometryDefault as geometry_default  # noqa: F401
 from .operators import Atb, Ax  # noqa: F401
-from .tv import im3ddenoise, minimizeTV, minTV, tvdenoise  # noqa: F401
+from .tv import AwminTV, im3ddenoise, minimizeTV, minPICCS, minTV, tvdenoise  # noqa: F401
 
 __version__ = "0.1.0"

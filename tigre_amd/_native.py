@@ -19,6 +19,9 @@ PROJ_SIDDON = 0
 PROJ_INTERPOLATED = 1
 BACKPROJ_FDK = 0
 BACKPROJ_MATCHED = 1
+DTYPE_F32 = 0
+DTYPE_BF16 = 1
+DTYPE_F16 = 2
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -57,6 +60,10 @@ SIGNATURES = {
                                  ctypes.c_void_p, ctypes.c_int]),
     "tigre_pocs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "tigre_aw_pocs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_int]),
+    "tigre_piccs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "tigre_tvdenoising": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "tigre_ax_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
@@ -65,8 +72,18 @@ SIGNATURES = {
     "tigre_atb_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p]),
+    "tigre_ax_device_typed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_void_p]),
+    "tigre_atb_device_typed": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_void_p]),
     "tigre_pocs_tv_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "tigre_aw_pocs_tv_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                               ctypes.c_int, ctypes.c_float, ctypes.c_void_p]),
+    "tigre_piccs_tv_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "tigre_tvdenoising_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.c_void_p]),

@@ -74,26 +74,37 @@ class ViewSets:
         return self.sets[j][0]
 
     def ax(self, img, j, kind=_native.PROJ_SIDDON, out=None):
+        """Projections of subset j, stored like ``img`` (float32, bfloat16 or float16)."""
         torch = _torch()
         idx, g, cg, ang = self.sets[j]
         if out is None:
-            out = torch.empty((len(idx),) + self.det_shape, dtype=torch.float32, device=img.device)
-        st = self.lib.tigre_ax_device(ctypes.c_void_p(img.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                      ang.ctypes.data_as(ctypes.c_void_p), len(idx), kind, _native.LAYOUT_PYTHON,
-                                      _stream_handle())
+            out = torch.empty((len(idx),) + self.det_shape, dtype=img.dtype, device=img.device)
+        st = self.lib.tigre_ax_device_typed(ctypes.c_void_p(img.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
+                                            ang.ctypes.data_as(ctypes.c_void_p), len(idx), kind,
+                                            _native.LAYOUT_PYTHON, _storage_code(img), _stream_handle())
         _native.check(st, "TIGRE: Call to Ax failed")
         return out
 
     def atb(self, proj, j, kind=_native.BACKPROJ_FDK, out=None):
+        """Backprojection of subset j, stored like ``proj``."""
         torch = _torch()
         idx, g, cg, ang = self.sets[j]
         if out is None:
-            out = torch.empty(self.vol_shape, dtype=torch.float32, device=proj.device)
-        st = self.lib.tigre_atb_device(ctypes.c_void_p(proj.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                       ang.ctypes.data_as(ctypes.c_void_p), len(idx), kind, _native.LAYOUT_PYTHON,
-                                       _stream_handle())
+            out = torch.empty(self.vol_shape, dtype=proj.dtype, device=proj.device)
+        st = self.lib.tigre_atb_device_typed(ctypes.c_void_p(proj.data_ptr()), cg.ptr(),
+                                             ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p),
+                                             len(idx), kind, _native.LAYOUT_PYTHON, _storage_code(proj),
+                                             _stream_handle())
         _native.check(st, "Atb:")
         return out
+
+
+def _storage_code(t):
+    torch = _torch()
+    codes = {torch.float32: _native.DTYPE_F32, torch.bfloat16: _native.DTYPE_BF16, torch.float16: _native.DTYPE_F16}
+    if t.dtype not in codes:
+        raise TypeError("volumes and projections are stored as float32, bfloat16 or float16, not " + str(t.dtype))
+    return codes[t.dtype]
 
 
 def _to_device(x):
@@ -104,7 +115,7 @@ def _to_device(x):
 
 
 def _result(x, return_tensor):
-    return x if return_tensor else x.cpu().numpy()
+    return x if return_tensor else x.float().cpu().numpy()
 
 
 def _norm(x):
@@ -115,16 +126,67 @@ def _norm(x):
 # ---------------------------------------------------------------------------
 # FDK
 # ---------------------------------------------------------------------------
-def fdk(proj, geo, angles, filter=None, verbose=False, return_tensor=False, **kwargs):
-    """Feldkamp-Davis-Kress, single_pass_algorithms.py:12-202: cosine weight,
-    ramp filtering, FDK-weighted backprojection."""
+def _wang_padding(p, geo):
+    """Zero padding for an offset detector (single_pass_algorithms.py:78-111):
+    2 |offset| / dU columns of zeros on the side facing away from the offset,
+    detector size and offset updated to match.  Returns (padded, geometry,
+    theta), theta = (sU / 2 - |offset|) sign(offset) = the overlap half-width."""
     torch = _torch()
-    if kwargs.get("dowang", True) and np.any(np.asarray(getattr(geo, "offDetector", 0))):
-        raise NotImplementedError("detector-offset (Wang) weighting is not implemented; pass dowang=False")
+    zgeo = copy.deepcopy(geo)
+    zgeo.offDetector = np.array(zgeo.offDetector, dtype=np.float64)
+    zgeo.nDetector = np.array(zgeo.nDetector)
+    zgeo.sDetector = np.array(zgeo.sDetector, dtype=np.float64)
+    # the offset the reference reads: [1] of a per-scan pair, [1, 0] of a per-view table
+    off = float(zgeo.offDetector[1, 0] if zgeo.offDetector.ndim == 2 else zgeo.offDetector[1])
+    dU = float(geo.dDetector[1])
+    pad = int(2 * off / dU)
+    zgeo.offDetector[1] = off - pad / 2 * dU
+    zgeo.nDetector[1] = abs(pad) + int(geo.nDetector[1])
+    zgeo.sDetector[1] = zgeo.nDetector[1] * dU
+    theta = (float(geo.sDetector[1]) / 2 - abs(off)) * np.sign(off)
+    zeros = torch.zeros(p.shape[0], p.shape[1], abs(pad), dtype=p.dtype, device=p.device)
+    padded = torch.cat((zeros, p), dim=2) if off > 0 else torch.cat((p, zeros), dim=2)
+    return padded.contiguous(), zgeo, theta
+
+
+def wang_weights(geo, theta):
+    """Detector-offset redundancy weights over the u axis (single_pass_algorithms.py:113-163):
+    positions t = (u_k + |offset + (DSD / DSO) COR|) DSO / DSD on the central plane;
+    w = (sin(pi / 2 atan(t / DSO) / atan(|theta| DSO / DSD / DSO)) + 1) / 2 inside the
+    overlap |t| <= |theta| DSO / DSD, 0 beyond it on the short side, 1 on the long side;
+    mirrored when theta < 0.  The projections are then scaled by 2 w."""
+    off = np.asarray(geo.offDetector, dtype=np.float64)
+    offset = float(off[1] if off.ndim == 1 else off[1, 0])
+    DSO = float(np.asarray(geo.DSO).reshape(-1)[0])
+    DSD = float(np.asarray(geo.DSD).reshape(-1)[0])
+    COR = float(np.asarray(getattr(geo, "COR", 0.0)).reshape(-1)[0])
+    offset = offset + (DSD / DSO) * COR
+    nU = int(geo.nDetector[1])
+    us = (np.arange(-nU / 2 + 0.5, nU / 2 - 0.5 + 1) * float(geo.dDetector[1]) + abs(offset)) * DSO / DSD
+    abstheta = abs(theta * DSO / DSD)
+    w = np.ones(nU)
+    inside = np.abs(us) <= abstheta
+    w[inside] = 0.5 * (np.sin((np.pi / 2) * np.arctan(us[inside] / DSO) / np.arctan(abstheta / DSO)) + 1)
+    w[us < -abstheta] = 0
+    return w[::-1].copy() if theta < 0 else w
+
+
+def fdk(proj, geo, angles, filter=None, verbose=False, return_tensor=False, **kwargs):
+    """Feldkamp-Davis-Kress, single_pass_algorithms.py:12-202: detector-offset
+    (Wang) weights when the detector is offset, cosine weight, ramp
+    filtering, FDK-weighted backprojection."""
+    torch = _torch()
+    p = _to_device(proj)
+    dowang = kwargs.get("dowang", True) and bool(np.any(np.asarray(getattr(geo, "offDetector", 0))))
+    if dowang:
+        if verbose:
+            print("FDK: applying detector offset weights")
+        p, geo, theta = _wang_padding(p, geo)
+        w = torch.as_tensor((2.0 * wang_weights(geo, theta)).astype(np.float32), device=p.device)
+        p = p * w
     g = copy.deepcopy(geo)
     g.check_geo(angles)
     g.filter = filter
-    p = _to_device(proj)
     nV, nU = int(g.nDetector[0]), int(g.nDetector[1])
     dV, dU = float(g.dDetector[0]), float(g.dDetector[1])
     DSD0 = float(np.asarray(g.DSD).reshape(-1)[0])
@@ -250,18 +312,24 @@ def sirt(proj, geo, angles, niter, **kwargs):
 # CGLS (krylov_subspace_algorithms.py:22-97)
 # ---------------------------------------------------------------------------
 def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=False, computel2=False, init=None,
-         **kwargs):
+         storage=None, **kwargs):
     """Conjugate gradient least squares with Siddon forward and matched
-    backprojection; FP64 inner products; restarts once on residual growth."""
+    backprojection (krylov_subspace_algorithms.py:22-97); FP64 inner products
+    and norms; restarts once on residual growth.
+
+    ``storage`` (a torch dtype: float32 by default, bfloat16 or float16) is the
+    type every volume and sinogram of the iteration is stored in; the operators
+    and the vector updates compute in float32 and round once on store."""
     torch = _torch()
+    dt = torch.float32 if storage is None else storage
     g = copy.deepcopy(geo)
     g.check_geo(angles)
     vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
-    b = _to_device(proj)
-    x = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda") if init is None else _to_device(init).clone()
+    b = _to_device(proj).to(dt)
+    x = torch.zeros(vs.vol_shape, dtype=dt, device="cuda") if init is None else _to_device(init).to(dt)
 
     def start():
-        r = b - vs.ax(x, 0)
+        r = torch.sub(b, vs.ax(x, 0))
         p = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
         return r, p, _norm(p) ** 2
 
@@ -272,10 +340,10 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
     while i < niter:
         q = vs.ax(p, 0)
         alpha = gamma / (_norm(q) ** 2)
-        x += alpha * p
-        l2[0, i] = _norm(b - vs.ax(x, 0))
+        torch.add(x, p, alpha=alpha, out=x)
+        l2[0, i] = _norm(torch.sub(b, vs.ax(x, 0)))
         if i > 0 and l2[0, i] > l2[0, i - 1]:
-            x -= alpha * p
+            torch.sub(x, p, alpha=alpha, out=x)
             if verbose:
                 print("re-initilization of CGLS called at iteration:" + str(i))
             if reinit_at + 1 == i or not restart:
@@ -284,10 +352,10 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
             reinit_at = i
             r, p, gamma = start()
             continue
-        r -= alpha * q
+        torch.sub(r, q, alpha=alpha, out=r)
         s = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
         gamma1 = _norm(s) ** 2
-        p = s + (gamma1 / gamma) * p
+        p = torch.add(s, p, alpha=gamma1 / gamma)
         gamma = gamma1
         i += 1
     res = _result(x, return_tensor)

@@ -145,8 +145,9 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
 
 // Quad copy of the projections (see above), view-major, rows of quad_det(nU)
 // elements.  Reads either detector layout.
-__global__ __launch_bounds__(256) void quad_projections_kernel(const float* __restrict__ proj, int nU, int nV,
-                                                               int nviews, int layout, float4* __restrict__ quads) {
+__global__ __launch_bounds__(256) void quad_projections_kernel(const void* __restrict__ proj, int dtype, int nU,
+                                                               int nV, int nviews, int layout,
+                                                               float4* __restrict__ quads) {
     const long long QU = quad_det(nU), QV = quad_det(nV);
     const long long total = QU * QV * nviews;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
@@ -159,21 +160,22 @@ __global__ __launch_bounds__(256) void quad_projections_kernel(const float* __re
             const long long uu = u + (k & 1), vv = v + (k >> 1);
             float val = 0.0f;
             if (uu >= 0 && uu < nU && vv >= 0 && vv < nV)
-                val = (layout == TIGRE_LAYOUT_MATLAB) ? proj[base + uu * nV + vv] : proj[base + vv * nU + uu];
+                val = load_storage(proj, (layout == TIGRE_LAYOUT_MATLAB) ? base + uu * nV + vv : base + vv * nU + uu,
+                                   dtype);
             t[k] = val;
         }
         quads[q] = make_float4(t[0], t[1], t[2], t[3]);
     }
 }
 
-int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,
+int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nviews, int layout, float* quads,
                             hipStream_t stream) {
     const long long total = quad_det(nU) * quad_det(nV) * (long long)nviews;
     long long blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
-    hipLaunchKernelGGL(quad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, nU, nV, nviews,
-                       layout, reinterpret_cast<float4*>(quads));
+    hipLaunchKernelGGL(quad_projections_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, proj, dtype, nU, nV,
+                       nviews, layout, reinterpret_cast<float4*>(quads));
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

@@ -263,8 +263,9 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
 
 // 4^3 bricks over padded coordinates [0, 4 nb): voxel index = padded - 1,
 // zero outside the volume.  Output element q: brick q >> 6, local q & 63.
-__global__ __launch_bounds__(256) void brick_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                           int nbx, int nby, int nbz, float* __restrict__ bricked) {
+__global__ __launch_bounds__(256) void brick_volume_kernel(const void* __restrict__ vol, int dtype, int nx, int ny,
+                                                           int nz, int nbx, int nby, int nbz,
+                                                           float* __restrict__ bricked) {
     const long long total = (long long)nbx * nby * nbz * 64;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
@@ -273,15 +274,16 @@ __global__ __launch_bounds__(256) void brick_volume_kernel(const float* __restri
         const long long bx = b % nbx, by = (b / nbx) % nby, bz = b / ((long long)nbx * nby);
         const long long i = bx * 4 + (l & 3) - 1, j = by * 4 + ((l >> 2) & 3) - 1, k = bz * 4 + (l >> 4) - 1;
         float v = 0.0f;
-        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = load_storage(vol, (k * ny + j) * nx + i, dtype);
         bricked[q] = v;
     }
 }
 
 // Apron bricks: brick b stores padded coordinates 4b .. 4b + 4 on each axis
 // (5^3 values, x fastest), so neighbouring bricks share one face.
-__global__ __launch_bounds__(256) void apron_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                           int nbx, int nby, int nbz, float* __restrict__ bricked) {
+__global__ __launch_bounds__(256) void apron_volume_kernel(const void* __restrict__ vol, int dtype, int nx, int ny,
+                                                           int nz, int nbx, int nby, int nbz,
+                                                           float* __restrict__ bricked) {
     const long long total = (long long)nbx * nby * nbz * 125;
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) void apron_volume_kernel(const float* __restri
         const long long bx = b % nbx, by = (b / nbx) % nby, bz = b / ((long long)nbx * nby);
         const long long i = bx * 4 + (l % 5) - 1, j = by * 4 + ((l / 5) % 5) - 1, k = bz * 4 + (l / 25) - 1;
         float v = 0.0f;
-        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = vol[(k * ny + j) * nx + i];
+        if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) v = load_storage(vol, (k * ny + j) * nx + i, dtype);
         bricked[q] = v;
     }
 }
@@ -326,16 +328,16 @@ BrickGrid interp_bricks(const HostGeometry& g) {
     return BrickGrid{(g.nVoxel[0] + 1) / 4 + 1, (g.nVoxel[1] + 1) / 4 + 1, (g.nVoxel[2] + 1) / 4 + 1, 125};
 }
 
-int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
+int launch_brick_volume(const void* vol, int dtype, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream) {
     const long long total = b.elements();
     KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
     if (b.per_brick == 64)
         hipLaunchKernelGGL(brick_volume_kernel, dim3((unsigned)grid_stride_blocks(total)), dim3(256), 0, stream, vol,
-                           g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
+                           dtype, g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
     else
         hipLaunchKernelGGL(apron_volume_kernel, dim3((unsigned)grid_stride_blocks(total)), dim3(256), 0, stream, vol,
-                           g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
+                           dtype, g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], b.nbx, b.nby, b.nbz, bricked);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }
@@ -364,7 +366,7 @@ int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGri
 }
 
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream) {
+                     int nviews, int layout, float* proj, hipStream_t stream) {
     long long su, sv, sview;
     out_strides(g, layout, &su, &sv, &sview);
     TIGRE_REQUIRE(b.elements() < (1LL << 32) && (long long)b.nbx * b.nby * 125 < (1LL << 24),
@@ -374,8 +376,8 @@ int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGri
     auto kern = (g.accuracy > 1.0) ? (buf ? ax_interp_kernel<true, true> : ax_interp_kernel<true, false>)
                                    : (buf ? ax_interp_kernel<false, true> : ax_interp_kernel<false, false>);
     KernelTimerScope timer(TIGRE_KERNEL_AX_INTERPOLATED, stream);
-    hipLaunchKernelGGL(kern, dim3(ray_blocks(g, g.nangles)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
-                       g.nVoxel[2], b.nbx, b.nby, b.nbz, d_views, g.nangles, g.nDetec[0], g.nDetec[1], su, sv, sview,
+    hipLaunchKernelGGL(kern, dim3(ray_blocks(g, nviews)), dim3(256), 0, stream, bricked, g.nVoxel[0], g.nVoxel[1],
+                       g.nVoxel[2], b.nbx, b.nby, b.nbz, d_views, nviews, g.nDetec[0], g.nDetec[1], su, sv, sview,
                        proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;

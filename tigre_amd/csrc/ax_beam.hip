@@ -62,12 +62,14 @@ __device__ __forceinline__ float corner_max(float x) {
 // Extent of the beam over principal planes lo .. hi: the beam's lines are
 // A(p) = cA + kA p, z(p) = cz + kz p (corner ray c in lane c; the other rows of
 // the wavefront hold copies), the voxels of the planes span P coordinates
-// [lo, hi + 1], and a margin of one voxel covers rounding.  Clipped to -1 .. n
-// (the walk never leaves it); a0 is aligned down to 4 for 16-byte rows.
+// [lo, hi + 1], and a margin of one voxel covers rounding; `tap` = 1 adds the
+// upper interpolation tap.  Clipped to -1 .. n (no walk or tap leaves it); a0
+// is aligned down to 4 for 16-byte rows.
 struct BoxExtent {
     int a0, a1, z0, z1;
 };
-__device__ inline BoxExtent beam_extent(int lo, int hi, float cA, float kA, float cz, float kz, int nA, int nz) {
+__device__ inline BoxExtent beam_extent(int lo, int hi, float cA, float kA, float cz, float kz, int nA, int nz,
+                                        int tap) {
     const float p0 = (float)lo, p1 = (float)(hi + 1);
     const float A0 = fmaf(kA, p0, cA), A1 = fmaf(kA, p1, cA);
     const float Z0 = fmaf(kz, p0, cz), Z1 = fmaf(kz, p1, cz);
@@ -79,9 +81,9 @@ __device__ inline BoxExtent beam_extent(int lo, int hi, float cA, float kA, floa
     zmax = fminf(fmaxf(zmax, -2.0f), (float)nz + 1.0f);
     BoxExtent e;
     e.a0 = max((int)floorf(amin) - 1, -1) & ~3;
-    e.a1 = min((int)floorf(amax) + 1, nA);
+    e.a1 = min((int)floorf(amax) + 1 + tap, nA);
     e.z0 = max((int)floorf(zmin) - 1, -1);
-    e.z1 = min((int)floorf(zmax) + 1, nz);
+    e.z1 = min((int)floorf(zmax) + 1 + tap, nz);
     return e;
 }
 
@@ -182,14 +184,86 @@ struct SweepShared {
     int ext[2];              // box extents na (multiple of 4) and nzb, maxed over the slabs
 };
 
-template <int kP>
+// Per-ray state of the interpolated projector (ray_interpolated_projection.cu
+// :117-206 restated): samples m = 0 .. count - 1 at q + m f (voxel-centre index
+// frame), already clipped to the ones inside the zero-bordered volume.
+struct RaySamples {
+    float qP, qA, qz, fP, fA, fz;
+    int m, count;
+    float sum;
+};
+
+// Trilinear (kMode 1) or nearest (kMode 2) sample of a box whose fastest axis
+// is A: `src` holds planes P, rows z, columns A with strides SP and SZ; (pc, zc,
+// ac) is the lower tap relative to the box origin.
+template <int kMode>
+__device__ __forceinline__ float box_sample(const float* __restrict__ src, int SP, int SZ, float P, float A, float Z,
+                                            int P0, int Z0, int A0) {
+    // tap indices relative to the box origin are >= 0 and < 2^24, strides < 2^24
+    if (kMode == 2) {
+        const unsigned pc = (unsigned)((int)floorf(P + 0.5f) - P0), zc = (unsigned)((int)floorf(Z + 0.5f) - Z0),
+                       ac = (unsigned)((int)floorf(A + 0.5f) - A0);
+        return src[__umul24(pc, (unsigned)SP) + __umul24(zc, (unsigned)SZ) + ac];
+    }
+    const float flP = floorf(P), flA = floorf(A), flZ = floorf(Z);
+    const float wP = P - flP, wA = A - flA, wZ = Z - flZ;
+    const unsigned o = __umul24((unsigned)((int)flP - P0), (unsigned)SP) +
+                       __umul24((unsigned)((int)flZ - Z0), (unsigned)SZ) + (unsigned)((int)flA - A0);
+    const float c00 = fmaf(wA, src[o + 1] - src[o], src[o]);
+    const float c10 = fmaf(wA, src[o + SZ + 1] - src[o + SZ], src[o + SZ]);
+    const float c01 = fmaf(wA, src[o + SP + 1] - src[o + SP], src[o + SP]);
+    const float c11 = fmaf(wA, src[o + SP + SZ + 1] - src[o + SP + SZ], src[o + SP + SZ]);
+    const float c0 = fmaf(wZ, c10 - c00, c00), c1 = fmaf(wZ, c11 - c01, c01);
+    return fmaf(wP, c1 - c0, c0);
+}
+
+// Samples of one slab (cells lo .. hi along P, box origin (a0, z0), LDS or
+// global `src`); stops at the first sample whose cell lies beyond the slab.
+// Returns true when the ray has no samples left.
+template <int kMode>
+__device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ src, int SP, int SZ, int P0, int Z0,
+                                   int A0, int lo, int hi, int dir, float hP, float hA, float hz, int guard) {
+    int m = r.m;
+    const int count = r.count;
+    const float qP = r.qP, qA = r.qA, qz = r.qz, fP = r.fP, fA = r.fA, fz = r.fz;
+    float sum = r.sum;
+    bool done = false;
+    for (; guard > 0; --guard) {
+        if (m >= count) {
+            done = true;
+            break;
+        }
+        const float fm = (float)m;
+        // positions stay in [-1, n] (rounding at the clipped ends), so the
+        // taps stay in the zero-bordered range
+        const float P = __builtin_amdgcn_fmed3f(fmaf(fm, fP, qP), -1.0f, hP);
+        const int pc = (int)floorf(kMode == 2 ? P + 0.5f : P);
+        if (dir > 0 ? pc > hi : pc < lo) break;
+        const float A = __builtin_amdgcn_fmed3f(fmaf(fm, fA, qA), -1.0f, hA);
+        const float Z = __builtin_amdgcn_fmed3f(fmaf(fm, fz, qz), -1.0f, hz);
+        sum += box_sample<kMode>(src, SP, SZ, P, A, Z, P0, Z0, A0);
+        ++m;
+    }
+    r.m = m;
+    r.sum = sum;
+    return done;
+}
+
+// One workgroup: the rays of a 16 x 16 detector tile in up to four views.
+// kMode 0: Siddon; 1: trilinear interpolation; 2: nearest-voxel sampling.
+template <int kP, int kMode>
 __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGeom& sg, int nx, int ny, int nz,
                                   const FwdView* __restrict__ views, const BeamGroup& grp, int tile_u, int tile_v,
                                   int nU, int nV, long long su, long long sv, long long sview,
-                                  float* __restrict__ out, double dvx, double dvy, double dvz, SweepShared& sh) {
+                                  float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
+                                  SweepShared& sh) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nP = kP == 0 ? nx : ny, nA = kP == 0 ? ny : nx;
     const int dir = grp.dir;
+    // Siddon visits principal voxels -1 .. nP; samples have lower taps in cells
+    // -1 .. nP - 1 and upper taps one plane further
+    const int last_cell = kMode == 0 ? nP : nP - 1;
+    const int tap = kMode == 0 ? 0 : 1;
 
     // ---- the lane's ray (FP64 set-up) ----
     const int slot = wave >> 2;
@@ -199,6 +273,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     const FwdView& w = views[grp.first + min(slot, grp.count - 1)];
     RayWalk r;
     r.sum = 0.0f;
+    RaySamples rs;
+    rs.sum = 0.0f;
     int state = 0;  // 0: no ray or finished, 1: waiting for the slab of its entry, 2: walking
     int ip0 = 0, ia0 = 0, iz0 = 0, sx = 0, sy = 0, sz = 0;
     double len = 0.0;
@@ -207,35 +283,75 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
         double rx = w.org[0] + u * w.du[0] + pv * w.dv[0] - w.src[0];
         double ry = w.org[1] + u * w.du[1] + pv * w.dv[1] - w.src[1];
         double rz = w.org[2] + u * w.du[2] + pv * w.dv[2] - w.src[2];
-        // components below 1e-3 voxel are treated as exactly zero (:155-159)
-        if (fabs(rx) < 1e-3) rx = 0.0;
-        if (fabs(ry) < 1e-3) ry = 0.0;
-        if (fabs(rz) < 1e-3) rz = 0.0;
-        len = sqrt(rx * rx * dvx * dvx + ry * ry * dvy * dvy + rz * rz * dvz * dvz);  // mm (:241, :267)
-        double am = -INFINITY, aM = INFINITY;
-        bool hit = slab(w.src[0], rx, nx, am, aM);
-        hit = hit && slab(w.src[1], ry, ny, am, aM);
-        hit = hit && slab(w.src[2], rz, nz, am, aM);
-        if (hit && am < aM) {  // a ray that misses the volume projects to 0 (:179-180)
-            const Axis ax = axis_setup(w.src[0], rx, am);
-            const Axis ay = axis_setup(w.src[1], ry, am);
-            const Axis az = axis_setup(w.src[2], rz, am);
-            r.tx = r.t0x = ax.t0;
-            r.ty = r.t0y = ay.t0;
-            r.tz = r.t0z = az.t0;
-            r.dtx = ax.dt;
-            r.dty = ay.dt;
-            r.dtz = az.dt;
-            r.kx = r.ky = r.kz = 0;
-            r.T = (float)(aM - am);
-            r.tc = 0.0f;
-            ip0 = kP == 0 ? ax.idx : ay.idx;
-            ia0 = kP == 0 ? ay.idx : ax.idx;
-            iz0 = az.idx;
-            sx = ax.step;
-            sy = ay.step;
-            sz = az.step;
-            state = 1;
+        if (kMode == 0) {
+            // components below 1e-3 voxel are treated as exactly zero (:155-159)
+            if (fabs(rx) < 1e-3) rx = 0.0;
+            if (fabs(ry) < 1e-3) ry = 0.0;
+            if (fabs(rz) < 1e-3) rz = 0.0;
+            len = sqrt(rx * rx * dvx * dvx + ry * ry * dvy * dvy + rz * rz * dvz * dvz);  // mm (:241, :267)
+            double am = -INFINITY, aM = INFINITY;
+            bool hit = slab(w.src[0], rx, nx, am, aM);
+            hit = hit && slab(w.src[1], ry, ny, am, aM);
+            hit = hit && slab(w.src[2], rz, nz, am, aM);
+            if (hit && am < aM) {  // a ray that misses the volume projects to 0 (:179-180)
+                const Axis ax = axis_setup(w.src[0], rx, am);
+                const Axis ay = axis_setup(w.src[1], ry, am);
+                const Axis az = axis_setup(w.src[2], rz, am);
+                r.tx = r.t0x = ax.t0;
+                r.ty = r.t0y = ay.t0;
+                r.tz = r.t0z = az.t0;
+                r.dtx = ax.dt;
+                r.dty = ay.dt;
+                r.dtz = az.dt;
+                r.kx = r.ky = r.kz = 0;
+                r.T = (float)(aM - am);
+                r.tc = 0.0f;
+                ip0 = kP == 0 ? ax.idx : ay.idx;
+                ia0 = kP == 0 ? ay.idx : ax.idx;
+                iz0 = az.idx;
+                sx = ax.step;
+                sy = ay.step;
+                sz = az.step;
+                state = 1;
+            }
+        } else {
+            // samples per ray and step (ray_interpolated_projection.cu:164-169)
+            const double n = ceil(sqrt(rx * rx + ry * ry + rz * rz) / acc);
+            if (n >= 1.0 && n < 1e9) {  // else: degenerate ray (pixel at the source), 0
+                const double st[3] = {rx / n, ry / n, rz / n};
+                // sample range [first, last] of the reference (:182-193), clipped to
+                // the samples strictly inside (-1, n) on every axis (the others are
+                // exactly zero)
+                const double last = (w.cap < n) ? ceil(w.cap) : n;
+                double lo = -INFINITY, hi = INFINITY;
+                const int nn[3] = {nx, ny, nz};
+                bool empty = false;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    if (st[a] == 0.0) {
+                        if (!(w.src[a] > -1.0 && w.src[a] < (double)nn[a])) empty = true;
+                    } else {
+                        const double a0 = (-1.0 - w.src[a]) / st[a], a1 = ((double)nn[a] - w.src[a]) / st[a];
+                        lo = fmax(lo, fmin(a0, a1));
+                        hi = fmin(hi, fmax(a0, a1));
+                    }
+                }
+                const double i0 = fmax(w.first, floor(lo));
+                const double i1 = fmin(last, ceil(hi));
+                len = sqrt(st[0] * dvx * st[0] * dvx + st[1] * dvy * st[1] * dvy + st[2] * dvz * st[2] * dvz);
+                if (!empty && i0 <= i1) {
+                    // positions relative to the first sample keep the FP32 loop accurate
+                    rs.qP = (float)(w.src[kP] + i0 * st[kP]);
+                    rs.qA = (float)(w.src[1 - kP] + i0 * st[1 - kP]);
+                    rs.qz = (float)(w.src[2] + i0 * st[2]);
+                    rs.fP = (float)st[kP];
+                    rs.fA = (float)st[1 - kP];
+                    rs.fz = (float)st[2];
+                    rs.m = 0;
+                    rs.count = (int)(i1 - i0) + 1;
+                    state = 2;
+                }
+            }
         }
     }
 
@@ -265,14 +381,14 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     bool staged = false;
     for (int attempt = 0; attempt < 5 && !staged; ++attempt) {
         const int pl_try = attempt == 0 ? kSlabPlanes : (attempt == 1 ? 6 : (attempt == 2 ? 4 : (attempt == 3 ? 2 : 1)));
-        const int ns = (nP + 2 + pl_try - 1) / pl_try;
+        const int ns = (last_cell + 2 + pl_try - 1) / pl_try;
         if (ns > kMaxSlabs) continue;
         if (tid == 0) sh.ext[0] = sh.ext[1] = 0;
         __syncthreads();
         for (int s = wave; s < ns; s += kBeamThreads / 64) {
             int lo, hi;
-            slab_range(s, pl_try, dir, nP, lo, hi);
-            const BoxExtent e = beam_extent(lo, hi, cA, kA, cz, kz, nA, nz);
+            slab_range(s, pl_try, dir, last_cell, lo, hi);
+            const BoxExtent e = beam_extent(lo, hi, cA, kA, cz, kz, nA, nz, tap);
             if (lane == 0) {
                 sh.origin[s] = make_int2(e.a0, e.z0);
                 atomicMax(&sh.ext[0], (e.a1 - e.a0 + 4) & ~3);
@@ -282,7 +398,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
         __syncthreads();
         na = sh.ext[0];
         nzb = sh.ext[1];
-        if (na * nzb * pl_try <= kSlabCap) {
+        if (na * nzb * (pl_try + tap) <= kSlabCap) {
             staged = true;
             planes = pl_try;
             nslabs = ns;
@@ -291,23 +407,32 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     }
 
     if (!staged) {
-        // The beam's box does not fit an LDS buffer: walk from the staging copy.
-        if (state == 1) {
-            r.off = (unsigned)stage_index(sg, ip0, iz0, ia0);
-            r.dX = (unsigned)(sx * (kP == 0 ? sg.NZ * sg.NA : 1));
-            r.dY = (unsigned)(sy * (kP == 1 ? sg.NZ * sg.NA : 1));
-            r.dZ = (unsigned)(sz * sg.NA);
-            r.kexit = -1;  // never reached
-            walk<kP>(r, stage, (nx + ny + nz + 8) / 2);
+        // The beam's box does not fit an LDS buffer: read the staging copy.
+        if (kMode == 0) {
+            if (state == 1) {
+                r.off = (unsigned)stage_index(sg, ip0, iz0, ia0);
+                r.dX = (unsigned)(sx * (kP == 0 ? sg.NZ * sg.NA : 1));
+                r.dY = (unsigned)(sy * (kP == 1 ? sg.NZ * sg.NA : 1));
+                r.dZ = (unsigned)(sz * sg.NA);
+                r.kexit = -1;  // never reached
+                walk<kP>(r, stage, (nx + ny + nz + 8) / 2);
+            }
+        } else if (state == 2) {
+            // box origin (-1, -1, -4): the bordered staging copy itself
+            sample_slab<kMode>(rs, stage, sg.NZ * sg.NA, sg.NA, -1, -1, -4, -1, last_cell, dir, (float)nP, (float)nA,
+                               (float)nz, 1 << 30);
         }
     } else {
         const int SP = nzb * na;
-        // offset steps inside a box
-        r.dX = (unsigned)(sx * (kP == 0 ? SP : 1));
-        r.dY = (unsigned)(sy * (kP == 1 ? SP : 1));
-        r.dZ = (unsigned)(sz * na);
+        const int splanes = planes + tap;  // staged planes per slab
+        if (kMode == 0) {
+            // offset steps inside a box
+            r.dX = (unsigned)(sx * (kP == 0 ? SP : 1));
+            r.dY = (unsigned)(sy * (kP == 1 ? SP : 1));
+            r.dZ = (unsigned)(sz * na);
+        }
         // this thread's quads of a box: row (plane * nzb + z) and 4-float column
-        const int nq4 = na >> 2, nq = planes * nzb * nq4;
+        const int nq4 = na >> 2, nq = splanes * nzb * nq4;
         unsigned long long qoff[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -321,10 +446,10 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
         // at buf[4 q]; the LDS base of each instruction is uniform per wavefront.
         auto fetch = [&](int s, float* dst) {
             int lo, hi;
-            slab_range(s, planes, dir, nP, lo, hi);
+            slab_range(s, planes, dir, last_cell, lo, hi);
             const int2 o = sh.origin[s];
             const unsigned long long base = stage_index(sg, lo, o.y, o.x);
-            const int nqs = (hi - lo + 1) * nzb * nq4;
+            const int nqs = (hi - lo + 1 + tap) * nzb * nq4;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 if (wave_first + j * kBeamThreads < nqs) {
@@ -341,20 +466,27 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
         for (int s = 0; s < nslabs; ++s) {
             if (s + 1 < nslabs) fetch(s + 1, sh.buf + ((s + 1) & 1) * kSlabCap);
             int lo, hi;
-            slab_range(s, planes, dir, nP, lo, hi);
+            slab_range(s, planes, dir, last_cell, lo, hi);
             const int2 o = sh.origin[s];
-            if (state == 2) {
-                // paused at the first plane of this slab: re-base the offset
-                r.off += (unsigned)(-((lo - plo) * SP + (o.y - pz0) * na + (o.x - pa0)));
-                r.kexit += planes;
-            } else if (state == 1 && ip0 >= lo && ip0 <= hi) {
-                r.off = (unsigned)(((ip0 - lo) * nzb + (iz0 - o.y)) * na + (ia0 - o.x));
-                r.kexit = dir > 0 ? hi + 1 - ip0 : ip0 - lo + 1;
-                state = 2;
-            }
-            if (state == 2) {
-                // crossings inside one slab: its planes plus the box's a and z planes
-                if (walk<kP>(r, sh.buf + (s & 1) * kSlabCap, (planes + na + nzb + 5) / 2)) state = 0;
+            const float* cur = sh.buf + (s & 1) * kSlabCap;
+            if (kMode == 0) {
+                if (state == 2) {
+                    // paused at the first plane of this slab: re-base the offset
+                    r.off += (unsigned)(-((lo - plo) * SP + (o.y - pz0) * na + (o.x - pa0)));
+                    r.kexit += planes;
+                } else if (state == 1 && ip0 >= lo && ip0 <= hi) {
+                    r.off = (unsigned)(((ip0 - lo) * nzb + (iz0 - o.y)) * na + (ia0 - o.x));
+                    r.kexit = dir > 0 ? hi + 1 - ip0 : ip0 - lo + 1;
+                    state = 2;
+                }
+                if (state == 2) {
+                    // crossings inside one slab: its planes plus the box's a and z planes
+                    if (walk<kP>(r, cur, (planes + na + nzb + 5) / 2)) state = 0;
+                }
+            } else if (state == 2) {
+                if (sample_slab<kMode>(rs, cur, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP, (float)nA, (float)nz,
+                                       1 << 30))
+                    state = 0;
             }
             plo = lo;
             pa0 = o.x;
@@ -366,16 +498,18 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
 
     if (active) {
         float* dst = out + (long long)w.pad[0] * sview + (long long)u * su + (long long)v * sv;
-        *dst = (float)((double)r.sum * len);
+        *dst = (float)((double)(kMode == 0 ? r.sum : rs.sum) * len);
     }
 }
 
 // Two workgroups per CU (8 wavefronts per SIMD): the register budget is held
 // to that occupancy, which the slab barriers need to keep the SIMDs busy.
-__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_siddon_beam_kernel(
+template <int kMode>
+__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_beam_kernel(
     const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
     const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
-    long long su, long long sv, long long sview, float* __restrict__ out, double dvx, double dvy, double dvz) {
+    long long su, long long sv, long long sview, float* __restrict__ out, double acc, double dvx, double dvy,
+    double dvz) {
     __shared__ SweepShared sh;
     const int tilesU = (nU + kBeamTile - 1) / kBeamTile, tilesV = (nV + kBeamTile - 1) / kBeamTile;
     const int tiles = tilesU * tilesV;
@@ -387,12 +521,12 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     sg.NZ = nz + 2;
     if (grp.principal == 0) {
         sg.NA = na_x + 8;
-        beam_sweep<0>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview, out,
-                      dvx, dvy, dvz, sh);
+        beam_sweep<0, kMode>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
+                             out, acc, dvx, dvy, dvz, sh);
     } else {
         sg.NA = na_y + 8;
-        beam_sweep<1>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview, out,
-                      dvx, dvy, dvz, sh);
+        beam_sweep<1, kMode>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
+                             out, acc, dvx, dvy, dvz, sh);
     }
 }
 
@@ -401,8 +535,9 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
 // x-fastest volume.  64 x 64 tiles of (p, a) per z row go through LDS, so both
 // the reads (x fastest) and the writes (a fastest) are coalesced whichever of
 // p, a is x.
-__global__ __launch_bounds__(256) void stage_volume_kernel(const float* __restrict__ vol, int nx, int ny, int nz,
-                                                           int principal, int na_pad, float* __restrict__ out) {
+__global__ __launch_bounds__(256) void stage_volume_kernel(const void* __restrict__ vol, int dtype, int nx, int ny,
+                                                           int nz, int principal, int na_pad,
+                                                           float* __restrict__ out) {
     __shared__ float tile[64][65];
     const int NP = (principal == 0 ? nx : ny) + 2, NZ = nz + 2, NA = na_pad + 8;
     const int a_base = blockIdx.x * 64 - 4, p_base = blockIdx.y * 64 - 1, z = (int)blockIdx.z - 1;
@@ -419,7 +554,7 @@ __global__ __launch_bounds__(256) void stage_volume_kernel(const float* __restri
         }
         float val = 0.0f;
         if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-            val = vol[((long long)z * ny + y) * nx + x];
+            val = load_storage(vol, ((long long)z * ny + y) * nx + x, dtype);
         if (principal == 0)
             tile[r][tx] = val;   // tile[a][p]
         else
@@ -444,21 +579,24 @@ long long stage_elements(const HostGeometry& g, int principal) {
     return (long long)(nP + 2) * (g.nVoxel[2] + 2) * (stage_pad(nA) + 8);
 }
 
-int launch_stage_volume(const float* vol, const HostGeometry& g, int principal, float* stage, hipStream_t stream) {
+int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int principal, float* stage,
+                        hipStream_t stream) {
     const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
     const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
     const int NA = stage_pad(nA) + 8;
     dim3 grid((unsigned)((NA + 63) / 64), (unsigned)((nP + 2 + 63) / 64), (unsigned)(g.nVoxel[2] + 2));
     TIGRE_REQUIRE(grid.y <= 65535u && grid.z <= 65535u, "volume too large for the staging copy");
     KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
-    hipLaunchKernelGGL(stage_volume_kernel, grid, dim3(256), 0, stream, vol, g.nVoxel[0], g.nVoxel[1], g.nVoxel[2],
+    hipLaunchKernelGGL(stage_volume_kernel, grid, dim3(256), 0, stream, vol, dtype, g.nVoxel[0], g.nVoxel[1],
+                       g.nVoxel[2],
                        principal, stage_pad(nA), stage);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }
 
-int launch_ax_siddon_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                          const int* d_groups, int ngroups, int layout, float* proj, hipStream_t stream) {
+int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
+                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj,
+                   hipStream_t stream) {
     const long long nU = g.nDetec[0], nV = g.nDetec[1];
     long long su = 1, sv = nU;
     if (layout == TIGRE_LAYOUT_MATLAB) {
@@ -471,11 +609,14 @@ int launch_ax_siddon_beam(const float* stage_x, const float* stage_y, const Host
     // LDS offsets are 32-bit; global element offsets of the staging copy too
     TIGRE_REQUIRE(stage_elements(g, 0) < (1LL << 32) && stage_elements(g, 1) < (1LL << 32),
                   "volume too large for 32-bit staging offsets");
-    KernelTimerScope timer(TIGRE_KERNEL_AX_SIDDON, stream);
-    hipLaunchKernelGGL(ax_siddon_beam_kernel, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y,
-                       g.nVoxel[0], g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
+    const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
+    // point filtering when accuracy > 1 (ray_interpolated_projection.cu:589-595)
+    auto kern = !interp ? ax_beam_kernel<0> : (g.accuracy > 1.0 ? ax_beam_kernel<2> : ax_beam_kernel<1>);
+    KernelTimerScope timer(interp ? TIGRE_KERNEL_AX_INTERPOLATED : TIGRE_KERNEL_AX_SIDDON, stream);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y, g.nVoxel[0],
+                       g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
                        reinterpret_cast<const BeamGroup*>(d_groups), ngroups, g.nDetec[0], g.nDetec[1], su, sv,
-                       nU * nV, proj, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

@@ -62,7 +62,7 @@ int tigre_ax(const float* img, const TigreGeometry* geo, float** result, const f
     TIGRE_CHECK(call.alloc(&d_img, nvox));
     TIGRE_CHECK(call.alloc(&d_proj, P * (size_t)nangles));
     TIGRE_HIP_TRY(hipMemcpyAsync(d_img, img, nvox * sizeof(float), hipMemcpyHostToDevice, call.stream));
-    TIGRE_CHECK(ax_device(d_img, g, d_proj, projection_type, layout, call.stream));
+    TIGRE_CHECK(ax_device(d_img, g, d_proj, projection_type, layout, TIGRE_DTYPE_F32, call.stream));
     for (int i = 0; i < nangles; ++i)
         TIGRE_HIP_TRY(hipMemcpyAsync(result[i], d_proj + P * (size_t)i, P * sizeof(float), hipMemcpyDeviceToHost,
                                      call.stream));
@@ -83,7 +83,7 @@ int tigre_atb(const float* projections, const TigreGeometry* geo, float* result,
     TIGRE_CHECK(call.alloc(&d_proj, np));
     TIGRE_CHECK(call.alloc(&d_vol, nvox));
     TIGRE_HIP_TRY(hipMemcpyAsync(d_proj, projections, np * sizeof(float), hipMemcpyHostToDevice, call.stream));
-    TIGRE_CHECK(atb_device(d_proj, g, d_vol, backprojection_type, layout, call.stream));
+    TIGRE_CHECK(atb_device(d_proj, g, d_vol, backprojection_type, layout, TIGRE_DTYPE_F32, call.stream));
     TIGRE_HIP_TRY(hipMemcpyAsync(result, d_vol, nvox * sizeof(float), hipMemcpyDeviceToHost, call.stream));
     return call.finish();
 }
@@ -102,6 +102,46 @@ int tigre_pocs_tv(const float* img, float* dst, float alpha, const long* image_s
     TIGRE_CHECK(call.alloc(&d_b, n));
     TIGRE_HIP_TRY(hipMemcpyAsync(d_a, img, n * sizeof(float), hipMemcpyHostToDevice, call.stream));
     TIGRE_CHECK(run_pocs_tv(d_a, d_b, alpha, image_size[0], image_size[1], image_size[2], maxIter, call.stream));
+    TIGRE_HIP_TRY(hipMemcpyAsync(dst, d_b, n * sizeof(float), hipMemcpyDeviceToHost, call.stream));
+    return call.finish();
+}
+
+int tigre_aw_pocs_tv(const float* img, float* dst, float alpha, const long* image_size, int maxIter, float delta,
+                     const int* gpuids, int ngpus) {
+    set_error("");
+    TIGRE_REQUIRE(img && dst && image_size, "null argument");
+    TIGRE_REQUIRE(image_size[0] > 0 && image_size[1] > 0 && image_size[2] > 0, "image size must be positive");
+    TIGRE_REQUIRE(maxIter >= 0, "maxIter must be non-negative");
+    HostCall call;
+    TIGRE_CHECK(call.begin(gpuids, ngpus));
+    const size_t n = (size_t)image_size[0] * image_size[1] * image_size[2];
+    float *d_a = nullptr, *d_b = nullptr;
+    TIGRE_CHECK(call.alloc(&d_a, n));
+    TIGRE_CHECK(call.alloc(&d_b, n));
+    TIGRE_HIP_TRY(hipMemcpyAsync(d_a, img, n * sizeof(float), hipMemcpyHostToDevice, call.stream));
+    TIGRE_CHECK(run_aw_pocs_tv(d_a, d_b, alpha, delta, image_size[0], image_size[1], image_size[2], maxIter,
+                               call.stream));
+    TIGRE_HIP_TRY(hipMemcpyAsync(dst, d_b, n * sizeof(float), hipMemcpyDeviceToHost, call.stream));
+    return call.finish();
+}
+
+int tigre_piccs_tv(const float* img, const float* prior, float* dst, float alpha, float ratio, const long* image_size,
+                   int maxIter, const int* gpuids, int ngpus) {
+    set_error("");
+    TIGRE_REQUIRE(img && prior && dst && image_size, "null argument");
+    TIGRE_REQUIRE(image_size[0] > 0 && image_size[1] > 0 && image_size[2] > 0, "image size must be positive");
+    TIGRE_REQUIRE(maxIter >= 0, "maxIter must be non-negative");
+    HostCall call;
+    TIGRE_CHECK(call.begin(gpuids, ngpus));
+    const size_t n = (size_t)image_size[0] * image_size[1] * image_size[2];
+    float *d_a = nullptr, *d_p = nullptr, *d_b = nullptr;
+    TIGRE_CHECK(call.alloc(&d_a, n));
+    TIGRE_CHECK(call.alloc(&d_p, n));
+    TIGRE_CHECK(call.alloc(&d_b, n));
+    TIGRE_HIP_TRY(hipMemcpyAsync(d_a, img, n * sizeof(float), hipMemcpyHostToDevice, call.stream));
+    TIGRE_HIP_TRY(hipMemcpyAsync(d_p, prior, n * sizeof(float), hipMemcpyHostToDevice, call.stream));
+    TIGRE_CHECK(run_piccs_tv(d_a, d_p, d_b, alpha, ratio, image_size[0], image_size[1], image_size[2], maxIter,
+                             call.stream));
     TIGRE_HIP_TRY(hipMemcpyAsync(dst, d_b, n * sizeof(float), hipMemcpyDeviceToHost, call.stream));
     return call.finish();
 }
@@ -131,7 +171,16 @@ int tigre_ax_device(const float* d_img, const TigreGeometry* geo, float* d_proj,
     TIGRE_REQUIRE(d_img != nullptr && d_proj != nullptr, "null device pointer");
     HostGeometry g;
     TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
-    return ax_device(d_img, g, d_proj, projection_type, layout, (hipStream_t)stream);
+    return ax_device(d_img, g, d_proj, projection_type, layout, TIGRE_DTYPE_F32, (hipStream_t)stream);
+}
+
+int tigre_ax_device_typed(const void* d_img, const TigreGeometry* geo, void* d_proj, const float* angles,
+                          int nangles, int projection_type, int layout, int dtype, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_img != nullptr && d_proj != nullptr, "null device pointer");
+    HostGeometry g;
+    TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    return ax_device(d_img, g, d_proj, projection_type, layout, dtype, (hipStream_t)stream);
 }
 
 int tigre_atb_device(const float* d_proj, const TigreGeometry* geo, float* d_vol, const float* angles, int nangles,
@@ -140,7 +189,16 @@ int tigre_atb_device(const float* d_proj, const TigreGeometry* geo, float* d_vol
     TIGRE_REQUIRE(d_proj != nullptr && d_vol != nullptr, "null device pointer");
     HostGeometry g;
     TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
-    return atb_device(d_proj, g, d_vol, backprojection_type, layout, (hipStream_t)stream);
+    return atb_device(d_proj, g, d_vol, backprojection_type, layout, TIGRE_DTYPE_F32, (hipStream_t)stream);
+}
+
+int tigre_atb_device_typed(const void* d_proj, const TigreGeometry* geo, void* d_vol, const float* angles,
+                           int nangles, int backprojection_type, int layout, int dtype, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_proj != nullptr && d_vol != nullptr, "null device pointer");
+    HostGeometry g;
+    TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    return atb_device(d_proj, g, d_vol, backprojection_type, layout, dtype, (hipStream_t)stream);
 }
 
 int tigre_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size, int maxIter,
@@ -151,6 +209,26 @@ int tigre_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const lo
     TIGRE_REQUIRE(maxIter >= 0, "maxIter must be non-negative");
     return run_pocs_tv(d_img, d_dst, alpha, image_size[0], image_size[1], image_size[2], maxIter,
                        (hipStream_t)stream);
+}
+
+int tigre_aw_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size, int maxIter,
+                            float delta, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_img && d_dst && image_size, "null argument");
+    TIGRE_REQUIRE(image_size[0] > 0 && image_size[1] > 0 && image_size[2] > 0, "image size must be positive");
+    TIGRE_REQUIRE(maxIter >= 0, "maxIter must be non-negative");
+    return run_aw_pocs_tv(d_img, d_dst, alpha, delta, image_size[0], image_size[1], image_size[2], maxIter,
+                          (hipStream_t)stream);
+}
+
+int tigre_piccs_tv_device(const float* d_img, const float* d_prior, float* d_dst, float alpha, float ratio,
+                          const long* image_size, int maxIter, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_img && d_prior && d_dst && image_size, "null argument");
+    TIGRE_REQUIRE(image_size[0] > 0 && image_size[1] > 0 && image_size[2] > 0, "image size must be positive");
+    TIGRE_REQUIRE(maxIter >= 0, "maxIter must be non-negative");
+    return run_piccs_tv(d_img, d_prior, d_dst, alpha, ratio, image_size[0], image_size[1], image_size[2], maxIter,
+                        (hipStream_t)stream);
 }
 
 int tigre_tvdenoising_device(const float* d_src, float* d_dst, float lambda, const float* spacing,

@@ -149,33 +149,54 @@ BrickGrid siddon_bricks(const HostGeometry& g);
 BrickGrid interp_bricks(const HostGeometry& g);
 
 // Launchers (one per operator; all enqueue on `stream`).
-int launch_brick_volume(const float* vol, const HostGeometry& g, const BrickGrid& b, float* bricked,
+int launch_brick_volume(const void* vol, int dtype, const HostGeometry& g, const BrickGrid& b, float* bricked,
                         hipStream_t stream);
 int launch_ax_siddon(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int nviews, int layout, float* proj, hipStream_t stream);
 // Beam-staged Siddon (ax_beam.hip).  Staging copy for principal axis P
 // (0 = x, 1 = y): [p][z][a] with a zero border, a = the other horizontal axis.
 long long stage_elements(const HostGeometry& g, int principal);
-int launch_stage_volume(const float* vol, const HostGeometry& g, int principal, float* stage, hipStream_t stream);
-// d_groups: ngroups records {first view, view count (1..4), principal, direction}.
-int launch_ax_siddon_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                          const int* d_groups, int ngroups, int layout, float* proj, hipStream_t stream);
+int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int principal, float* stage,
+                        hipStream_t stream);
+// Beam-staged forward projection (Siddon or interpolated); d_groups: ngroups
+// records {first view, view count (1..4), principal, direction}.
+int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
+                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj,
+                   hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
-                     int layout, float* proj, hipStream_t stream);
-int launch_quad_projections(const float* proj, int nU, int nV, int nviews, int layout, float* quads,
+                     int nviews, int layout, float* proj, hipStream_t stream);
+int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nviews, int layout, float* quads,
                             hipStream_t stream);
 int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched,
                bool axial, float* vol, hipStream_t stream);
 int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
                 hipStream_t stream);
+int run_aw_pocs_tv(const float* img, float* dst, float alpha, float delta, long nx, long ny, long nz, int maxiter,
+                   hipStream_t stream);
+int run_piccs_tv(const float* img, const float* prior, float* dst, float alpha, float ratio, long nx, long ny, long nz,
+                 int maxiter, hipStream_t stream);
 int run_tvdenoise(const float* src, float* dst, float lambda, const float* spacing, long nx, long ny,
                   long nz, int maxiter, hipStream_t stream);
 
 // Operator drivers on device pointers (ops.cpp).
-int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int projection_type, int layout,
+int ax_device(const void* d_img, const HostGeometry& g, void* d_proj, int projection_type, int layout, int dtype,
               hipStream_t stream);
-int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int backprojection_type,
-               int layout, hipStream_t stream);
+int atb_device(const void* d_proj, const HostGeometry& g, void* d_vol, int backprojection_type, int layout,
+               int dtype, hipStream_t stream);
+
+// Storage types (TIGRE_DTYPE_*): element i of a buffer of `dtype`, widened to
+// float.  Used by the re-layout kernels, so every operator computes in float.
+__device__ __forceinline__ float load_storage(const void* __restrict__ p, long long i, int dtype) {
+    if (dtype == TIGRE_DTYPE_BF16) return __uint_as_float((unsigned)static_cast<const unsigned short*>(p)[i] << 16);
+    if (dtype == TIGRE_DTYPE_F16) return (float)static_cast<const _Float16*>(p)[i];
+    return static_cast<const float*>(p)[i];
+}
+inline size_t storage_bytes(int dtype) { return dtype == TIGRE_DTYPE_F32 ? 4 : 2; }
+inline bool valid_dtype(int dtype) {
+    return dtype == TIGRE_DTYPE_F32 || dtype == TIGRE_DTYPE_BF16 || dtype == TIGRE_DTYPE_F16;
+}
+// float -> dtype, round to nearest even (storage.hip)
+int launch_store_converted(const float* src, void* dst, long long n, int dtype, hipStream_t stream);
 
 // Clamp into [-1, hi]; NaN maps to -1 (fmaxf returns the non-NaN operand), so
 // a clamped coordinate always addresses the zero-padded border or the data.

@@ -40,30 +40,24 @@ BeamClass beam_class(const HostGeometry& g, const FwdView& w) {
 
 }  // namespace
 
-int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int projection_type, int layout,
-              hipStream_t stream) {
+int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int projection_type, int layout,
+              int dtype, hipStream_t stream) {
     TIGRE_REQUIRE(projection_type == TIGRE_PROJ_SIDDON || projection_type == TIGRE_PROJ_INTERPOLATED,
                   "unknown projection type");
     TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
+    TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
+    // narrow storage: the kernels write float projections, rounded at the end
+    const long long nproj = (long long)g.nDetec[0] * g.nDetec[1] * g.nangles;
+    float* d_proj = static_cast<float*>(d_proj_out);
+    if (dtype != TIGRE_DTYPE_F32)
+        TIGRE_HIP_TRY(hipMallocAsync((void**)&d_proj, (size_t)nproj * sizeof(float), stream));
     const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
     std::vector<FwdView> views;
     forward_views(g, interp, &views);
-    if (interp) {
-        void* d_views = nullptr;
-        TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(FwdView), stream));
-        const BrickGrid b = interp_bricks(g);
-        float* bricked = nullptr;
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
-        TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
-        TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views, layout, d_proj, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
-        return TIGRE_OK;
-    }
 
-    // Siddon: beam-eligible views grouped by (principal, direction), in
-    // rotation-angle order inside a class, four per workgroup; the others go
-    // through the gather-driven kernel.  pad[0] keeps each view's output index.
+    // Beam-eligible views grouped by (principal, direction), in rotation-angle
+    // order inside a class, up to four per workgroup; the others go through the
+    // gather-driven kernels.  pad[0] keeps each view's output index.
     std::vector<int> cls(views.size());
     std::vector<int> beam, rest;
     for (size_t i = 0; i < views.size(); ++i) {
@@ -120,32 +114,45 @@ int ax_device(const float* d_img, const HostGeometry& g, float* d_proj, int proj
             for (int gi = 0; gi < ngroups; ++gi) used = used || groups[4 * gi + 2] == a;
             if (!used) continue;
             TIGRE_HIP_TRY(hipMallocAsync((void**)&stage[a], (size_t)stage_elements(g, a) * sizeof(float), stream));
-            TIGRE_CHECK(launch_stage_volume(d_img, g, a, stage[a], stream));
+            TIGRE_CHECK(launch_stage_volume(d_img, dtype, g, a, stage[a], stream));
         }
-        TIGRE_CHECK(launch_ax_siddon_beam(stage[0], stage[1], g, (const FwdView*)d_views, (const int*)d_groups,
-                                          ngroups, layout, d_proj, stream));
+        TIGRE_CHECK(launch_ax_beam(stage[0], stage[1], g, (const FwdView*)d_views, (const int*)d_groups, ngroups,
+                                   projection_type, layout, d_proj, stream));
         for (int a = 0; a < 2; ++a)
             if (stage[a]) TIGRE_HIP_TRY(hipFreeAsync(stage[a], stream));
         TIGRE_HIP_TRY(hipFreeAsync(d_groups, stream));
     }
     if (nbeam < (int)ordered.size()) {
-        const BrickGrid b = siddon_bricks(g);
+        const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
         float* bricked = nullptr;
         TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
-        TIGRE_CHECK(launch_brick_volume(d_img, g, b, bricked, stream));
-        TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views + nbeam, (int)ordered.size() - nbeam,
-                                     layout, d_proj, stream));
+        TIGRE_CHECK(launch_brick_volume(d_img, dtype, g, b, bricked, stream));
+        const int nrest = (int)ordered.size() - nbeam;
+        if (interp)
+            TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views + nbeam, nrest, layout, d_proj,
+                                         stream));
+        else
+            TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views + nbeam, nrest, layout, d_proj,
+                                         stream));
         TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
     }
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
+    if (dtype != TIGRE_DTYPE_F32) {
+        TIGRE_CHECK(launch_store_converted(d_proj, d_proj_out, nproj, dtype, stream));
+        TIGRE_HIP_TRY(hipFreeAsync(d_proj, stream));
+    }
     return TIGRE_OK;
 }
 
-int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int backprojection_type, int layout,
-               hipStream_t stream) {
+int atb_device(const void* d_proj, const HostGeometry& g, void* d_vol_out, int backprojection_type, int layout,
+               int dtype, hipStream_t stream) {
     TIGRE_REQUIRE(backprojection_type == TIGRE_BACKPROJ_FDK || backprojection_type == TIGRE_BACKPROJ_MATCHED,
                   "unknown backprojection type");
     TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
+    TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
+    const long long nvox = (long long)g.nVoxel[0] * g.nVoxel[1] * g.nVoxel[2];
+    float* d_vol = static_cast<float*>(d_vol_out);
+    if (dtype != TIGRE_DTYPE_F32) TIGRE_HIP_TRY(hipMallocAsync((void**)&d_vol, (size_t)nvox * sizeof(float), stream));
     std::vector<BwdView> views;
     backward_views(g, &views);
     const bool axial = backward_is_axial(g, views);
@@ -155,11 +162,15 @@ int atb_device(const float* d_proj, const HostGeometry& g, float* d_vol, int bac
         (size_t)quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * (size_t)g.nangles * 4 * sizeof(float);
     float* quads = nullptr;
     TIGRE_HIP_TRY(hipMallocAsync((void**)&quads, quad_bytes, stream));
-    TIGRE_CHECK(launch_quad_projections(d_proj, g.nDetec[0], g.nDetec[1], g.nangles, layout, quads, stream));
+    TIGRE_CHECK(launch_quad_projections(d_proj, dtype, g.nDetec[0], g.nDetec[1], g.nangles, layout, quads, stream));
     TIGRE_CHECK(launch_atb(quads, g, (const BwdView*)d_views, backprojection_type == TIGRE_BACKPROJ_MATCHED, axial,
                            d_vol, stream));
     TIGRE_HIP_TRY(hipFreeAsync(quads, stream));
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
+    if (dtype != TIGRE_DTYPE_F32) {
+        TIGRE_CHECK(launch_store_converted(d_vol, d_vol_out, nvox, dtype, stream));
+        TIGRE_HIP_TRY(hipFreeAsync(d_vol, stream));
+    }
     return TIGRE_OK;
 }
 

@@ -1,7 +1,8 @@
 // TV steps for gfx950: POCS-TV steepest descent (minimizeTV) and the ROF
 // primal-dual denoiser.
 //
-// POCS-TV, GD_TV.cu:489-597.  One iteration is two passes over the volume:
+// POCS-TV, GD_TV.cu:489-597 (and its adaptive-weighted and PICCS variants,
+// GD_AwTV.cu, PICCS.cu).  One iteration is two passes over the volume:
 //   pass 1  g = grad TV(f) at every voxel, sum of g^2 in FP64 per workgroup;
 //   pass 2  every workgroup sums the partials in a fixed order (the same
 //           result in every workgroup and every run), then writes
@@ -20,35 +21,76 @@ namespace {
 constexpr int kTvBlocks = 1024;
 constexpr int kTvThreads = 256;
 
-// Volume value with the z zero padding of the single-device pocs_tv layout
-// (image between two zero slices, GD_TV.cu:367-368,456-465).
-__device__ inline float fz(const float* __restrict__ f, long nx, long ny, long nz, long z, long y, long x) {
-    return (z < 0 || z >= nz) ? 0.0f : f[(z * ny + y) * nx + x];
-}
+// Voxel loaders: the image, or the image minus a prior (PICCS).
+struct Plain {
+    const float* __restrict__ f;
+    __device__ float operator()(long i) const { return f[i]; }
+};
+struct MinusPrior {
+    const float* __restrict__ f;
+    const float* __restrict__ p;
+    __device__ float operator()(long i) const { return f[i] - p[i]; }
+};
 
 // Backward differences at (z, y, x) (GD_TV.cu:95-115): y/x terms dropped at the
-// low edge, z always has a (possibly zero) neighbour.
-__device__ inline void bdiff(const float* __restrict__ f, long nx, long ny, long nz, long z, long y, long x,
-                             float d[3]) {
-    const float c = fz(f, nx, ny, nz, z, y, x);
-    d[0] = c - fz(f, nx, ny, nz, z - 1, y, x);
-    d[1] = (y >= 1) ? c - fz(f, nx, ny, nz, z, y - 1, x) : 0.0f;
-    d[2] = (x >= 1) ? c - fz(f, nx, ny, nz, z, y, x - 1) : 0.0f;
+// low edge.  kPadZ: the image sits between zero slices (the single-device
+// pocs_tv / aw_pocs_tv layout, GD_TV.cu:367-368,456-465), so z always has a
+// (possibly zero) neighbour; otherwise (PICCS.cu:109-128) the z term is
+// dropped at the low edge like the others.
+template <bool kPadZ, class L>
+__device__ inline void bdiff(const L& f, long nx, long ny, long nz, long z, long y, long x, float d[3]) {
+    const long i = (z * ny + y) * nx + x;
+    const bool in = z >= 0 && z < nz;
+    const float c = in ? f(i) : 0.0f;
+    if (kPadZ)
+        d[0] = c - ((z >= 1 && z - 1 < nz) ? f(i - nx * ny) : 0.0f);
+    else
+        d[0] = (z >= 1) ? c - f(i - nx * ny) : 0.0f;
+    d[1] = (y >= 1 && in) ? c - f(i - nx) : 0.0f;
+    d[2] = (x >= 1 && in) ? c - f(i - 1) : 0.0f;
 }
 
-// TV sub-gradient at one voxel, GD_TV.cu:117-143.
-__device__ inline float tv_grad(const float* __restrict__ f, long nx, long ny, long nz, long z, long y, long x) {
-    const float eps = 1e-8f;
-    float df[3], di[3] = {0.f, 0.f, 0.f}, dj[3] = {0.f, 0.f, 0.f}, dk[3];
-    bdiff(f, nx, ny, nz, z, y, x, df);
-    if (x + 1 < nx) bdiff(f, nx, ny, nz, z, y, x + 1, di);
-    if (y + 1 < ny) bdiff(f, nx, ny, nz, z, y + 1, x, dj);
-    bdiff(f, nx, ny, nz, z + 1, y, x, dk);
-    float g = (df[0] + df[1] + df[2]) / (sqrtf(df[0] * df[0] + df[1] * df[1] + df[2] * df[2]) + eps);
-    g -= di[2] / (sqrtf(di[0] * di[0] + di[1] * di[1] + di[2] * di[2]) + eps);
-    g -= dj[1] / (sqrtf(dj[0] * dj[0] + dj[1] * dj[1] + dj[2] * dj[2]) + eps);
-    g -= dk[0] / (sqrtf(dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2]) + eps);
+// (Weighted) TV sub-gradient at one voxel, GD_TV.cu:117-143 / GD_AwTV.cu:117-170
+// / PICCS.cu:132-157.  kWeighted: each difference component d carries the
+// weight exp(-(d / delta)^2) (AwTV; the reference uses the fast __expf, here
+// the accurate expf).
+template <bool kPadZ, bool kWeighted, class L>
+__device__ inline float tv_grad(const L& f, long nx, long ny, long nz, long z, long y, long x, float eps,
+                                float inv_delta) {
+    float df[3], di[3] = {0.f, 0.f, 0.f}, dj[3] = {0.f, 0.f, 0.f}, dk[3] = {0.f, 0.f, 0.f};
+    bdiff<kPadZ>(f, nx, ny, nz, z, y, x, df);
+    if (x + 1 < nx) bdiff<kPadZ>(f, nx, ny, nz, z, y, x + 1, di);
+    if (y + 1 < ny) bdiff<kPadZ>(f, nx, ny, nz, z, y + 1, x, dj);
+    if (kPadZ || z + 1 < nz) bdiff<kPadZ>(f, nx, ny, nz, z + 1, y, x, dk);
+    auto w = [&](float d) { return kWeighted ? expf(-(d * inv_delta) * (d * inv_delta)) : 1.0f; };
+    const float w0 = w(df[0]), w1 = w(df[1]), w2 = w(df[2]);
+    const float wi0 = w(di[0]), wi1 = w(di[1]), wi2 = w(di[2]);
+    const float wj0 = w(dj[0]), wj1 = w(dj[1]), wj2 = w(dj[2]);
+    const float wk0 = w(dk[0]), wk1 = w(dk[1]), wk2 = w(dk[2]);
+    float g = (w0 * df[0] + w1 * df[1] + w2 * df[2]) /
+              (sqrtf(w0 * df[0] * df[0] + w1 * df[1] * df[1] + w2 * df[2] * df[2]) + eps);
+    g -= wi2 * di[2] / (sqrtf(wi0 * di[0] * di[0] + wi1 * di[1] * di[1] + wi2 * di[2] * di[2]) + eps);
+    g -= wj1 * dj[1] / (sqrtf(wj0 * dj[0] * dj[0] + wj1 * dj[1] * dj[1] + wj2 * dj[2] * dj[2]) + eps);
+    g -= wk0 * dk[0] / (sqrtf(wk0 * dk[0] * dk[0] + wk1 * dk[1] * dk[1] + wk2 * dk[2] * dk[2]) + eps);
     return g;
+}
+
+// Descent direction of one iteration at one voxel.  kKind 0: TV (GD_TV),
+// 1: adaptive-weighted TV (GD_AwTV), 2: PICCS, (1 - ratio) grad TV(f) + ratio
+// grad TV(f - prior) (PICCS.cu:322-347).
+struct TvParams {
+    const float* prior;
+    float inv_delta, ratio;
+};
+
+template <int kKind>
+__device__ inline float tv_direction(const float* __restrict__ f, const TvParams& tp, long nx, long ny, long nz,
+                                     long z, long y, long x) {
+    if (kKind == 0) return tv_grad<true, false>(Plain{f}, nx, ny, nz, z, y, x, 1e-8f, 0.0f);
+    if (kKind == 1) return tv_grad<true, true>(Plain{f}, nx, ny, nz, z, y, x, 1e-8f, tp.inv_delta);
+    const float a = tv_grad<false, false>(Plain{f}, nx, ny, nz, z, y, x, 1e-6f, 0.0f);
+    const float b = tv_grad<false, false>(MinusPrior{f, tp.prior}, nx, ny, nz, z, y, x, 1e-6f, 0.0f);
+    return (1.0f - tp.ratio) * a + tp.ratio * b;
 }
 
 // Workgroup sum of one double per lane (wavefront shuffles, then LDS).
@@ -66,39 +108,45 @@ __device__ inline double block_sum(double v) {
 }
 }  // namespace
 
-__global__ __launch_bounds__(kTvThreads) void tv_norm_kernel(const float* __restrict__ f, long nx, long ny, long nz,
-                                                             double* __restrict__ partial) {
+template <int kKind>
+__global__ __launch_bounds__(kTvThreads) void tv_norm_kernel(const float* __restrict__ f, TvParams tp, long nx, long ny,
+                                                             long nz, double* __restrict__ partial) {
     const long n = nx * ny * nz;
     double s = 0.0;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         const long x = q % nx, y = (q / nx) % ny, z = q / (nx * ny);
-        const float g = tv_grad(f, nx, ny, nz, z, y, x);
+        const float g = tv_direction<kKind>(f, tp, nx, ny, nz, z, y, x);
         s += (double)g * (double)g;
     }
     s = block_sum(s);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// f' = f - alpha g / sqrt(sum g^2 + norm_eps); norm_eps = 1e-7 for TV and AwTV
+// (GD_TV.cu:561), 0 for PICCS (PICCS.cu:370).
+template <int kKind>
 __global__ __launch_bounds__(kTvThreads) void tv_step_kernel(const float* __restrict__ f, float* __restrict__ out,
-                                                             long nx, long ny, long nz,
+                                                             TvParams tp, long nx, long ny, long nz,
                                                              const double* __restrict__ partial, int npartial,
-                                                             float alpha) {
+                                                             float alpha, double norm_eps) {
     __shared__ float s_scale;
     double s = 0.0;
     for (int i = threadIdx.x; i < npartial; i += blockDim.x) s += partial[i];
     s = block_sum(s);
-    if (threadIdx.x == 0) s_scale = (float)((double)alpha / sqrt(s + 1e-7));
+    if (threadIdx.x == 0) s_scale = (float)((double)alpha / sqrt(s + norm_eps));
     __syncthreads();
     const float scale = s_scale;
     const long n = nx * ny * nz;
     for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
         const long x = q % nx, y = (q / nx) % ny, z = q / (nx * ny);
-        out[q] = f[q] - tv_grad(f, nx, ny, nz, z, y, x) * scale;
+        out[q] = f[q] - tv_direction<kKind>(f, tp, nx, ny, nz, z, y, x) * scale;
     }
 }
 
-int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
-                hipStream_t stream) {
+namespace {
+template <int kKind>
+int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp, double norm_eps, long nx, long ny,
+                   long nz, int maxiter, hipStream_t stream) {
     const size_t n = (size_t)nx * ny * nz;
     if (maxiter <= 0) {
         if (dst != img) TIGRE_HIP_TRY(hipMemcpyAsync(dst, img, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
@@ -114,9 +162,10 @@ int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, lon
     for (int it = 0; it < maxiter; ++it) {
         float* o = (it == maxiter - 1 && dst != img) ? dst : buf[cur];
         KernelTimerScope timer(TIGRE_KERNEL_POCS_TV, stream);
-        hipLaunchKernelGGL(tv_norm_kernel, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, nx, ny, nz, partial);
-        hipLaunchKernelGGL(tv_step_kernel, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, o, nx, ny, nz,
-                           partial, kTvBlocks, alpha);
+        hipLaunchKernelGGL(tv_norm_kernel<kKind>, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, tp, nx, ny, nz,
+                           partial);
+        hipLaunchKernelGGL(tv_step_kernel<kKind>, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, o, tp, nx, ny,
+                           nz, partial, kTvBlocks, alpha, norm_eps);
         TIGRE_HIP_TRY(hipGetLastError());
         src = o;
         cur ^= 1;
@@ -126,6 +175,24 @@ int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, lon
     TIGRE_HIP_TRY(hipFreeAsync(buf[1], stream));
     TIGRE_HIP_TRY(hipFreeAsync(partial, stream));
     return TIGRE_OK;
+}
+}  // namespace
+
+int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
+                hipStream_t stream) {
+    return run_tv_descent<0>(img, dst, alpha, TvParams{nullptr, 0.0f, 0.0f}, 1e-7, nx, ny, nz, maxiter, stream);
+}
+
+int run_aw_pocs_tv(const float* img, float* dst, float alpha, float delta, long nx, long ny, long nz, int maxiter,
+                   hipStream_t stream) {
+    TIGRE_REQUIRE(delta != 0.0f, "delta must be nonzero");
+    return run_tv_descent<1>(img, dst, alpha, TvParams{nullptr, 1.0f / delta, 0.0f}, 1e-7, nx, ny, nz, maxiter,
+                             stream);
+}
+
+int run_piccs_tv(const float* img, const float* prior, float* dst, float alpha, float ratio, long nx, long ny, long nz,
+                 int maxiter, hipStream_t stream) {
+    return run_tv_descent<2>(img, dst, alpha, TvParams{prior, 0.0f, ratio}, 0.0, nx, ny, nz, maxiter, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -9,7 +9,9 @@ and Atb.py:9-40 (via _Ax.pyx:32-101 and _Atb.pyx:31-90):
 ``img`` is (nZ, nY, nX) float32, projections are (nangles, nV, nU) float32.
 NumPy inputs follow the reference contract (host in, host out).  torch
 tensors that live on a GPU stay there: the operator runs on the current
-stream and returns a device tensor, with no host round trip.
+stream and returns a device tensor, with no host round trip.  Device tensors
+may also be stored in bfloat16 or float16: the result has the input's storage
+type, and all arithmetic is float32.
 """
 from __future__ import annotations
 
@@ -64,6 +66,16 @@ def _check_mode(geo):
         raise NotImplementedError("tigre_amd implements cone-beam operators only (geo.mode=%r)" % (geo.mode,))
 
 
+def _storage_code(t):
+    """Storage type code of a device tensor: float32, bfloat16 or float16."""
+    import torch
+
+    codes = {torch.float32: _native.DTYPE_F32, torch.bfloat16: _native.DTYPE_BF16, torch.float16: _native.DTYPE_F16}
+    if t.dtype not in codes:
+        raise TypeError("Input data should be float32 (or, on the GPU, bfloat16 / float16), not " + str(t.dtype))
+    return codes[t.dtype]
+
+
 def _stream_handle():
     import torch
 
@@ -77,8 +89,7 @@ def Ax(img, geo, angles, projection_type="Siddon", **kwargs):
     if device:
         import torch
 
-        if img.dtype != torch.float32:
-            raise TypeError("Input data should be float32, not " + str(img.dtype))
+        code = _storage_code(img)
         shape = tuple(img.shape)
     else:
         if img.dtype != np.float32:
@@ -99,11 +110,11 @@ def Ax(img, geo, angles, projection_type="Siddon", **kwargs):
         import torch
 
         src = img.contiguous()
-        out = torch.empty((n, nv, nu), dtype=torch.float32, device=img.device)
+        out = torch.empty((n, nv, nu), dtype=img.dtype, device=img.device)
         with torch.cuda.device(img.device):
-            st = lib.tigre_ax_device(ctypes.c_void_p(src.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                     ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON,
-                                     _stream_handle())
+            st = lib.tigre_ax_device_typed(ctypes.c_void_p(src.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
+                                           ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON,
+                                           code, _stream_handle())
         _native.check(st, "TIGRE: Call to Ax failed")
         return out
     src = np.ascontiguousarray(img)
@@ -126,8 +137,7 @@ def Atb(projections, geo, angles, backprojection_type="FDK", **kwargs):
     if device:
         import torch
 
-        if projections.dtype != torch.float32:
-            raise TypeError("Input data should be float32, not " + str(projections.dtype))
+        code = _storage_code(projections)
     else:
         if projections.dtype != np.float32:
             raise TypeError("Input data should be float32, not " + str(projections.dtype))
@@ -148,11 +158,11 @@ def Atb(projections, geo, angles, backprojection_type="FDK", **kwargs):
         import torch
 
         src = projections.contiguous()
-        out = torch.empty(vshape, dtype=torch.float32, device=projections.device)
+        out = torch.empty(vshape, dtype=projections.dtype, device=projections.device)
         with torch.cuda.device(projections.device):
-            st = lib.tigre_atb_device(ctypes.c_void_p(src.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                      ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON,
-                                      _stream_handle())
+            st = lib.tigre_atb_device_typed(ctypes.c_void_p(src.data_ptr()), cg.ptr(),
+                                            ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p), n,
+                                            kind, _native.LAYOUT_PYTHON, code, _stream_handle())
         _native.check(st, "Atb:")
         return out
     src = np.ascontiguousarray(projections)

@@ -1,4 +1,5 @@
-"""TV steps: ``minTV`` / ``minimizeTV`` (POCS-TV steepest descent) and
+"""TV steps: ``minTV`` / ``minimizeTV`` (POCS-TV steepest descent), its
+adaptive-weighted and PICCS variants ``AwminTV`` / ``minPICCS``, and
 ``tvdenoise`` / ``im3ddenoise`` (ROF primal-dual denoise).
 
 Signatures follow Python/tigre/utilities/cuda_interface/_minTV.pyx:39-64,
@@ -14,7 +15,7 @@ import numpy as np
 from . import _native
 from .operators import _gpu_ids, _is_device_tensor, _stream_handle
 
-__all__ = ["minTV", "minimizeTV", "tvdenoise", "im3ddenoise"]
+__all__ = ["minTV", "minimizeTV", "AwminTV", "minPICCS", "tvdenoise", "im3ddenoise"]
 
 
 def _sizes(shape):
@@ -51,6 +52,62 @@ def minTV(src, alpha=15.0, maxiter=100, gpuids=None):
 def minimizeTV(img, dtvg=1.0, ng=20, gpuids=None):
     """MATLAB-style name (MATLAB/Utilities/minimizeTV.m:23-39): ng steps of size dtvg."""
     return minTV(img, dtvg, ng, gpuids=gpuids)
+
+
+def AwminTV(src, alpha=15.0, maxiter=100, delta=-0.005, gpuids=None):
+    """Adaptive-weighted TV steps (_AwminTV.pyx:34-59, GD_AwTV.cu): as minTV with
+    each difference d weighted by exp(-(d / delta)^2)."""
+    lib = _native.lib()
+    if _is_device_tensor(src):
+        import torch
+
+        if src.dtype != torch.float32 or src.dim() != 3:
+            raise TypeError("AwminTV expects a 3D float32 tensor")
+        x = src.contiguous()
+        out = torch.empty_like(x)
+        with torch.cuda.device(x.device):
+            st = lib.tigre_aw_pocs_tv_device(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                             float(alpha), _sizes(x.shape), int(maxiter), float(delta),
+                                             _stream_handle())
+        _native.check(st, "TIGRE:Call to aw_pocs_tv failed")
+        return out
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    out = np.empty_like(src)
+    ids, nids = _gpu_ids(gpuids)
+    st = lib.tigre_aw_pocs_tv(src.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), float(alpha),
+                              _sizes(src.shape), int(maxiter), float(delta), ids, nids)
+    _native.check(st, "TIGRE:Call to aw_pocs_tv failed")
+    return out
+
+
+def minPICCS(src, prior, alpha=15.0, maxiter=100, ratio=0.95, gpuids=None):
+    """PICCS steps (_minPICCS.pyx:34-66, PICCS.cu): descent along
+    (1 - ratio) grad TV(img) + ratio grad TV(img - prior)."""
+    lib = _native.lib()
+    if _is_device_tensor(src):
+        import torch
+
+        x, p = src.contiguous(), prior.contiguous()
+        if x.dtype != torch.float32 or p.dtype != torch.float32 or x.shape != p.shape or x.dim() != 3:
+            raise TypeError("minPICCS expects two 3D float32 tensors of one shape")
+        out = torch.empty_like(x)
+        with torch.cuda.device(x.device):
+            st = lib.tigre_piccs_tv_device(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(p.data_ptr()),
+                                           ctypes.c_void_p(out.data_ptr()), float(alpha), float(ratio),
+                                           _sizes(x.shape), int(maxiter), _stream_handle())
+        _native.check(st, "minimizeTV:PICCS_TV:")
+        return out
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    prior = np.ascontiguousarray(prior, dtype=np.float32)
+    if src.shape != prior.shape or src.ndim != 3:
+        raise ValueError("minPICCS expects two 3D arrays of one shape")
+    out = np.empty_like(src)
+    ids, nids = _gpu_ids(gpuids)
+    st = lib.tigre_piccs_tv(src.ctypes.data_as(ctypes.c_void_p), prior.ctypes.data_as(ctypes.c_void_p),
+                            out.ctypes.data_as(ctypes.c_void_p), float(alpha), float(ratio), _sizes(src.shape),
+                            int(maxiter), ids, nids)
+    _native.check(st, "minimizeTV:PICCS_TV:")
+    return out
 
 
 def tvdenoise(src, maxiter=100, lamda=15.0, gpuids=None, spacing=(1.0, 1.0, 1.0)):

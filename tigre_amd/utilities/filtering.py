@@ -51,7 +51,10 @@ def filtering(proj, geo, angles, parker=False, verbose=False):
     import torch
 
     if parker:
-        raise NotImplementedError("Parker short-scan weights are not implemented")
+        # as the reference: weights applied to the (views, u, v) transpose
+        from .parkerweight import parkerweight
+
+        proj = parkerweight(proj.transpose(1, 2).contiguous(), geo, angles, parker).transpose(1, 2).contiguous()
     n_u = int(geo.nDetector[1])
     filt, filt_len = ramp_filter(n_u, getattr(geo, "filter", None))
     pad = int((filt_len - n_u) // 2)
