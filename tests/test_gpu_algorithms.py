@@ -83,3 +83,23 @@ def test_fdk_offset_detector(cuda):
         err, oracle.rel_l2(got, img), oracle.rel_l2(plain, img)))
     assert err <= 1e-5
     assert oracle.rel_l2(got, img) < oracle.rel_l2(plain, img)
+
+
+@pytest.mark.parametrize("variant", ["awasd_pocs", "piccs", "os_asd_pocs"])
+def test_asd_pocs_variants_run(cuda, variant):
+    """The ASD-POCS family: each variant reduces the data residual well below
+    that of the zero image (the TV step follows the non-negativity clamp, so
+    small negative values are expected)."""
+    geo = cone_geometry((32, 32, 32), (32.0, 32.0, 32.0), (40, 40), (1.6, 1.6), 1000.0, 1536.0)
+    angles = np.linspace(0, 2 * np.pi, 40, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 5, seed=51)
+    gp = prepared(geo, angles)
+    proj = oracle.ax(img, gp).astype(np.float32)
+    kw = {"prior": img * 0.9} if variant == "piccs" else {}
+    if variant == "os_asd_pocs":
+        kw["blocksize"] = 10
+    rec = getattr(algs, variant)(proj, geo, angles, 3, **kw)
+    r0 = np.linalg.norm(proj)
+    r = np.linalg.norm(oracle.ax(rec.astype(np.float32), gp) - proj)
+    print("%s residual %.3e of %.3e" % (variant, r, r0))
+    assert r < 0.5 * r0

@@ -1,7 +1,8 @@
 """Reconstruction algorithms on device-resident data.
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
-``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``cgls``, ``asd_pocs``.
+``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``cgls``, ``asd_pocs``,
+``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -24,7 +25,8 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "cgls", "asd_pocs", "ViewSets"]
+__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "cgls", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs",
+           "piccs", "os_piccs", "ViewSets"]
 
 
 def _torch():
@@ -328,34 +330,48 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
     b = _to_device(proj).to(dt)
     x = torch.zeros(vs.vol_shape, dtype=dt, device="cuda") if init is None else _to_device(init).to(dt)
 
+    # Narrow storage keeps the search direction normalised, p = n_p p_hat with
+    # the scalar n_p in FP64: A p can exceed the float16 range (A^T A has a
+    # norm in the thousands), A p_hat does not.  Same iteration as the plain
+    # form in exact arithmetic.
+    scaled = dt != torch.float32
+
     def start():
         r = torch.sub(b, vs.ax(x, 0))
         p = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
-        return r, p, _norm(p) ** 2
+        n_p = _norm(p)
+        if scaled:
+            p = torch.div(p, n_p)
+            return r, p, n_p, n_p ** 2
+        return r, p, 1.0, n_p ** 2
 
-    r, p, gamma = start()
+    r, p, n_p, gamma = start()
     l2 = np.zeros((1, niter))
     reinit_at = 0
     i = 0
     while i < niter:
         q = vs.ax(p, 0)
-        alpha = gamma / (_norm(q) ** 2)
-        torch.add(x, p, alpha=alpha, out=x)
+        alpha = gamma / ((n_p * _norm(q)) ** 2)
+        step = alpha * n_p
+        torch.add(x, p, alpha=step, out=x)
         l2[0, i] = _norm(torch.sub(b, vs.ax(x, 0)))
         if i > 0 and l2[0, i] > l2[0, i - 1]:
-            torch.sub(x, p, alpha=alpha, out=x)
+            torch.sub(x, p, alpha=step, out=x)
             if verbose:
                 print("re-initilization of CGLS called at iteration:" + str(i))
             if reinit_at + 1 == i or not restart:
                 print("CGLS exited due to divergence.")
                 break
             reinit_at = i
-            r, p, gamma = start()
+            r, p, n_p, gamma = start()
             continue
-        torch.sub(r, q, alpha=alpha, out=r)
+        torch.sub(r, q, alpha=step, out=r)
         s = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
         gamma1 = _norm(s) ** 2
-        p = torch.add(s, p, alpha=gamma1 / gamma)
+        p = torch.add(s, p, alpha=gamma1 / gamma * n_p)
+        if scaled:
+            n_p = _norm(p)
+            p = torch.div(p, n_p)
         gamma = gamma1
         i += 1
     res = _result(x, return_tensor)
@@ -367,17 +383,18 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
 # ---------------------------------------------------------------------------
 def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, tviter=20, maxl2err=None,
              lmbda=1.0, lmbda_red=1.0, OrderStrategy=None, verbose=False, return_tensor=False, regularizer=None,
-             **kwargs):
+             blocksize=1, **kwargs):
     """Adaptive steepest descent - projection onto convex sets.
 
-    Each outer iteration: one SART sweep with one view per subset and a
-    non-negativity clamp, then ``tviter`` TV steepest-descent steps of size
-    dtvg with the step adapted as in the reference; stops on the reference's
-    criteria.  ``regularizer`` (default: minTV) may replace the TV step with
-    any callable (image_tensor, dtvg) -> image_tensor."""
+    Each outer iteration: one SART sweep (``blocksize`` views per subset; 1 for
+    ASD-POCS, 20 by default for the OS variants) with a non-negativity clamp,
+    then ``tviter`` TV steepest-descent steps of size dtvg with the step
+    adapted as in the reference; stops on the reference's criteria.
+    ``regularizer`` (default: minTV) may replace the TV step with any callable
+    (image_tensor, dtvg) -> image_tensor."""
     from ..tv import minTV
 
-    st = _SartState(proj, geo, angles, 1, order=OrderStrategy, lmbda=lmbda, noneg=True, verbose=verbose)
+    st = _SartState(proj, geo, angles, blocksize, order=OrderStrategy, lmbda=lmbda, noneg=True, verbose=verbose)
     full = ViewSets(st.geo, st.angles, [np.arange(len(st.angles))])
     if maxl2err is None:
         epsilon = 0.2 * _norm(full.ax(fdk(st.proj, st.geo, st.angles, return_tensor=True), 0) - st.proj)
@@ -413,3 +430,40 @@ def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, t
             break
     res = _result(st.res, return_tensor)
     return (res, stats) if kwargs.get("return_stats", False) else res
+
+
+def os_asd_pocs(proj, geo, angles, niter, blocksize=20, **kwargs):
+    """Ordered-subsets ASD-POCS (pocs_algorithms.py OS_ASD_POCS): blocksize 20."""
+    return asd_pocs(proj, geo, angles, niter, blocksize=blocksize, **kwargs)
+
+
+def awasd_pocs(proj, geo, angles, niter, delta=-0.005, tviter=20, blocksize=1, **kwargs):
+    """ASD-POCS with the adaptive-weighted (edge preserving) TV step AwminTV
+    (pocs_algorithms.py AwASD_POCS; delta controls the smoothing at edges)."""
+    from ..tv import AwminTV
+
+    reg = lambda img, step: AwminTV(img, step, tviter, delta)  # noqa: E731
+    return asd_pocs(proj, geo, angles, niter, tviter=tviter, regularizer=reg, blocksize=blocksize, **kwargs)
+
+
+def os_awasd_pocs(proj, geo, angles, niter, blocksize=20, **kwargs):
+    """Ordered-subsets AwASD-POCS (pocs_algorithms.py OS_AwASD_POCS)."""
+    return awasd_pocs(proj, geo, angles, niter, blocksize=blocksize, **kwargs)
+
+
+def piccs(proj, geo, angles, niter, prior=None, prior_ratio=0.2, tviter=20, blocksize=1, **kwargs):
+    """Prior-image constrained compressed sensing (pocs_algorithms.py PICCS):
+    the ASD-POCS loop with the TV step minPICCS(img, prior, dtvg, tviter,
+    prior_ratio)."""
+    from ..tv import minPICCS
+
+    if prior is None:
+        raise ValueError("PICCS needs a prior image (prior=...)")
+    pr = _to_device(prior)
+    reg = lambda img, step: minPICCS(img, pr, step, tviter, prior_ratio)  # noqa: E731
+    return asd_pocs(proj, geo, angles, niter, tviter=tviter, regularizer=reg, blocksize=blocksize, **kwargs)
+
+
+def os_piccs(proj, geo, angles, niter, blocksize=20, **kwargs):
+    """Ordered-subsets PICCS (pocs_algorithms.py OS_PICCS)."""
+    return piccs(proj, geo, angles, niter, blocksize=blocksize, **kwargs)
