@@ -16,6 +16,8 @@
 #ifndef TIGRE_AMD_H
 #define TIGRE_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -179,6 +181,43 @@ int tigre_get_gpu_name(int device, char* name, int len);
 #define TIGRE_KERNEL_COUNT           7
 int tigre_kernel_timing(int enable);
 int tigre_kernel_time(int kernel, double* total_ms, int* launches);
+
+/* ---------------------------------------------------------------------- */
+/* Projection plans                                                       */
+/* ---------------------------------------------------------------------- */
+
+/* A plan holds the per-view records of one geometry and view list, built
+ * and uploaded once on the current device; its launches allocate nothing,
+ * synchronise nothing and upload nothing, so a caller can capture them in a
+ * HIP graph.  They replace the per-call setup of Ax/Atb inside an iterative
+ * method (the subset loop of iterative_recon_alg.py:326-345, whose Ax/Atb
+ * calls rebuild the geometry on every subset).  Tunables: views_per_group
+ * (1, 2, 4) beam-workgroup views, max_slab_planes (1..8) LDS slab depth,
+ * slices_per_thread (4, 8, 16) backprojector slices per lane.  Results do not
+ * depend on views_per_group or max_slab_planes; slices_per_thread changes the
+ * backprojection by rounding only. */
+typedef struct TigrePlan_* TigrePlan;
+int tigre_plan_create(const TigreGeometry* geo, const float* angles, int nangles, int views_per_group,
+                      int max_slab_planes, int slices_per_thread, TigrePlan* out);
+/* Scratch bytes every launch of this plan needs for storage type dtype
+ * (the workspace must be 256-byte aligned, device memory of the plan's device). */
+int tigre_plan_workspace_size(TigrePlan plan, int dtype, size_t* bytes);
+/* tigre_ax_device_typed / tigre_atb_device_typed on the plan's views. */
+int tigre_plan_ax(TigrePlan plan, const void* d_img, void* d_proj, int projection_type, int layout, int dtype,
+                  void* d_workspace, size_t workspace_bytes, void* stream);
+int tigre_plan_atb(TigrePlan plan, const void* d_proj, void* d_vol, int backprojection_type, int layout, int dtype,
+                   void* d_workspace, size_t workspace_bytes, void* stream);
+/* SART image update of one subset, fused into the backprojector
+ * (iterative_recon_alg.py:389-399 and the clip at :344):
+ * x = x + inv_v * Atb(residual), then max(x, 0) when noneg; inv_v = lambda / V,
+ * an nVoxelY x nVoxelX image broadcast over z.  FP32 storage. */
+int tigre_plan_sart_update(TigrePlan plan, const float* d_residual, float* d_x, const float* d_inv_v,
+                           int backprojection_type, int layout, int noneg, void* d_workspace, size_t workspace_bytes,
+                           void* stream);
+/* SART residual (iterative_recon_alg.py:395): ax = W * (proj - ax), n elements,
+ * 16-byte aligned buffers. */
+int tigre_sart_residual(const float* d_proj, const float* d_w, float* d_ax, long long n, void* stream);
+int tigre_plan_destroy(TigrePlan plan);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* tigre_last_error(void);

@@ -26,7 +26,7 @@ from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
 __all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "cgls", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs",
-           "piccs", "os_piccs", "ViewSets"]
+           "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
 
 
 def _torch():
@@ -467,3 +467,6 @@ def piccs(proj, geo, angles, niter, prior=None, prior_ratio=0.2, tviter=20, bloc
 def os_piccs(proj, geo, angles, niter, blocksize=20, **kwargs):
     """Ordered-subsets PICCS (pocs_algorithms.py OS_PICCS)."""
     return piccs(proj, geo, angles, niter, blocksize=blocksize, **kwargs)
+
+
+from .batched import BatchedOSSART  # noqa: E402  (uses _SartState above)

@@ -26,17 +26,19 @@
 namespace tigre {
 
 namespace {
-constexpr int kZ = 16;
-
 // 1/x with the hardware reciprocal (1 ulp), instead of a correctly rounded division.
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 }
 
-template <bool kMatched, bool kAxial>
+// kZ: slices per lane (4, 8 or 16; fewer gives more workgroups on small
+// volumes).  With ep.x set, the result is not stored but applied as the SART
+// update x + inv_v[iy, ix] * result, clamped at 0 when ep.noneg, written to
+// vol (which may alias ep.x: each voxel is read and written by one lane).
+template <bool kMatched, bool kAxial, int kZ>
 __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
-                                                  const BwdView* __restrict__ views, float* __restrict__ vol, int nx,
+                                                  const BwdView* __restrict__ views, float* vol, int nx,
                                                   int ny, int nz, float dvx, float dvy, float dvz, float dU, float dV,
-                                                  float scale) {
+                                                  float scale, AtbUpdate ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -137,10 +139,29 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         }
     }
     const long long plane = (long long)nx * ny;
-    float* out = vol + (long long)z0 * plane + (long long)iy * nx + ix;
+    const long long at = (long long)z0 * plane + (long long)iy * nx + ix;
+    float* out = vol + at;
+    if (ep.x == nullptr) {
 #pragma unroll
-    for (int k = 0; k < kZ; ++k)
-        if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
+        for (int k = 0; k < kZ; ++k)
+            if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
+    } else {
+        // the rounding steps of x + (inv_v * (acc * scale)): three roundings,
+        // no fused multiply-add (contraction is off in this block)
+#pragma clang fp contract(off)
+        const float inv = ep.inv_v[(long long)iy * nx + ix];
+        const float* xin = ep.x + at;
+#pragma unroll
+        for (int k = 0; k < kZ; ++k) {
+            if (z0 + k < nz) {
+                const float b = acc[k] * scale;
+                const float t = inv * b;
+                float v = xin[(long long)k * plane] + t;
+                if (ep.noneg) v = fmaxf(v, 0.0f);
+                out[(long long)k * plane] = v;
+            }
+        }
+    }
 }
 
 // Quad copy of the projections (see above), view-major, rows of quad_det(nU)
@@ -181,25 +202,33 @@ int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nvi
 }
 
 int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched, bool axial,
-               float* vol, hipStream_t stream) {
+               float* vol, int zper, const AtbUpdate& ep, hipStream_t stream) {
+    TIGRE_REQUIRE(zper == 4 || zper == 8 || zper == 16, "slices per lane must be 4, 8 or 16");
     const int nx = g.nVoxel[0], ny = g.nVoxel[1], nz = g.nVoxel[2];
-    dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + kZ - 1) / kZ);
+    dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + zper - 1) / zper);
     TIGRE_REQUIRE(grid.z <= 65535u && grid.y <= 65535u, "volume too large for one launch");
     // matched: final scaling by dVx dVy dVz / (dU dV) (voxel_backprojection2.cu:585)
     const float scale = matched ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
     const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
                         (float)g.dDetec[1], scale, 0.0f};
     KernelTimerScope timer(matched ? TIGRE_KERNEL_ATB_MATCHED : TIGRE_KERNEL_ATB_FDK, stream);
-#define TIGRE_ATB_LAUNCH(M, A)                                                                                   \
-    hipLaunchKernelGGL((atb_kernel<M, A>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],        \
-                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5])
+#define TIGRE_ATB_LAUNCH(M, A, Z)                                                                                \
+    hipLaunchKernelGGL((atb_kernel<M, A, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],     \
+                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5], ep)
+#define TIGRE_ATB_Z(M, A)                                  \
+    do {                                                   \
+        if (zper == 16) TIGRE_ATB_LAUNCH(M, A, 16);        \
+        else if (zper == 8) TIGRE_ATB_LAUNCH(M, A, 8);     \
+        else TIGRE_ATB_LAUNCH(M, A, 4);                    \
+    } while (0)
     if (matched) {
-        if (axial) TIGRE_ATB_LAUNCH(true, true);
-        else TIGRE_ATB_LAUNCH(true, false);
+        if (axial) TIGRE_ATB_Z(true, true);
+        else TIGRE_ATB_Z(true, false);
     } else {
-        if (axial) TIGRE_ATB_LAUNCH(false, true);
-        else TIGRE_ATB_LAUNCH(false, false);
+        if (axial) TIGRE_ATB_Z(false, true);
+        else TIGRE_ATB_Z(false, false);
     }
+#undef TIGRE_ATB_Z
 #undef TIGRE_ATB_LAUNCH
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;

@@ -36,7 +36,7 @@ constexpr int kBeamTile = 16;       // detector pixels per tile edge
 constexpr int kBeamViews = 4;       // views per workgroup
 constexpr int kBeamThreads = 1024;  // kBeamViews * kBeamTile * kBeamTile
 constexpr int kSlabCap = 9216;      // floats per LDS slab buffer (two per workgroup)
-constexpr int kSlabPlanes = 8;      // principal planes per slab, at most
+constexpr int kSlabPlanesMax = 8;   // principal planes per slab, at most (the tunable limit is <= this)
 
 // Minimum / maximum over lanes 0..15 (the 16 corner rays; the other rows of
 // the wavefront hold copies), by four DPP row shifts, returned as a uniform.
@@ -256,7 +256,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                                   const FwdView* __restrict__ views, const BeamGroup& grp, int tile_u, int tile_v,
                                   int nU, int nV, long long su, long long sv, long long sview,
                                   float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
-                                  SweepShared& sh) {
+                                  int max_planes, SweepShared& sh) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int nP = kP == 0 ? nx : ny, nA = kP == 0 ? ny : nx;
     const int dir = grp.dir;
@@ -380,7 +380,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     int planes = 0, nslabs = 0, na = 0, nzb = 0;
     bool staged = false;
     for (int attempt = 0; attempt < 5 && !staged; ++attempt) {
-        const int pl_try = attempt == 0 ? kSlabPlanes : (attempt == 1 ? 6 : (attempt == 2 ? 4 : (attempt == 3 ? 2 : 1)));
+        const int pl_try = attempt == 0 ? max_planes
+                                        : min(max_planes, attempt == 1 ? 6 : (attempt == 2 ? 4 : (attempt == 3 ? 2 : 1)));
         const int ns = (last_cell + 2 + pl_try - 1) / pl_try;
         if (ns > kMaxSlabs) continue;
         if (tid == 0) sh.ext[0] = sh.ext[1] = 0;
@@ -509,7 +510,7 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
     const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
     long long su, long long sv, long long sview, float* __restrict__ out, double acc, double dvx, double dvy,
-    double dvz) {
+    double dvz, int max_planes) {
     __shared__ SweepShared sh;
     const int tilesU = (nU + kBeamTile - 1) / kBeamTile, tilesV = (nV + kBeamTile - 1) / kBeamTile;
     const int tiles = tilesU * tilesV;
@@ -522,11 +523,11 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     if (grp.principal == 0) {
         sg.NA = na_x + 8;
         beam_sweep<0, kMode>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
-                             out, acc, dvx, dvy, dvz, sh);
+                             out, acc, dvx, dvy, dvz, max_planes, sh);
     } else {
         sg.NA = na_y + 8;
         beam_sweep<1, kMode>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
-                             out, acc, dvx, dvy, dvz, sh);
+                             out, acc, dvx, dvy, dvz, max_planes, sh);
     }
 }
 
@@ -595,8 +596,9 @@ int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int p
 }
 
 int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj,
+                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj, int max_planes,
                    hipStream_t stream) {
+    TIGRE_REQUIRE(max_planes >= 1 && max_planes <= kSlabPlanesMax, "slab planes out of range");
     const long long nU = g.nDetec[0], nV = g.nDetec[1];
     long long su = 1, sv = nU;
     if (layout == TIGRE_LAYOUT_MATLAB) {
@@ -616,7 +618,7 @@ int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometr
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y, g.nVoxel[0],
                        g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
                        reinterpret_cast<const BeamGroup*>(d_groups), ngroups, g.nDetec[0], g.nDetec[1], su, sv,
-                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2]);
+                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2], max_planes);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

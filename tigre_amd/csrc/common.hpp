@@ -161,14 +161,22 @@ int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int p
 // Beam-staged forward projection (Siddon or interpolated); d_groups: ngroups
 // records {first view, view count (1..4), principal, direction}.
 int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj,
+                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj, int max_planes,
                    hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int nviews, int layout, float* proj, hipStream_t stream);
 int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nviews, int layout, float* quads,
                             hipStream_t stream);
+// Optional SART epilogue of the backprojector: vol = max?(x + inv_v * Atb, 0),
+// inv_v an nY x nX image broadcast over z (x == nullptr: plain store).
+struct AtbUpdate {
+    const float* x = nullptr;
+    const float* inv_v = nullptr;
+    int noneg = 0;
+};
+// zper: slices per lane, 4, 8 or 16.
 int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched,
-               bool axial, float* vol, hipStream_t stream);
+               bool axial, float* vol, int zper, const AtbUpdate& ep, hipStream_t stream);
 int run_pocs_tv(const float* img, float* dst, float alpha, long nx, long ny, long nz, int maxiter,
                 hipStream_t stream);
 int run_aw_pocs_tv(const float* img, float* dst, float alpha, float delta, long nx, long ny, long nz, int maxiter,
@@ -177,6 +185,34 @@ int run_piccs_tv(const float* img, const float* prior, float* dst, float alpha, 
                  int maxiter, hipStream_t stream);
 int run_tvdenoise(const float* src, float* dst, float lambda, const float* spacing, long nx, long ny,
                   long nz, int maxiter, hipStream_t stream);
+
+// Forward-projection records: view records (beam-grouped views first, the
+// gather kernels' views after them), workgroup groups {first, count,
+// principal, direction}, and which staging copies the groups read.
+struct ForwardRecords {
+    std::vector<FwdView> ordered;
+    std::vector<int> groups;
+    int nbeam = 0;
+    bool stage_used[2] = {false, false};
+    int ngroups() const { return (int)groups.size() / 4; }
+    int nrest() const { return (int)ordered.size() - nbeam; }
+};
+// views_per_group: 1..4 views per beam workgroup.
+void build_forward_records(const HostGeometry& g, bool interp, int views_per_group, ForwardRecords* rec);
+// Scratch of one forward projection (staging copies, bricks), bytes; the
+// layout used by ax_launch: each piece 256-byte aligned, in that order.
+size_t ax_workspace_bytes(const HostGeometry& g, const ForwardRecords& rec, bool interp);
+int ax_launch(const void* d_img, int dtype, const HostGeometry& g, const ForwardRecords& rec, const FwdView* d_views,
+              const int* d_groups, int projection_type, int layout, float* d_proj, char* ws, int max_planes,
+              hipStream_t stream);
+// Scratch of one backprojection (the quad projections), bytes.
+size_t atb_workspace_bytes(const HostGeometry& g);
+int atb_launch(const void* d_proj, int dtype, const HostGeometry& g, const BwdView* d_views, bool axial,
+               int backprojection_type, int layout, float* d_vol, char* ws, int zper, const AtbUpdate& ep,
+               hipStream_t stream);
+
+// r = W (p - Ax) over n elements, written over ax (sart.hip).
+int launch_sart_residual(const float* proj, const float* w, float* ax, long long n, hipStream_t stream);
 
 // Operator drivers on device pointers (ops.cpp).
 int ax_device(const void* d_img, const HostGeometry& g, void* d_proj, int projection_type, int layout, int dtype,

@@ -40,24 +40,12 @@ BeamClass beam_class(const HostGeometry& g, const FwdView& w) {
 
 }  // namespace
 
-int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int projection_type, int layout,
-              int dtype, hipStream_t stream) {
-    TIGRE_REQUIRE(projection_type == TIGRE_PROJ_SIDDON || projection_type == TIGRE_PROJ_INTERPOLATED,
-                  "unknown projection type");
-    TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
-    TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
-    // narrow storage: the kernels write float projections, rounded at the end
-    const long long nproj = (long long)g.nDetec[0] * g.nDetec[1] * g.nangles;
-    float* d_proj = static_cast<float*>(d_proj_out);
-    if (dtype != TIGRE_DTYPE_F32)
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&d_proj, (size_t)nproj * sizeof(float), stream));
-    const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
+void build_forward_records(const HostGeometry& g, bool interp, int views_per_group, ForwardRecords* rec) {
     std::vector<FwdView> views;
     forward_views(g, interp, &views);
-
     // Beam-eligible views grouped by (principal, direction), in rotation-angle
-    // order inside a class, up to four per workgroup; the others go through the
-    // gather-driven kernels.  pad[0] keeps each view's output index.
+    // order inside a class, up to views_per_group per workgroup; the others go
+    // through the gather-driven kernels.  pad[0] keeps each view's output index.
     std::vector<int> cls(views.size());
     std::vector<int> beam, rest;
     for (size_t i = 0; i < views.size(); ++i) {
@@ -87,60 +75,110 @@ int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int pr
         spacing = gaps[gaps.size() / 2];
     }
     const double max_spread = std::max(3.5 * spacing, 0.05);
-    std::vector<FwdView> ordered;
-    ordered.reserve(views.size());
-    std::vector<int> groups;  // {first, count, principal, dir} per workgroup row
+    const size_t per = (size_t)std::min(4, std::max(1, views_per_group));
+    rec->ordered.clear();
+    rec->groups.clear();
+    rec->ordered.reserve(views.size());
     for (size_t k = 0; k < beam.size();) {
         const int c = cls[beam[k]];
         size_t e = k;
-        while (e < beam.size() && e - k < 4 && cls[beam[e]] == c && angle(beam[e]) - angle(beam[k]) <= max_spread)
+        while (e < beam.size() && e - k < per && cls[beam[e]] == c && angle(beam[e]) - angle(beam[k]) <= max_spread)
             ++e;
-        groups.insert(groups.end(), {(int)ordered.size(), (int)(e - k), c / 2, (c & 1) ? 1 : -1});
-        for (size_t j = k; j < e; ++j) ordered.push_back(views[beam[j]]);
+        rec->groups.insert(rec->groups.end(), {(int)rec->ordered.size(), (int)(e - k), c / 2, (c & 1) ? 1 : -1});
+        rec->stage_used[c / 2] = true;
+        for (size_t j = k; j < e; ++j) rec->ordered.push_back(views[beam[j]]);
         k = e;
     }
-    const int nbeam = (int)ordered.size();
-    for (int i : rest) ordered.push_back(views[i]);
-    const int ngroups = (int)groups.size() / 4;
+    rec->nbeam = (int)rec->ordered.size();
+    for (int i : rest) rec->ordered.push_back(views[i]);
+}
 
-    void* d_views = nullptr;
-    TIGRE_CHECK(upload_async(&d_views, ordered.data(), ordered.size() * sizeof(FwdView), stream));
-    if (ngroups > 0) {
-        void* d_groups = nullptr;
-        TIGRE_CHECK(upload_async(&d_groups, groups.data(), groups.size() * sizeof(int), stream));
-        float* stage[2] = {nullptr, nullptr};
-        for (int a = 0; a < 2; ++a) {
-            bool used = false;
-            for (int gi = 0; gi < ngroups; ++gi) used = used || groups[4 * gi + 2] == a;
-            if (!used) continue;
-            TIGRE_HIP_TRY(hipMallocAsync((void**)&stage[a], (size_t)stage_elements(g, a) * sizeof(float), stream));
-            TIGRE_CHECK(launch_stage_volume(d_img, dtype, g, a, stage[a], stream));
-        }
-        TIGRE_CHECK(launch_ax_beam(stage[0], stage[1], g, (const FwdView*)d_views, (const int*)d_groups, ngroups,
-                                   projection_type, layout, d_proj, stream));
-        for (int a = 0; a < 2; ++a)
-            if (stage[a]) TIGRE_HIP_TRY(hipFreeAsync(stage[a], stream));
-        TIGRE_HIP_TRY(hipFreeAsync(d_groups, stream));
-    }
-    if (nbeam < (int)ordered.size()) {
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+}  // namespace
+
+size_t ax_workspace_bytes(const HostGeometry& g, const ForwardRecords& rec, bool interp) {
+    size_t bytes = 0;
+    for (int a = 0; a < 2; ++a)
+        if (rec.stage_used[a]) bytes += align256((size_t)stage_elements(g, a) * sizeof(float));
+    if (rec.nrest() > 0) {
         const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
-        float* bricked = nullptr;
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&bricked, (size_t)b.elements() * sizeof(float), stream));
-        TIGRE_CHECK(launch_brick_volume(d_img, dtype, g, b, bricked, stream));
-        const int nrest = (int)ordered.size() - nbeam;
-        if (interp)
-            TIGRE_CHECK(launch_ax_interp(bricked, g, b, (const FwdView*)d_views + nbeam, nrest, layout, d_proj,
-                                         stream));
-        else
-            TIGRE_CHECK(launch_ax_siddon(bricked, g, b, (const FwdView*)d_views + nbeam, nrest, layout, d_proj,
-                                         stream));
-        TIGRE_HIP_TRY(hipFreeAsync(bricked, stream));
+        bytes += align256((size_t)b.elements() * sizeof(float));
     }
+    return bytes;
+}
+
+int ax_launch(const void* d_img, int dtype, const HostGeometry& g, const ForwardRecords& rec, const FwdView* d_views,
+              const int* d_groups, int projection_type, int layout, float* d_proj, char* ws, int max_planes,
+              hipStream_t stream) {
+    const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
+    float* stage[2] = {nullptr, nullptr};
+    for (int a = 0; a < 2; ++a) {
+        if (!rec.stage_used[a]) continue;
+        stage[a] = reinterpret_cast<float*>(ws);
+        ws += align256((size_t)stage_elements(g, a) * sizeof(float));
+        TIGRE_CHECK(launch_stage_volume(d_img, dtype, g, a, stage[a], stream));
+    }
+    if (rec.ngroups() > 0)
+        TIGRE_CHECK(launch_ax_beam(stage[0], stage[1], g, d_views, d_groups, rec.ngroups(), projection_type, layout,
+                                   d_proj, max_planes, stream));
+    if (rec.nrest() > 0) {
+        const BrickGrid b = interp ? interp_bricks(g) : siddon_bricks(g);
+        float* bricked = reinterpret_cast<float*>(ws);
+        TIGRE_CHECK(launch_brick_volume(d_img, dtype, g, b, bricked, stream));
+        if (interp)
+            TIGRE_CHECK(launch_ax_interp(bricked, g, b, d_views + rec.nbeam, rec.nrest(), layout, d_proj, stream));
+        else
+            TIGRE_CHECK(launch_ax_siddon(bricked, g, b, d_views + rec.nbeam, rec.nrest(), layout, d_proj, stream));
+    }
+    return TIGRE_OK;
+}
+
+int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int projection_type, int layout,
+              int dtype, hipStream_t stream) {
+    TIGRE_REQUIRE(projection_type == TIGRE_PROJ_SIDDON || projection_type == TIGRE_PROJ_INTERPOLATED,
+                  "unknown projection type");
+    TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
+    TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
+    const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
+    // narrow storage: the kernels write float projections, rounded at the end
+    const long long nproj = (long long)g.nDetec[0] * g.nDetec[1] * g.nangles;
+    float* d_proj = static_cast<float*>(d_proj_out);
+    if (dtype != TIGRE_DTYPE_F32)
+        TIGRE_HIP_TRY(hipMallocAsync((void**)&d_proj, (size_t)nproj * sizeof(float), stream));
+    ForwardRecords rec;
+    build_forward_records(g, interp, 4, &rec);
+    void* d_views = nullptr;
+    void* d_groups = nullptr;
+    TIGRE_CHECK(upload_async(&d_views, rec.ordered.data(), rec.ordered.size() * sizeof(FwdView), stream));
+    if (rec.ngroups() > 0)
+        TIGRE_CHECK(upload_async(&d_groups, rec.groups.data(), rec.groups.size() * sizeof(int), stream));
+    char* ws = nullptr;
+    const size_t wsb = ax_workspace_bytes(g, rec, interp);
+    if (wsb) TIGRE_HIP_TRY(hipMallocAsync((void**)&ws, wsb, stream));
+    TIGRE_CHECK(ax_launch(d_img, dtype, g, rec, (const FwdView*)d_views, (const int*)d_groups, projection_type, layout,
+                          d_proj, ws, 8, stream));
+    if (ws) TIGRE_HIP_TRY(hipFreeAsync(ws, stream));
+    if (d_groups) TIGRE_HIP_TRY(hipFreeAsync(d_groups, stream));
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
     if (dtype != TIGRE_DTYPE_F32) {
         TIGRE_CHECK(launch_store_converted(d_proj, d_proj_out, nproj, dtype, stream));
         TIGRE_HIP_TRY(hipFreeAsync(d_proj, stream));
     }
+    return TIGRE_OK;
+}
+
+size_t atb_workspace_bytes(const HostGeometry& g) {
+    return align256((size_t)quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * (size_t)g.nangles * 4 * sizeof(float));
+}
+
+int atb_launch(const void* d_proj, int dtype, const HostGeometry& g, const BwdView* d_views, bool axial,
+               int backprojection_type, int layout, float* d_vol, char* ws, int zper, const AtbUpdate& ep,
+               hipStream_t stream) {
+    float* quads = reinterpret_cast<float*>(ws);
+    TIGRE_CHECK(launch_quad_projections(d_proj, dtype, g.nDetec[0], g.nDetec[1], g.nangles, layout, quads, stream));
+    TIGRE_CHECK(launch_atb(quads, g, d_views, backprojection_type == TIGRE_BACKPROJ_MATCHED, axial, d_vol, zper, ep,
+                           stream));
     return TIGRE_OK;
 }
 
@@ -158,14 +196,11 @@ int atb_device(const void* d_proj, const HostGeometry& g, void* d_vol_out, int b
     const bool axial = backward_is_axial(g, views);
     void* d_views = nullptr;
     TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(BwdView), stream));
-    const size_t quad_bytes =
-        (size_t)quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * (size_t)g.nangles * 4 * sizeof(float);
-    float* quads = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&quads, quad_bytes, stream));
-    TIGRE_CHECK(launch_quad_projections(d_proj, dtype, g.nDetec[0], g.nDetec[1], g.nangles, layout, quads, stream));
-    TIGRE_CHECK(launch_atb(quads, g, (const BwdView*)d_views, backprojection_type == TIGRE_BACKPROJ_MATCHED, axial,
-                           d_vol, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(quads, stream));
+    char* ws = nullptr;
+    TIGRE_HIP_TRY(hipMallocAsync((void**)&ws, atb_workspace_bytes(g), stream));
+    TIGRE_CHECK(atb_launch(d_proj, dtype, g, (const BwdView*)d_views, axial, backprojection_type, layout, d_vol, ws,
+                           16, AtbUpdate{}, stream));
+    TIGRE_HIP_TRY(hipFreeAsync(ws, stream));
     TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
     if (dtype != TIGRE_DTYPE_F32) {
         TIGRE_CHECK(launch_store_converted(d_vol, d_vol_out, nvox, dtype, stream));
