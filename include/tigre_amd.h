@@ -219,6 +219,34 @@ int tigre_plan_sart_update(TigrePlan plan, const float* d_residual, float* d_x, 
 int tigre_sart_residual(const float* d_proj, const float* d_w, float* d_ax, long long n, void* stream);
 int tigre_plan_destroy(TigrePlan plan);
 
+/* ---------------------------------------------------------------------- */
+/* Learned proximal step: 3x3x3 convolutions (BF16 MFMA, FP32 accumulate)  */
+/* ---------------------------------------------------------------------- */
+
+/* No counterpart in the reference.  Building blocks of the residual CNN that
+ * can replace minimizeTV (GD_TV.hpp:58) inside ASD-POCS: forward, data-
+ * gradient and weight-gradient passes of 'same' 3x3x3 convolutions with 32
+ * channels between layers.  Volumes are n x d x h x w (x fastest);
+ * activations are bf16 with 32 channels per voxel, channel fastest; a
+ * single-channel FP32 volume (the image, or the loss gradient) can stand in
+ * for an activation input (it becomes channel 0).  Taps index (dz, dy, dx) in
+ * {-1,0,1}^3 as 9(dz+1) + 3(dy+1) + (dx+1); outside the volume inputs are 0. */
+#define TIGRE_CNN_IN_BF16X32 0   /* bf16 activations, 32 channels */
+#define TIGRE_CNN_IN_F32X1   1   /* one FP32 channel */
+#define TIGRE_CNN_EPI_BIAS_RELU 0  /* y = bf16(max(conv + bias, 0)), 32 channels */
+#define TIGRE_CNN_EPI_RELU_MASK 1  /* y = bf16(conv * (aux > 0)), aux bf16 activations (data gradient) */
+#define TIGRE_CNN_EPI_RESIDUAL  2  /* y = aux + conv[channel 0] + bias[0], FP32, one channel */
+/* y[v][o] = sum_tap sum_i x[v + tap][i] * w[tap][o][i], w: 27 x 32 x 32 bf16
+ * (16-byte aligned), bias: 32 floats (unused by RELU_MASK). */
+int tigre_cnn_conv(const void* d_x, int x_mode, const void* d_w, const float* d_bias, const void* d_aux, void* d_y,
+                   int epilogue, int n, int d, int h, int w, void* stream);
+/* dw[tap][o][i] = sum_v x[v + tap][i] * dy[v][o] (FP32, 28 x 32 x 32; row 27
+ * holds the bias gradient sum_v dy[v][o] at [27][o][0]).  Deterministic: a
+ * fixed-order reduction of per-block partial sums kept in the workspace. */
+int tigre_cnn_wgrad_workspace_size(int n, int d, int h, int w, size_t* bytes);
+int tigre_cnn_wgrad(const void* d_x, int x_mode, const void* d_dy, int dy_mode, float* d_dw, int n, int d, int h,
+                    int w, void* d_workspace, size_t workspace_bytes, void* stream);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char* tigre_last_error(void);
 

@@ -22,6 +22,11 @@ BACKPROJ_MATCHED = 1
 DTYPE_F32 = 0
 DTYPE_BF16 = 1
 DTYPE_F16 = 2
+CNN_IN_BF16X32 = 0
+CNN_IN_F32X1 = 1
+CNN_EPI_BIAS_RELU = 0
+CNN_EPI_RELU_MASK = 1
+CNN_EPI_RESIDUAL = 2
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -100,6 +105,14 @@ SIGNATURES = {
     "tigre_sart_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
                                            ctypes.c_void_p]),
     "tigre_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tigre_cnn_conv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p]),
+    "tigre_cnn_wgrad_workspace_size": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_size_t)]),
+    "tigre_cnn_wgrad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_void_p]),
     "tigre_get_gpu_count": (ctypes.c_int, []),
     "tigre_get_gpu_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "tigre_kernel_timing": (ctypes.c_int, [ctypes.c_int]),

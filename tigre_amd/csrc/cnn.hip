@@ -1,0 +1,273 @@
+// 3D convolutions of the learned proximal network, BF16 MFMA on gfx950.
+//
+// The network (a residual CNN that stands in for the TV step of ASD-POCS) is
+// six 3x3x3 'same' convolutions with 32 channels between them.  All of them,
+// forward and backward, run through two implicit-GEMM kernels built on
+// v_mfma_f32_16x16x32_bf16 (A: 16 rows x 32 k, B: 32 k x 16 columns, FP32
+// accumulate; lane l holds row / column l % 16 and k = 8 (l / 16) .. + 7, and
+// result register r of lane l is row 4 (l / 16) + r, column l % 16):
+//
+//  * conv_kernel: y[v][o] = sum_tap sum_i x[v + tap][i] w[tap][o][i].  GEMM
+//    rows are 16 consecutive voxels along x, columns the 32 output channels
+//    (two MFMA column blocks), k one tap's 32 input channels: one 16-byte
+//    load per lane per tap is an A fragment (8 channels of one voxel).  A
+//    wavefront computes four such tiles (64 voxels) and reuses each tap's
+//    weight fragments across them.  Inputs are activations (bf16, 32
+//    channels, channel fastest) or a single FP32 channel (the image, or the
+//    loss gradient), widened into channel 0 of the fragment.
+//  * wgrad_kernel: dw[tap][o][i] = sum_v x[v + tap][i] dy[v][o], GEMM k =
+//    32 voxels of an x-row.  The x and dy rows are staged in LDS transposed
+//    (channel-major), so fragments are again 16-byte reads.  Tap 27 is the
+//    bias: x replaced by 1 in channel 0, so dw[27][o][0] = sum_v dy[v][o].
+//
+// Volumes are N x D x H x W (x fastest); taps index (dz, dy, dx) in
+// {-1, 0, 1}^3 as 9 (dz + 1) + 3 (dy + 1) + (dx + 1); outside the volume the
+// input is zero.
+#include "common.hpp"
+
+namespace tigre {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kC = 32;            // channels between layers
+constexpr int kTilesPerWave = 4;  // 16-voxel tiles per wavefront (64 voxels along x)
+
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// A fragment of 8 channels (8 q .. 8 q + 7) of voxel `vox`, or zeros.
+template <int kIn>
+__device__ __forceinline__ bf16x8 load_frag(const void* __restrict__ src, long long vox, int q, bool ok) {
+    bf16x8 f;
+    if (kIn == TIGRE_CNN_IN_BF16X32) {
+        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+        if (ok) raw = *reinterpret_cast<const uint4*>(static_cast<const unsigned short*>(src) + vox * kC + 8 * q);
+        f = __builtin_bit_cast(bf16x8, raw);
+    } else {
+        float v = 0.0f;
+        if (ok && q == 0) v = static_cast<const float*>(src)[vox];
+        uint4 raw = make_uint4((unsigned)bf16_bits(v), 0u, 0u, 0u);
+        f = __builtin_bit_cast(bf16x8, raw);
+    }
+    return f;
+}
+
+template <int kIn, int kEpi>
+__global__ __launch_bounds__(256) void conv_kernel(const void* __restrict__ x, const unsigned short* __restrict__ w,
+                                                   const float* __restrict__ bias, const void* __restrict__ aux,
+                                                   void* __restrict__ y, int N, int D, int H, int W) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const long long row = (long long)blockIdx.y * 4 + wave;  // (n, z, y) row of the wavefront
+    const long long rows = (long long)N * D * H;
+    if (row >= rows) return;  // uniform per wavefront
+    const int yy0 = (int)(row % H), zz0 = (int)((row / H) % D);
+    const long long n = row / ((long long)H * D);
+    const int x0 = blockIdx.x * (16 * kTilesPerWave);
+
+    f32x4 acc[kTilesPerWave][2];
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int tap = 0; tap < 27; ++tap) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int zz = zz0 + dz, yy = yy0 + dy;
+        const bool row_ok = zz >= 0 && zz < D && yy >= 0 && yy < H;
+        const long long base = ((n * D + zz) * H + yy) * (long long)W;
+        const unsigned short* wt = w + (long long)tap * kC * kC;
+        const bf16x8 b0 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + m * kC + 8 * q));
+        const bf16x8 b1 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + (16 + m) * kC + 8 * q));
+#pragma unroll
+        for (int t = 0; t < kTilesPerWave; ++t) {
+            const int xx = x0 + 16 * t + m + dx;
+            const bf16x8 a = load_frag<kIn>(x, base + xx, q, row_ok && xx >= 0 && xx < W);
+            acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[t][0], 0, 0, 0);
+            acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[t][1], 0, 0, 0);
+        }
+    }
+
+    const long long obase = row * (long long)W;
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            const int o = hb * 16 + m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xo = x0 + 16 * t + 4 * q + r;
+                if (xo >= W) continue;
+                const long long vox = obase + xo;
+                float v = acc[t][hb][r];
+                if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+                    v += bias[o];
+                    v = v > 0.0f ? v : 0.0f;
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                } else if (kEpi == TIGRE_CNN_EPI_RELU_MASK) {
+                    const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
+                    v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                } else {  // residual output: channel 0 only, FP32
+                    if (o == 0) static_cast<float*>(y)[vox] = static_cast<const float*>(aux)[vox] + (v + bias[0]);
+                }
+            }
+        }
+    }
+}
+
+// Weight gradient.  Block: one tap (blockIdx.x, 27 = bias) and one chunk of
+// x-row segments of 32 voxels (blockIdx.y); wavefront w owns the 16 x 16
+// block (input channels 16 (w & 1) .., output channels 16 (w >> 1) ..).
+constexpr int kLdsStride = 40;  // bf16 per LDS row: 32 voxels + 8 padding (80 bytes)
+
+template <int kXin, int kDyin>
+__global__ __launch_bounds__(256) void wgrad_kernel(const void* __restrict__ x, const void* __restrict__ dy,
+                                                    float* __restrict__ part, int N, int D, int H, int W,
+                                                    long long segs_per_chunk) {
+    __shared__ __attribute__((aligned(16))) unsigned short xt[kC * kLdsStride];
+    __shared__ __attribute__((aligned(16))) unsigned short dt[kC * kLdsStride];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int tap = blockIdx.x;
+    const int dz = tap / 9 - 1, dyy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int segW = (W + 31) / 32;
+    const long long nseg = (long long)N * D * H * segW;
+    const long long s0 = (long long)blockIdx.y * segs_per_chunk;
+    const long long s1 = s0 + segs_per_chunk < nseg ? s0 + segs_per_chunk : nseg;
+    const int ib = 16 * (wave & 1), ob = 16 * (wave >> 1);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    // staging role of this thread: tensor (x or dy), voxel of the segment, 8-channel group
+    const bool stage_x = tid < 128;
+    const int sv = (tid & 127) >> 2, sq = tid & 3;
+    for (long long s = s0; s < s1; ++s) {
+        const int xs = (int)(s % segW);
+        const long long row = s / segW;
+        const int y0 = (int)(row % H), z0 = (int)((row / H) % D);
+        const long long n = row / ((long long)H * D);
+        const int xv = xs * 32 + sv;
+        unsigned short vals[8];
+        if (stage_x) {
+            if (tap == 27) {
+                for (int j = 0; j < 8; ++j) vals[j] = 0;
+                if (sq == 0 && xv < W) vals[0] = 0x3f80;  // 1.0
+            } else {
+                const int zz = z0 + dz, yy = y0 + dyy, xx = xv + dx;
+                const bool ok = xv < W && zz >= 0 && zz < D && yy >= 0 && yy < H && xx >= 0 && xx < W;
+                const long long vox = ((n * D + zz) * H + yy) * (long long)W + xx;
+                const bf16x8 f = load_frag<kXin>(x, vox, sq, ok);
+                const uint4 raw = __builtin_bit_cast(uint4, f);
+                const unsigned r4[4] = {raw.x, raw.y, raw.z, raw.w};
+                for (int j = 0; j < 4; ++j) {
+                    vals[2 * j] = (unsigned short)(r4[j] & 0xffffu);
+                    vals[2 * j + 1] = (unsigned short)(r4[j] >> 16);
+                }
+            }
+            for (int j = 0; j < 8; ++j) xt[(8 * sq + j) * kLdsStride + sv] = vals[j];
+        } else {
+            const long long vox = row * (long long)W + xv;
+            const bf16x8 f = load_frag<kDyin>(dy, vox, sq, xv < W);
+            const uint4 raw = __builtin_bit_cast(uint4, f);
+            const unsigned r4[4] = {raw.x, raw.y, raw.z, raw.w};
+            for (int j = 0; j < 4; ++j) {
+                vals[2 * j] = (unsigned short)(r4[j] & 0xffffu);
+                vals[2 * j + 1] = (unsigned short)(r4[j] >> 16);
+            }
+            for (int j = 0; j < 8; ++j) dt[(8 * sq + j) * kLdsStride + sv] = vals[j];
+        }
+        __syncthreads();
+        const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xt + (ib + m) * kLdsStride + 8 * q));
+        const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(dt + (ob + m) * kLdsStride + 8 * q));
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
+    }
+    // D[i][o]: row i = ib + 4 q + r, column o = ob + m; stored as part[chunk][tap][o][i]
+    float* p = part + ((long long)blockIdx.y * 28 + tap) * kC * kC;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) p[(ob + m) * kC + ib + 4 * q + r] = acc[r];
+}
+
+// out[i] = sum over chunks of part[c][i], in chunk order.
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const float* __restrict__ part, int nchunk,
+                                                           float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 28 * kC * kC) return;
+    float s = 0.0f;
+    for (int c = 0; c < nchunk; ++c) s += part[(long long)c * 28 * kC * kC + i];
+    out[i] = s;
+}
+
+long long wgrad_chunks(long long nseg) {
+    long long c = (nseg + 15) / 16;  // at least 16 segments per block
+    if (c > 256) c = 256;
+    return c < 1 ? 1 : c;
+}
+
+}  // namespace
+
+size_t cnn_wgrad_workspace(int N, int D, int H, int W) {
+    const long long nseg = (long long)N * D * H * ((W + 31) / 32);
+    return (size_t)wgrad_chunks(nseg) * 28 * kC * kC * sizeof(float);
+}
+
+int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const float* bias, const void* aux, void* y,
+                    int epi, int N, int D, int H, int W, hipStream_t stream) {
+    TIGRE_REQUIRE(N > 0 && D > 0 && H > 0 && W > 0, "empty volume");
+    TIGRE_REQUIRE(x_mode == TIGRE_CNN_IN_BF16X32 || x_mode == TIGRE_CNN_IN_F32X1, "unknown input mode");
+    TIGRE_REQUIRE(epi == TIGRE_CNN_EPI_BIAS_RELU || epi == TIGRE_CNN_EPI_RELU_MASK || epi == TIGRE_CNN_EPI_RESIDUAL,
+                  "unknown epilogue");
+    TIGRE_REQUIRE(epi == TIGRE_CNN_EPI_RELU_MASK || bias != nullptr, "missing bias");
+    TIGRE_REQUIRE(epi == TIGRE_CNN_EPI_BIAS_RELU || aux != nullptr, "missing mask / residual input");
+    TIGRE_REQUIRE((reinterpret_cast<uintptr_t>(x) & 15) == 0 && (reinterpret_cast<uintptr_t>(w) & 15) == 0,
+                  "activations and weights must be 16-byte aligned");
+    const long long rows = (long long)N * D * H;
+    TIGRE_REQUIRE((rows + 3) / 4 <= 2147483647LL, "volume too large");
+    dim3 grid((unsigned)((W + 16 * kTilesPerWave - 1) / (16 * kTilesPerWave)), (unsigned)((rows + 3) / 4));
+#define TIGRE_CONV(I, E) hipLaunchKernelGGL((conv_kernel<I, E>), grid, dim3(256), 0, stream, x, w, bias, aux, y, N, D, H, W)
+    if (x_mode == TIGRE_CNN_IN_BF16X32) {
+        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_BIAS_RELU);
+        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_RELU_MASK);
+        else TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_RESIDUAL);
+    } else {
+        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_BIAS_RELU);
+        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_RELU_MASK);
+        else TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_RESIDUAL);
+    }
+#undef TIGRE_CONV
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_cnn_wgrad(const void* x, int x_mode, const void* dy, int dy_mode, float* dw, int N, int D, int H, int W,
+                     void* ws, size_t ws_bytes, hipStream_t stream) {
+    TIGRE_REQUIRE(N > 0 && D > 0 && H > 0 && W > 0, "empty volume");
+    TIGRE_REQUIRE(x_mode == TIGRE_CNN_IN_BF16X32 || x_mode == TIGRE_CNN_IN_F32X1, "unknown input mode");
+    TIGRE_REQUIRE(dy_mode == TIGRE_CNN_IN_BF16X32 || dy_mode == TIGRE_CNN_IN_F32X1, "unknown gradient mode");
+    TIGRE_REQUIRE(ws != nullptr && ws_bytes >= cnn_wgrad_workspace(N, D, H, W), "workspace too small");
+    const long long nseg = (long long)N * D * H * ((W + 31) / 32);
+    const long long nchunk = wgrad_chunks(nseg);
+    const long long per = (nseg + nchunk - 1) / nchunk;
+    float* part = static_cast<float*>(ws);
+    dim3 grid(28, (unsigned)nchunk);
+#define TIGRE_WG(XI, DI) hipLaunchKernelGGL((wgrad_kernel<XI, DI>), grid, dim3(256), 0, stream, x, dy, part, N, D, H, W, per)
+    if (x_mode == TIGRE_CNN_IN_BF16X32) {
+        if (dy_mode == TIGRE_CNN_IN_BF16X32) TIGRE_WG(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_IN_BF16X32);
+        else TIGRE_WG(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_IN_F32X1);
+    } else {
+        if (dy_mode == TIGRE_CNN_IN_BF16X32) TIGRE_WG(TIGRE_CNN_IN_F32X1, TIGRE_CNN_IN_BF16X32);
+        else TIGRE_WG(TIGRE_CNN_IN_F32X1, TIGRE_CNN_IN_F32X1);
+    }
+#undef TIGRE_WG
+    TIGRE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((28 * kC * kC + 255) / 256), dim3(256), 0, stream, part, (int)nchunk,
+                       dw);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+}  // namespace tigre

@@ -214,6 +214,13 @@ int atb_launch(const void* d_proj, int dtype, const HostGeometry& g, const BwdVi
 // r = W (p - Ax) over n elements, written over ax (sart.hip).
 int launch_sart_residual(const float* proj, const float* w, float* ax, long long n, hipStream_t stream);
 
+// Convolutions of the learned proximal network (cnn.hip).
+size_t cnn_wgrad_workspace(int N, int D, int H, int W);
+int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const float* bias, const void* aux, void* y,
+                    int epi, int N, int D, int H, int W, hipStream_t stream);
+int launch_cnn_wgrad(const void* x, int x_mode, const void* dy, int dy_mode, float* dw, int N, int D, int H, int W,
+                     void* ws, size_t ws_bytes, hipStream_t stream);
+
 // Operator drivers on device pointers (ops.cpp).
 int ax_device(const void* d_img, const HostGeometry& g, void* d_proj, int projection_type, int layout, int dtype,
               hipStream_t stream);

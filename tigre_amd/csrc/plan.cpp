@@ -195,6 +195,30 @@ int tigre_sart_residual(const float* d_proj, const float* d_w, float* d_ax, long
     return launch_sart_residual(d_proj, d_w, d_ax, n, (hipStream_t)stream);
 }
 
+int tigre_cnn_conv(const void* d_x, int x_mode, const void* d_w, const float* d_bias, const void* d_aux, void* d_y,
+                   int epilogue, int n, int d, int h, int w, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_x != nullptr && d_w != nullptr && d_y != nullptr, "null device pointer");
+    return launch_cnn_conv(d_x, x_mode, static_cast<const unsigned short*>(d_w), d_bias, d_aux, d_y, epilogue, n, d,
+                           h, w, (hipStream_t)stream);
+}
+
+int tigre_cnn_wgrad_workspace_size(int n, int d, int h, int w, size_t* bytes) {
+    set_error("");
+    TIGRE_REQUIRE(bytes != nullptr, "null argument");
+    TIGRE_REQUIRE(n > 0 && d > 0 && h > 0 && w > 0, "empty volume");
+    *bytes = cnn_wgrad_workspace(n, d, h, w);
+    return TIGRE_OK;
+}
+
+int tigre_cnn_wgrad(const void* d_x, int x_mode, const void* d_dy, int dy_mode, float* d_dw, int n, int d, int h,
+                    int w, void* d_workspace, size_t workspace_bytes, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_x != nullptr && d_dy != nullptr && d_dw != nullptr, "null device pointer");
+    return launch_cnn_wgrad(d_x, x_mode, d_dy, dy_mode, d_dw, n, d, h, w, d_workspace, workspace_bytes,
+                            (hipStream_t)stream);
+}
+
 int tigre_plan_destroy(TigrePlan plan) {
     set_error("");
     if (plan == nullptr) return TIGRE_OK;

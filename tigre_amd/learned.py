@@ -1,0 +1,239 @@
+"""Learned proximal step for ASD-POCS: a residual 3D CNN on BF16 MFMA kernels.
+
+The TV step of ASD-POCS (minimizeTV, pocs_algorithms.py:145-149 calling
+GD_TV.hpp:58) moves the image by at most ``tviter`` steps of size dtvg.  The
+learned step replaces it with
+
+    P(x, s) = x + s * F(x),    F = conv6 o relu o conv5 ... relu o conv1,
+
+six 3x3x3 convolutions with 32 channels between them, BF16 weights and
+activations, FP32 accumulation, trained so that x + F(x) matches
+x + (minimizeTV(x, s) - x) / s on pairs that the plain TV step generates.
+Every convolution, forward and backward, runs in the library's hand-written
+MFMA kernels (tigre_cnn_conv / tigre_cnn_wgrad); torch only holds device
+memory, the small FP32 master weights and the Adam update.
+
+``LearnedProx`` is the ``regularizer`` callable of
+:func:`tigre_amd.algorithms.asd_pocs`; plain TV is its fallback, taken when the
+learned step does not lower the TV objective or raises the data residual.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+from .operators import _stream_handle
+
+C = 32
+LAYERS = 6
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class ResidualCNN:
+    """The network, its FP32 master weights and its kernels.
+
+    Master weights W[l] have shape (27, Cout, Cin) (taps (dz, dy, dx) index
+    9(dz+1) + 3(dy+1) + dx+1), Cin = 1 for the first layer, Cout = 1 for the
+    last; biases b[l] have Cout entries."""
+
+    def __init__(self, seed=0, device="cuda"):
+        torch = _torch()
+        g = torch.Generator().manual_seed(int(seed))
+        shapes = [(27, C, 1)] + [(27, C, C)] * 4 + [(27, 1, C)]
+        self.W, self.b = [], []
+        for l, s in enumerate(shapes):
+            fan_in = 27 * s[2]
+            std = (2.0 / fan_in) ** 0.5 if l < LAYERS - 1 else 0.1 * (1.0 / fan_in) ** 0.5
+            self.W.append((torch.randn(s, generator=g) * std).to(device))
+            self.b.append(torch.zeros(s[1], device=device))
+        self.device = device
+        self.lib = _native.lib()
+        self._adam = None
+        self.refresh()
+
+    # ---- kernel-side copies ------------------------------------------------
+    def refresh(self):
+        """Rebuild the bf16 kernel weights (padded to 32 x 32 per tap), the
+        flipped / transposed data-gradient weights and padded biases."""
+        torch = _torch()
+        self.Wk, self.Wd, self.bk = [], [], []
+        for l in range(LAYERS):
+            w = torch.zeros((27, C, C), device=self.device)
+            co, ci = self.W[l].shape[1], self.W[l].shape[2]
+            w[:, :co, :ci] = self.W[l]
+            wk = w.to(torch.bfloat16).contiguous()
+            self.Wk.append(wk)
+            # data gradient: conv of dy with taps mirrored and channels swapped
+            self.Wd.append(wk.flip(0).transpose(1, 2).contiguous())
+            bb = torch.zeros(C, device=self.device)
+            bb[:co] = self.b[l]
+            self.bk.append(bb)
+
+    def parameters(self):
+        return self.W + self.b
+
+    def state(self):
+        return {"W": [w.cpu().numpy() for w in self.W], "b": [v.cpu().numpy() for v in self.b]}
+
+    def load(self, state):
+        torch = _torch()
+        self.W = [torch.as_tensor(np.asarray(w, dtype=np.float32), device=self.device) for w in state["W"]]
+        self.b = [torch.as_tensor(np.asarray(v, dtype=np.float32), device=self.device) for v in state["b"]]
+        self.refresh()
+
+    # ---- launches ----------------------------------------------------------
+    def _conv(self, x, x_mode, w, bias, aux, y, epi, shape):
+        n, d, h, wd = shape
+        st = self.lib.tigre_cnn_conv(_p(x), x_mode, _p(w), _p(bias), _p(aux), _p(y), epi, n, d, h, wd,
+                                     _stream_handle())
+        _native.check(st, "TIGRE: CNN convolution failed")
+
+    def _wgrad(self, x, x_mode, dy, dy_mode, shape, ws, out):
+        n, d, h, wd = shape
+        st = self.lib.tigre_cnn_wgrad(_p(x), x_mode, _p(dy), dy_mode, _p(out), n, d, h, wd, _p(ws), ws.numel(),
+                                      _stream_handle())
+        _native.check(st, "TIGRE: CNN weight gradient failed")
+
+    def _shape(self, img):
+        s = tuple(int(v) for v in img.shape)
+        if len(s) == 3:
+            s = (1,) + s
+        if len(s) != 4:
+            raise ValueError("expected a volume (D, H, W) or a batch (N, D, H, W)")
+        return s
+
+    def forward(self, img, scale=1.0, keep=False):
+        """img + scale * F(img) for an FP32 device volume or batch; with keep,
+        the activations are kept for ``backward``."""
+        torch = _torch()
+        if img.dtype != torch.float32 or not img.is_contiguous():
+            img = img.float().contiguous()
+        shape = self._shape(img)
+        acts = []
+        bufs = [torch.empty(shape + (C,), dtype=torch.bfloat16, device=img.device) for _ in range(5 if keep else 2)]
+        x, x_mode = img, _native.CNN_IN_F32X1
+        for l in range(LAYERS - 1):
+            y = bufs[l] if keep else bufs[l % 2]
+            self._conv(x, x_mode, self.Wk[l], self.bk[l], None, y, _native.CNN_EPI_BIAS_RELU, shape)
+            acts.append(y)
+            x, x_mode = y, _native.CNN_IN_BF16X32
+        out = torch.empty_like(img)
+        wl, bl = self.Wk[5], self.bk[5]
+        if scale != 1.0:
+            wl = (self.W[5] * scale)
+            w = torch.zeros((27, C, C), device=img.device)
+            w[:, :1, :] = wl
+            wl = w.to(torch.bfloat16).contiguous()
+            bl = self.bk[5] * scale
+        self._conv(x, x_mode, wl, bl, img, out, _native.CNN_EPI_RESIDUAL, shape)
+        if keep:
+            self._saved = (img, acts, shape)
+        return out
+
+    __call__ = forward
+
+    def backward(self, dout):
+        """Gradients (dW, db) of sum(out * dout) after ``forward(..., keep=True)`` with scale 1."""
+        torch = _torch()
+        img, acts, shape = self._saved
+        nbytes = ctypes.c_size_t()
+        _native.check(self.lib.tigre_cnn_wgrad_workspace_size(*shape, ctypes.byref(nbytes)), "wgrad workspace")
+        ws = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=img.device)
+        dout = dout.float().contiguous()
+        dW, db = [None] * LAYERS, [None] * LAYERS
+        raw = torch.empty((28, C, C), device=img.device)
+        # last layer: input h5, gradient dout (one FP32 channel)
+        self._wgrad(acts[4], _native.CNN_IN_BF16X32, dout, _native.CNN_IN_F32X1, shape, ws, raw)
+        dW[5] = raw[:27, :1, :].clone()
+        db[5] = raw[27, :1, 0].clone()
+        g = torch.empty(shape + (C,), dtype=torch.bfloat16, device=img.device)
+        self._conv(dout, _native.CNN_IN_F32X1, self.Wd[5], None, acts[4], g, _native.CNN_EPI_RELU_MASK, shape)
+        g2 = torch.empty_like(g)
+        for l in range(LAYERS - 2, 0, -1):
+            self._wgrad(acts[l - 1], _native.CNN_IN_BF16X32, g, _native.CNN_IN_BF16X32, shape, ws, raw)
+            dW[l] = raw[:27].clone()
+            db[l] = raw[27, :, 0].clone()
+            self._conv(g, _native.CNN_IN_BF16X32, self.Wd[l], None, acts[l - 1], g2, _native.CNN_EPI_RELU_MASK, shape)
+            g, g2 = g2, g
+        self._wgrad(img, _native.CNN_IN_F32X1, g, _native.CNN_IN_BF16X32, shape, ws, raw)
+        dW[0] = raw[:27, :, :1].clone()
+        db[0] = raw[27, :, 0].clone()
+        self._saved = None
+        return dW, db
+
+    # ---- training ----------------------------------------------------------
+    def train_step(self, x, target, lr=1e-3, betas=(0.9, 0.999), eps=1e-8):
+        """One Adam step on the mean squared error of forward(x) against target; returns the loss."""
+        torch = _torch()
+        out = self.forward(x, keep=True)
+        diff = out - target
+        loss = float(torch.mean(diff.double() ** 2))
+        dW, db = self.backward(diff * (2.0 / diff.numel()))
+        grads = dW + db
+        params = self.W + self.b
+        if self._adam is None:
+            self._adam = {"t": 0, "m": [torch.zeros_like(p) for p in params], "v": [torch.zeros_like(p) for p in params]}
+        a = self._adam
+        a["t"] += 1
+        b1, b2 = betas
+        c1, c2 = 1.0 - b1 ** a["t"], 1.0 - b2 ** a["t"]
+        for p, g, m, v in zip(params, grads, a["m"], a["v"]):
+            m.mul_(b1).add_(g, alpha=1.0 - b1)
+            v.mul_(b2).addcmul_(g, g, value=1.0 - b2)
+            p.sub_(lr * (m / c1) / ((v / c2).sqrt() + eps))
+        self.refresh()
+        return loss
+
+
+def tv_value(x, eps=1e-8):
+    """Isotropic TV objective sum sqrt(|forward differences|^2 + eps) (diagnostic of the fallback rule)."""
+    torch = _torch()
+    d0 = torch.zeros_like(x)
+    d1 = torch.zeros_like(x)
+    d2 = torch.zeros_like(x)
+    d0[:-1] = x[1:] - x[:-1]
+    d1[:, :-1] = x[:, 1:] - x[:, :-1]
+    d2[:, :, :-1] = x[:, :, 1:] - x[:, :, :-1]
+    return float(torch.sqrt(d0.double() ** 2 + d1.double() ** 2 + d2.double() ** 2 + eps).sum())
+
+
+class LearnedProx:
+    """ASD-POCS ``regularizer``: the learned step, with plain TV as fallback.
+
+    The learned result is kept when its TV objective is below the input's and,
+    if ``residual`` (a callable image -> data residual norm) is given, its data
+    residual does not exceed the input's by more than ``tol``; otherwise the
+    step is recomputed with minTV(x, dtvg, tviter).  ``taken`` / ``fallbacks``
+    count the two outcomes."""
+
+    def __init__(self, net, tviter=20, residual=None, tol=1e-3):
+        self.net = net
+        self.tviter = tviter
+        self.residual = residual
+        self.tol = tol
+        self.taken = 0
+        self.fallbacks = 0
+
+    def __call__(self, img, dtvg):
+        from .tv import minTV
+
+        y = self.net.forward(img, scale=float(dtvg))
+        ok = tv_value(y) < tv_value(img)
+        if ok and self.residual is not None:
+            ok = self.residual(y) <= self.residual(img) * (1.0 + self.tol)
+        if ok:
+            self.taken += 1
+            return y
+        self.fallbacks += 1
+        return minTV(img, dtvg, self.tviter)
