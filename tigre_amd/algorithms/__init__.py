@@ -391,7 +391,8 @@ def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, t
     then ``tviter`` TV steepest-descent steps of size dtvg with the step
     adapted as in the reference; stops on the reference's criteria.
     ``regularizer`` (default: minTV) may replace the TV step with any callable
-    (image_tensor, dtvg) -> image_tensor."""
+    (image_tensor, dtvg) -> image_tensor; ``callback(n_iter, image_tensor)``,
+    if given, runs after each iteration, outside its timed part."""
     from ..tv import minTV
 
     st = _SartState(proj, geo, angles, blocksize, order=OrderStrategy, lmbda=lmbda, noneg=True, verbose=verbose)
@@ -424,6 +425,8 @@ def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, t
 
         c = float(torch.dot(dg_vec.reshape(-1).double(), dp_vec.reshape(-1).double())) / max(dg * dp, 1e-6)
         stats.append((n_iter, dd, dp, dg, c, time.perf_counter() - t0))
+        if kwargs.get("callback") is not None:
+            kwargs["callback"](n_iter, st.res)
         if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter >= niter:
             if verbose:
                 print("Stop criteria met: c = %s beta = %s iter = %d" % (c, beta, n_iter))

@@ -237,3 +237,59 @@ class LearnedProx:
             return y
         self.fallbacks += 1
         return minTV(img, dtvg, self.tviter)
+
+
+# ---- data for training and evaluation ----------------------------------------
+def poisson_sinogram(proj, I0=1e5, seed=0):
+    """Noisy line integrals -ln(Poisson(I0 e^{-p}) / I0), counts clamped at 1."""
+    torch = _torch()
+    g = torch.Generator(device=proj.device).manual_seed(int(seed))
+    counts = torch.poisson(I0 * torch.exp(-proj.double()), generator=g).clamp_(min=1.0)
+    return (-torch.log(counts / I0)).float().contiguous()
+
+
+def collect_pairs(proj, geo, angles, niter, tviter=20, alpha=0.002, **kwargs):
+    """(pre-TV image, minimizeTV output, dtvg) of every iteration of plain ASD-POCS."""
+    from .algorithms import asd_pocs
+    from .tv import minTV
+
+    pairs = []
+
+    def recording(img, step):
+        out = minTV(img, step, tviter)
+        pairs.append((img.clone(), out.clone(), float(step)))
+        return out
+
+    asd_pocs(proj, geo, angles, niter, alpha=alpha, tviter=tviter, regularizer=recording, return_tensor=True,
+             **kwargs)
+    return pairs
+
+
+def make_patches(pairs, count, size, seed=0):
+    """``count`` random size^3 crops: inputs x and training targets x + (TV(x) - x) / dtvg."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    for k in range(count):
+        pre, post, step = pairs[int(rng.integers(len(pairs)))]
+        D, H, W = pre.shape
+        z, y, x = (int(rng.integers(0, max(1, n - size + 1))) for n in (D, H, W))
+        a = pre[z:z + size, y:y + size, x:x + size]
+        b = post[z:z + size, y:y + size, x:x + size]
+        xs.append(a)
+        ts.append(a + (b - a) / step)
+    return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
+
+
+def train(net, x, target, steps, batch=8, lr=1e-3, seed=0, log=None):
+    """Adam on random minibatches of (x, target); returns the loss history."""
+    torch = _torch()
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    losses = []
+    n = x.shape[0]
+    for s in range(steps):
+        idx = torch.randint(0, n, (min(batch, n),), generator=g).to(x.device)
+        losses.append(net.train_step(x[idx].contiguous(), target[idx].contiguous(), lr=lr))
+        if log is not None and (s % 50 == 0 or s == steps - 1):
+            log("step %d loss %.4e" % (s, losses[-1]))
+    return losses
