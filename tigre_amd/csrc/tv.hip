@@ -2,15 +2,21 @@
 // primal-dual denoiser.
 //
 // POCS-TV, GD_TV.cu:489-597 (and its adaptive-weighted and PICCS variants,
-// GD_AwTV.cu, PICCS.cu).  One iteration is two passes over the volume:
-//   pass 1  g = grad TV(f) at every voxel, sum of g^2 in FP64 per workgroup;
-//   pass 2  every workgroup sums the partials in a fixed order (the same
-//           result in every workgroup and every run), then writes
-//           f' = f - alpha * g / sqrt(sum + 1e-7), recomputing g from f.
-// The gradient is never stored: 12 bytes of HBM traffic per voxel and
-// iteration instead of the reference's read-gradient-copy-reduce-scale chain,
-// and no host round trip for the norm.  The grids are fixed-size grid-stride
-// launches so the partial array stays small.
+// GD_AwTV.cu, PICCS.cu).  The TV sub-gradient of GD_TV.cu:117-143 at voxel v
+// is the divergence of the normalised backward-difference field
+//     n_c(v) = w_c d_c(v) / (sqrt(sum_k w_k d_k(v)^2) + eps),
+//     g(v)   = n_x(v) + n_y(v) + n_z(v) - n_x(v + x) - n_y(v + y) - n_z(v + z),
+// (w = 1 for TV, exp(-(d / delta)^2) per component for AwTV, with the fast
+// exponential as the reference's __expf), so each n is
+// computed once per voxel instead of four times.  One iteration is two passes:
+//   pass 1  g at every voxel, sum of g^2 in FP64 per workgroup;
+//   scale   one workgroup sums the partials in a fixed order (the same result
+//           every run) into alpha / sqrt(sum + norm_eps);
+//   pass 2  f' = f - scale * g, recomputing g from f.
+// Each pass is a 64 x 8 column tile that marches along z through a chunk of
+// planes: the f planes (with a one-voxel halo) and two planes of n live in
+// LDS, so f is read from HBM about once per pass: 12 bytes of traffic per
+// voxel and iteration, no host round trip for the norm.
 #include <cmath>
 
 #include "common.hpp"
@@ -18,80 +24,14 @@
 namespace tigre {
 
 namespace {
-constexpr int kTvBlocks = 1024;
+constexpr int kTX = 64, kTY = 8, kZC = 16;  // tile columns, rows, planes per chunk
 constexpr int kTvThreads = 256;
 
-// Voxel loaders: the image, or the image minus a prior (PICCS).
-struct Plain {
-    const float* __restrict__ f;
-    __device__ float operator()(long i) const { return f[i]; }
-};
-struct MinusPrior {
-    const float* __restrict__ f;
-    const float* __restrict__ p;
-    __device__ float operator()(long i) const { return f[i] - p[i]; }
-};
-
-// Backward differences at (z, y, x) (GD_TV.cu:95-115): y/x terms dropped at the
-// low edge.  kPadZ: the image sits between zero slices (the single-device
-// pocs_tv / aw_pocs_tv layout, GD_TV.cu:367-368,456-465), so z always has a
-// (possibly zero) neighbour; otherwise (PICCS.cu:109-128) the z term is
-// dropped at the low edge like the others.
-template <bool kPadZ, class L>
-__device__ inline void bdiff(const L& f, long nx, long ny, long nz, long z, long y, long x, float d[3]) {
-    const long i = (z * ny + y) * nx + x;
-    const bool in = z >= 0 && z < nz;
-    const float c = in ? f(i) : 0.0f;
-    if (kPadZ)
-        d[0] = c - ((z >= 1 && z - 1 < nz) ? f(i - nx * ny) : 0.0f);
-    else
-        d[0] = (z >= 1) ? c - f(i - nx * ny) : 0.0f;
-    d[1] = (y >= 1 && in) ? c - f(i - nx) : 0.0f;
-    d[2] = (x >= 1 && in) ? c - f(i - 1) : 0.0f;
-}
-
-// (Weighted) TV sub-gradient at one voxel, GD_TV.cu:117-143 / GD_AwTV.cu:117-170
-// / PICCS.cu:132-157.  kWeighted: each difference component d carries the
-// weight exp(-(d / delta)^2) (AwTV; the reference uses the fast __expf, here
-// the accurate expf).
-template <bool kPadZ, bool kWeighted, class L>
-__device__ inline float tv_grad(const L& f, long nx, long ny, long nz, long z, long y, long x, float eps,
-                                float inv_delta) {
-    float df[3], di[3] = {0.f, 0.f, 0.f}, dj[3] = {0.f, 0.f, 0.f}, dk[3] = {0.f, 0.f, 0.f};
-    bdiff<kPadZ>(f, nx, ny, nz, z, y, x, df);
-    if (x + 1 < nx) bdiff<kPadZ>(f, nx, ny, nz, z, y, x + 1, di);
-    if (y + 1 < ny) bdiff<kPadZ>(f, nx, ny, nz, z, y + 1, x, dj);
-    if (kPadZ || z + 1 < nz) bdiff<kPadZ>(f, nx, ny, nz, z + 1, y, x, dk);
-    auto w = [&](float d) { return kWeighted ? expf(-(d * inv_delta) * (d * inv_delta)) : 1.0f; };
-    const float w0 = w(df[0]), w1 = w(df[1]), w2 = w(df[2]);
-    const float wi0 = w(di[0]), wi1 = w(di[1]), wi2 = w(di[2]);
-    const float wj0 = w(dj[0]), wj1 = w(dj[1]), wj2 = w(dj[2]);
-    const float wk0 = w(dk[0]), wk1 = w(dk[1]), wk2 = w(dk[2]);
-    float g = (w0 * df[0] + w1 * df[1] + w2 * df[2]) /
-              (sqrtf(w0 * df[0] * df[0] + w1 * df[1] * df[1] + w2 * df[2] * df[2]) + eps);
-    g -= wi2 * di[2] / (sqrtf(wi0 * di[0] * di[0] + wi1 * di[1] * di[1] + wi2 * di[2] * di[2]) + eps);
-    g -= wj1 * dj[1] / (sqrtf(wj0 * dj[0] * dj[0] + wj1 * dj[1] * dj[1] + wj2 * dj[2] * dj[2]) + eps);
-    g -= wk0 * dk[0] / (sqrtf(wk0 * dk[0] * dk[0] + wk1 * dk[1] * dk[1] + wk2 * dk[2] * dk[2]) + eps);
-    return g;
-}
-
-// Descent direction of one iteration at one voxel.  kKind 0: TV (GD_TV),
-// 1: adaptive-weighted TV (GD_AwTV), 2: PICCS, (1 - ratio) grad TV(f) + ratio
-// grad TV(f - prior) (PICCS.cu:322-347).
 struct TvParams {
-    const float* prior;
-    float inv_delta, ratio;
+    const float* prior;  // PICCS prior image
+    float inv_delta;     // AwTV 1 / delta
+    float ratio;         // PICCS prior ratio
 };
-
-template <int kKind>
-__device__ inline float tv_direction(const float* __restrict__ f, const TvParams& tp, long nx, long ny, long nz,
-                                     long z, long y, long x) {
-    if (kKind == 0) return tv_grad<true, false>(Plain{f}, nx, ny, nz, z, y, x, 1e-8f, 0.0f);
-    if (kKind == 1) return tv_grad<true, true>(Plain{f}, nx, ny, nz, z, y, x, 1e-8f, tp.inv_delta);
-    const float a = tv_grad<false, false>(Plain{f}, nx, ny, nz, z, y, x, 1e-6f, 0.0f);
-    const float b = tv_grad<false, false>(MinusPrior{f, tp.prior}, nx, ny, nz, z, y, x, 1e-6f, 0.0f);
-    return (1.0f - tp.ratio) * a + tp.ratio * b;
-}
 
 // Workgroup sum of one double per lane (wavefront shuffles, then LDS).
 __device__ inline double block_sum(double v) {
@@ -108,64 +48,178 @@ __device__ inline double block_sum(double v) {
 }
 }  // namespace
 
-template <int kKind>
-__global__ __launch_bounds__(kTvThreads) void tv_norm_kernel(const float* __restrict__ f, TvParams tp, long nx, long ny,
-                                                             long nz, double* __restrict__ partial) {
-    const long n = nx * ny * nz;
-    double s = 0.0;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const long x = q % nx, y = (q / nx) % ny, z = q / (nx * ny);
-        const float g = tv_direction<kKind>(f, tp, nx, ny, nz, z, y, x);
-        s += (double)g * (double)g;
+// kKind 0: TV (GD_TV; the image sits between zero slices, so z always has a
+// neighbour, GD_TV.cu:367-368,456-465), 1: adaptive-weighted TV (GD_AwTV,
+// same padding), 2: PICCS, (1 - ratio) g(f) + ratio g(f - prior) with the z
+// term dropped at the edges like x and y (PICCS.cu:109-157,322-347).  At the
+// low x / y edge the difference is 0; beyond the high edge n is 0
+// (GD_TV.cu:104-114).  kStep false: pass 1 (partials); true: pass 2.
+template <int kKind, bool kStep>
+__global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __restrict__ f, float* __restrict__ out,
+                                                             TvParams tp, int nx, int ny, int nz,
+                                                             double* __restrict__ partial,
+                                                             const float* __restrict__ scale_in) {
+    constexpr int NF = kKind == 2 ? 2 : 1;
+    constexpr bool kPadZ = kKind != 2;
+    constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
+    constexpr int FW = kTX + 2, FH = kTY + 2, NW = kTX + 1, NH = kTY + 1;
+    __shared__ float F[NF][2][FH][FW];       // f planes p - 1, p (ring), halo of one voxel
+    __shared__ float Nn[NF][2][3][NH][NW];   // n planes p - 1, p (ring)
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
+    const int z1 = min(z0 + kZC, nz);
+    const long long plane = (long long)nx * ny;
+    const float scale = kStep ? scale_in[0] : 0.0f;
+
+    // A plane (with halo) is fetched into registers one step before it is
+    // stored to LDS, so its loads are in flight during the previous plane's
+    // arithmetic.
+    constexpr int kPer = (FH * FW + kTvThreads - 1) / kTvThreads;
+    float pre[NF][kPer];
+    auto fetch_plane = [&](int p) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = tid + j * kTvThreads;
+            const int ly = i / FW, lx = i - ly * FW;
+            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            const bool ok = i < FH * FW && p >= 0 && p < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+            const long long idx = p * plane + (long long)gy * nx + gx;
+            pre[0][j] = ok ? f[idx] : 0.0f;
+            if (NF == 2) pre[NF - 1][j] = ok ? tp.prior[idx] : 0.0f;
+        }
+    };
+    auto store_plane = [&](int slot) {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int i = tid + j * kTvThreads;
+            if (i >= FH * FW) continue;
+            const int ly = i / FW, lx = i - ly * FW;
+            F[0][slot][ly][lx] = pre[0][j];
+            if (NF == 2) F[NF - 1][slot][ly][lx] = pre[0][j] - pre[NF - 1][j];
+        }
+    };
+    auto compute_n = [&](int p, int fs, int ps, int ns) {
+        for (int i = tid; i < NH * NW; i += kTvThreads) {
+            const int ly = i / NW, lx = i - ly * NW;
+            const int gx = x0 + lx, gy = y0 + ly;
+            const bool exists = gx < nx && gy < ny && p >= 0 && (kPadZ ? p <= nz : p < nz);
+            const bool in = p < nz;
+#pragma unroll
+            for (int k = 0; k < NF; ++k) {
+                float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+                if (exists) {
+                    const float c = in ? F[k][fs][ly + 1][lx + 1] : 0.0f;
+                    const float below = p >= 1 ? F[k][ps][ly + 1][lx + 1] : 0.0f;
+                    const float d0 = kPadZ ? c - below : (p >= 1 ? c - below : 0.0f);
+                    const float d1 = (gy >= 1 && in) ? c - F[k][fs][ly][lx + 1] : 0.0f;
+                    const float d2 = (gx >= 1 && in) ? c - F[k][fs][ly + 1][lx] : 0.0f;
+                    float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
+                    if (kKind == 1) {
+                        const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
+                        w0 = __expf(-a0 * a0);
+                        w1 = __expf(-a1 * a1);
+                        w2 = __expf(-a2 * a2);
+                    }
+                    // hardware square root and reciprocal (1 ulp each)
+                    const float inv = __builtin_amdgcn_rcpf(
+                        __builtin_amdgcn_sqrtf(w0 * d0 * d0 + w1 * d1 * d1 + w2 * d2 * d2) + eps);
+                    n0 = w0 * d0 * inv;
+                    n1 = w1 * d1 * inv;
+                    n2 = w2 * d2 * inv;
+                }
+                Nn[k][ns][0][ly][lx] = n0;
+                Nn[k][ns][1][ly][lx] = n1;
+                Nn[k][ns][2][ly][lx] = n2;
+            }
+        }
+    };
+
+    double acc = 0.0;
+    fetch_plane(z0 - 1);
+    store_plane(0);
+    fetch_plane(z0);
+    store_plane(1);
+    fetch_plane(z0 + 1);
+    __syncthreads();
+    compute_n(z0, 1, 0, 0);
+    __syncthreads();
+    for (int p = z0 + 1; p <= z1; ++p) {
+        const int fs = (p - z0 + 1) & 1, ns = (p - z0) & 1;  // slots of plane p
+        store_plane(fs);
+        if (p < z1) fetch_plane(p + 1);
+        __syncthreads();
+        compute_n(p, fs, fs ^ 1, ns);
+        __syncthreads();
+        // g at plane p - 1 for the tile: two rows per thread
+        const int tx = tid & 63;
+        for (int ty = tid >> 6; ty < kTY; ty += kTvThreads / 64) {
+            const int gx = x0 + tx, gy = y0 + ty;
+            if (gx >= nx || gy >= ny) continue;
+            const int no = ns ^ 1;
+            float g[NF];
+#pragma unroll
+            for (int k = 0; k < NF; ++k)
+                g[k] = Nn[k][no][0][ty][tx] + Nn[k][no][1][ty][tx] + Nn[k][no][2][ty][tx] -
+                       Nn[k][no][2][ty][tx + 1] - Nn[k][no][1][ty + 1][tx] - Nn[k][ns][0][ty][tx];
+            const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
+            if (kStep) {
+                const long long idx = (long long)(p - 1) * plane + (long long)gy * nx + gx;
+                out[idx] = F[0][fs ^ 1][ty + 1][tx + 1] - gv * scale;
+            } else {
+                acc += (double)gv * (double)gv;
+            }
+        }
+        __syncthreads();  // the next plane overwrites slot fs ^ 1 and n slot ns ^ 1
     }
-    s = block_sum(s);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (!kStep) {
+        acc = block_sum(acc);
+        if (tid == 0)
+            partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc;
+    }
 }
 
-// f' = f - alpha g / sqrt(sum g^2 + norm_eps); norm_eps = 1e-7 for TV and AwTV
-// (GD_TV.cu:561), 0 for PICCS (PICCS.cu:370).
-template <int kKind>
-__global__ __launch_bounds__(kTvThreads) void tv_step_kernel(const float* __restrict__ f, float* __restrict__ out,
-                                                             TvParams tp, long nx, long ny, long nz,
-                                                             const double* __restrict__ partial, int npartial,
-                                                             float alpha, double norm_eps) {
-    __shared__ float s_scale;
+// scale = alpha / sqrt(sum of partials + norm_eps), summed in a fixed order;
+// norm_eps = 1e-7 for TV and AwTV (GD_TV.cu:561), 0 for PICCS (PICCS.cu:370).
+__global__ __launch_bounds__(kTvThreads) void tv_scale_kernel(const double* __restrict__ partial, int npartial,
+                                                              float alpha, double norm_eps, float* __restrict__ scale) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < npartial; i += blockDim.x) s += partial[i];
+    for (int i = threadIdx.x; i < npartial; i += kTvThreads) s += partial[i];
     s = block_sum(s);
-    if (threadIdx.x == 0) s_scale = (float)((double)alpha / sqrt(s + norm_eps));
-    __syncthreads();
-    const float scale = s_scale;
-    const long n = nx * ny * nz;
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-        const long x = q % nx, y = (q / nx) % ny, z = q / (nx * ny);
-        out[q] = f[q] - tv_direction<kKind>(f, tp, nx, ny, nz, z, y, x) * scale;
-    }
+    if (threadIdx.x == 0) scale[0] = (float)((double)alpha / sqrt(s + norm_eps));
 }
 
 namespace {
 template <int kKind>
 int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp, double norm_eps, long nx, long ny,
                    long nz, int maxiter, hipStream_t stream) {
+    TIGRE_REQUIRE(nx > 0 && ny > 0 && nz > 0, "empty image");
+    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && ny < 65535L * kTY && nx < 65535L * kTX, "image too large");
     const size_t n = (size_t)nx * ny * nz;
     if (maxiter <= 0) {
         if (dst != img) TIGRE_HIP_TRY(hipMemcpyAsync(dst, img, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
         return TIGRE_OK;
     }
+    const dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY),
+                    (unsigned)((nz + kZC - 1) / kZC));
+    const int nblocks = (int)(grid.x * grid.y * grid.z);
     float* buf[2] = {nullptr, nullptr};
     double* partial = nullptr;
+    float* scale = nullptr;
     TIGRE_HIP_TRY(hipMallocAsync((void**)&buf[0], n * sizeof(float), stream));
     TIGRE_HIP_TRY(hipMallocAsync((void**)&buf[1], n * sizeof(float), stream));
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&partial, kTvBlocks * sizeof(double), stream));
+    TIGRE_HIP_TRY(hipMallocAsync((void**)&partial, (size_t)nblocks * sizeof(double), stream));
+    TIGRE_HIP_TRY(hipMallocAsync((void**)&scale, 256, stream));
     const float* src = img;
     int cur = 0;
     for (int it = 0; it < maxiter; ++it) {
         float* o = (it == maxiter - 1 && dst != img) ? dst : buf[cur];
         KernelTimerScope timer(TIGRE_KERNEL_POCS_TV, stream);
-        hipLaunchKernelGGL(tv_norm_kernel<kKind>, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, tp, nx, ny, nz,
-                           partial);
-        hipLaunchKernelGGL(tv_step_kernel<kKind>, dim3(kTvBlocks), dim3(kTvThreads), 0, stream, src, o, tp, nx, ny,
-                           nz, partial, kTvBlocks, alpha, norm_eps);
+        hipLaunchKernelGGL((tv_tile_kernel<kKind, false>), grid, dim3(kTvThreads), 0, stream, src, nullptr, tp,
+                           (int)nx, (int)ny, (int)nz, partial, nullptr);
+        hipLaunchKernelGGL(tv_scale_kernel, dim3(1), dim3(kTvThreads), 0, stream, partial, nblocks, alpha, norm_eps,
+                           scale);
+        hipLaunchKernelGGL((tv_tile_kernel<kKind, true>), grid, dim3(kTvThreads), 0, stream, src, o, tp, (int)nx,
+                           (int)ny, (int)nz, nullptr, scale);
         TIGRE_HIP_TRY(hipGetLastError());
         src = o;
         cur ^= 1;
@@ -174,6 +228,7 @@ int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp
     TIGRE_HIP_TRY(hipFreeAsync(buf[0], stream));
     TIGRE_HIP_TRY(hipFreeAsync(buf[1], stream));
     TIGRE_HIP_TRY(hipFreeAsync(partial, stream));
+    TIGRE_HIP_TRY(hipFreeAsync(scale, stream));
     return TIGRE_OK;
 }
 }  // namespace
