@@ -242,3 +242,23 @@ def test_minPICCS(cuda):
     print("minPICCS rel-L2 %.3e" % err)
     assert err <= TOL
     assert np.array_equal(tigre_amd.minPICCS(img, prior, 0.02, 15, ratio=0.6), got)
+
+
+def test_siddon_exact_crossing_ties(cuda):
+    """Views at odd multiples of 45 degrees whose central detector column passes
+    through voxel corners: x and y crossings coincide exactly at every step,
+    also at the planes where the beam kernel moves a ray to its next LDS slab
+    (regression: a tie at that plane must not end the slab before the
+    principal crossing)."""
+    import torch
+
+    geo = cone_geometry((40, 48, 48), (40.0, 48.0, 48.0), (40, 64), (1.0, 1.0), 300.0, 450.0)
+    geo.offDetector = np.array([0.0, 0.5])  # a pixel centre on the central ray
+    angles = np.array([np.pi / 4, 3 * np.pi / 4, 5 * np.pi / 4, 7 * np.pi / 4, 0.3, 1.9])
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 6, seed=17) + 0.1
+    got = tigre_amd.Ax(torch.from_numpy(img).to(cuda), geo, angles, "Siddon").cpu().numpy()
+    ref = oracle.ax(img, prepared(geo, angles), "Siddon")
+    assert np.all(np.isfinite(got))
+    err = oracle.rel_l2(got, ref)
+    print("Siddon with exact crossing ties rel-L2 %.3e" % err)
+    assert err <= TOL

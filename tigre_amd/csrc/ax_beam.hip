@@ -113,7 +113,7 @@ __device__ __forceinline__ unsigned long long stage_index(const StageGeom& s, in
 // t0 + k dt (recomputed exactly, never accumulated).
 struct RayWalk {
     float tx, ty, tz, t0x, t0y, t0z, dtx, dty, dtz;
-    int kx, ky, kz;
+    float kx, ky, kz;      // crossings so far per axis (exact small integers)
     unsigned off;          // voxel offset in the current source (LDS box or staging copy)
     unsigned dX, dY, dZ;   // offset steps of one crossing on each axis
     int kexit;             // principal crossing count at which the ray leaves the slab
@@ -121,53 +121,70 @@ struct RayWalk {
 };
 
 // Walks crossings until the ray leaves the volume (returns true) or its
-// principal crossing count reaches kexit (returns false).  The axis of each
-// crossing is selected without branches; every step reads one voxel, issued
-// one step before its value is needed.
-template <int kP>
-__device__ inline bool walk(RayWalk& r, const float* __restrict__ src, int guard) {
-    // locals only: a conditional between struct members would be compiled as
-    // a select of their addresses and force the struct into scratch memory
-    float tx = r.tx, ty = r.ty, tz = r.tz, tc = r.tc, sum = r.sum;
-    int kx = r.kx, ky = r.ky, kz = r.kz;
-    unsigned off = r.off;
-    const unsigned dX = r.dX, dY = r.dY, dZ = r.dZ;
-    const float T = r.T, t0x = r.t0x, t0y = r.t0y, t0z = r.t0z, dtx = r.dtx, dty = r.dty, dtz = r.dtz;
-    const int kexit = r.kexit;
-    bool fin = false;
-    float va = src[off], vb = 0.0f;
+// principal crossing count reaches kexit (returns false).  Voxel reads are
+// at offset (off + slot) from `base`, in bytes for the LDS slabs (base is the
+// shared buffer, a constant address folded into the load; slot selects the
+// half of the double buffer) and in elements for the staging copy in global
+// memory.  The axis of each crossing is selected without branches, the
+// principal axis P first on ties, then the other horizontal axis A, then z;
+// every step reads one voxel, issued one step before its value is needed.
+// Leaving the slab and leaving the volume are one test, tm >= Te with
+// Te = min(T, time of the slab's last principal crossing): that time is the
+// same fma as the crossing's own, and P wins ties, so the test can only
+// pass on the principal crossing itself (or at the volume's exit).
+template <int kP, bool kLds>
+__device__ inline bool walk(RayWalk& r, const float* __restrict__ base, unsigned slot, int guard) {
+    // locals only, named by role (P principal, A other horizontal, z): a
+    // conditional between struct members would be compiled as a select of
+    // their addresses and force the struct into scratch memory
+    float tp = kP == 0 ? r.tx : r.ty, ta = kP == 0 ? r.ty : r.tx, tz = r.tz;
+    float kp = kP == 0 ? r.kx : r.ky, ka = kP == 0 ? r.ky : r.kx, kz = r.kz;
+    const float t0p = kP == 0 ? r.t0x : r.t0y, t0a = kP == 0 ? r.t0y : r.t0x, t0z = r.t0z;
+    const float dtp = kP == 0 ? r.dtx : r.dty, dta = kP == 0 ? r.dty : r.dtx, dtz = r.dtz;
+    // LDS: byte offsets; global staging copy: element offsets (its byte offsets can pass 2^32)
+    constexpr int kSh = kLds ? 2 : 0;
+    const unsigned dP = (kP == 0 ? r.dX : r.dY) << kSh, dA = (kP == 0 ? r.dY : r.dX) << kSh, dZ = r.dZ << kSh;
+    unsigned off = (r.off + slot) << kSh;
+    float tc = r.tc, sum = r.sum;
+    const float T = r.T;
+    const float Te = r.kexit >= 1 ? fminf(T, fmaf((float)(r.kexit - 1), dtp, t0p)) : T;
+    auto read = [&](unsigned o) {
+        return kLds ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + o) : base[o];
+    };
+    float tm = tc;
+    float va = read(off), vb = 0.0f;
 #define TIGRE_BEAM_STEP(CUR, NEXT)                                       \
     {                                                                    \
-        const float tm = fminf(tx, fminf(ty, tz));                       \
-        fin = tm >= T;                                                   \
-        sum = fmaf((fin ? T : tm) - tc, CUR, sum);                       \
-        tc = tm;                                                         \
-        const bool bx = tx == tm, by = !bx && ty == tm, bz = !bx && !by; \
-        kx += bx ? 1 : 0;                                                \
-        ky += by ? 1 : 0;                                                \
-        kz += bz ? 1 : 0;                                                \
-        tx = fmaf((float)kx, dtx, t0x);                                  \
-        ty = fmaf((float)ky, dty, t0y);                                  \
-        tz = fmaf((float)kz, dtz, t0z);                                  \
-        off += bx ? dX : (by ? dY : dZ);                                 \
-        if (fin || (kP == 0 ? kx : ky) == kexit) break;                  \
-        NEXT = src[off];                                                 \
+        tm = fminf(tp, fminf(ta, tz));                                   \
+        const float tcl = fminf(tm, Te);                                 \
+        sum = fmaf(tcl - tc, CUR, sum);                                  \
+        tc = tcl;                                                        \
+        const bool bp = tp == tm, ba = ta == tm;                         \
+        kp += bp ? 1.0f : 0.0f;                                          \
+        ka += (ba && !bp) ? 1.0f : 0.0f;                                 \
+        kz += (bp || ba) ? 0.0f : 1.0f;                                  \
+        off += bp ? dP : (ba ? dA : dZ);                                 \
+        tp = fmaf(kp, dtp, t0p);                                         \
+        ta = fmaf(ka, dta, t0a);                                         \
+        tz = fmaf(kz, dtz, t0z);                                         \
+        if (tm >= Te) break;                                             \
+        NEXT = read(off);                                                \
     }
     for (; guard > 0; --guard) {
         TIGRE_BEAM_STEP(va, vb)
         TIGRE_BEAM_STEP(vb, va)
     }
 #undef TIGRE_BEAM_STEP
-    r.tx = tx;
-    r.ty = ty;
+    r.tx = kP == 0 ? tp : ta;
+    r.ty = kP == 0 ? ta : tp;
     r.tz = tz;
-    r.kx = kx;
-    r.ky = ky;
+    r.kx = kP == 0 ? kp : ka;
+    r.ky = kP == 0 ? ka : kp;
     r.kz = kz;
-    r.off = off;
+    r.off = (off >> kSh) - slot;
     r.tc = tc;
     r.sum = sum;
-    return fin;
+    return tm >= T;
 }
 
 // Workgroup table entry: views [first, first + count) of the ordered view
@@ -303,7 +320,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 r.dtx = ax.dt;
                 r.dty = ay.dt;
                 r.dtz = az.dt;
-                r.kx = r.ky = r.kz = 0;
+                r.kx = r.ky = r.kz = 0.0f;
                 r.T = (float)(aM - am);
                 r.tc = 0.0f;
                 ip0 = kP == 0 ? ax.idx : ay.idx;
@@ -416,7 +433,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 r.dY = (unsigned)(sy * (kP == 1 ? sg.NZ * sg.NA : 1));
                 r.dZ = (unsigned)(sz * sg.NA);
                 r.kexit = -1;  // never reached
-                walk<kP>(r, stage, (nx + ny + nz + 8) / 2);
+                walk<kP, false>(r, stage, 0u, (nx + ny + nz + 8) / 2);
             }
         } else if (state == 2) {
             // box origin (-1, -1, -4): the bordered staging copy itself
@@ -470,6 +487,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
             slab_range(s, planes, dir, last_cell, lo, hi);
             const int2 o = sh.origin[s];
             const float* cur = sh.buf + (s & 1) * kSlabCap;
+            const unsigned slot = (unsigned)((s & 1) * kSlabCap);
             if (kMode == 0) {
                 if (state == 2) {
                     // paused at the first plane of this slab: re-base the offset
@@ -482,7 +500,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 }
                 if (state == 2) {
                     // crossings inside one slab: its planes plus the box's a and z planes
-                    if (walk<kP>(r, cur, (planes + na + nzb + 5) / 2)) state = 0;
+                    if (walk<kP, true>(r, sh.buf, slot, (planes + na + nzb + 5) / 2)) state = 0;
                 }
             } else if (state == 2) {
                 if (sample_slab<kMode>(rs, cur, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP, (float)nA, (float)nz,
