@@ -59,9 +59,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size once (counter calibration)")
-    ap.add_argument("--workload", choices=["pair", "batched", "learned"], default="pair",
+    ap.add_argument("--workload", choices=["pair", "batched", "learned", "precision"], default="pair",
                     help="pair: Ax+Atb at 512^3 (default); batched: 64 uneven OS-SART cases in one graph; "
-                         "learned: ASD-POCS with the learned proximal step vs plain TV at 256^3")
+                         "learned: ASD-POCS with the learned proximal step vs plain TV at 256^3; "
+                         "precision: BF16 / FP16 / FP32 storage at 1024^3 with CGLS")
     ap.add_argument("--train-steps", type=int, default=1000, help="learned: Adam steps")
     ap.add_argument("--cases", type=int, default=64, help="batched: number of cases")
     ap.add_argument("--iters", type=int, default=10, help="batched: OS-SART iterations per case")
@@ -76,6 +77,9 @@ def main():
         return
     if args.workload == "learned":
         bench_learned(args)
+        return
+    if args.workload == "precision":
+        bench_precision(args)
         return
 
     import numpy as np
@@ -438,6 +442,111 @@ def bench_learned(args):
                      "issued_tflop": round(flops_exec / 1e12, 3)},
     }
     print(json.dumps(result), flush=True)
+
+
+def bench_precision(args):
+    """Storage precision at scale: 1024^3 volume (sVoxel 256 mm, 0.25 mm voxels),
+    720 views over 2*pi, 1024 x 768 detector at 0.4 mm, DSO 1000 / DSD 1536 mm,
+    per-view detector offsets drawn from U[-2, 2] pixels on both axes (seeded);
+    seeded 30-ellipsoid phantom, projections from Ax.  For FP32, BF16 and FP16
+    storage (FP32 arithmetic): Ax Siddon and Atb matched timed and compared with
+    the FP32 result and, on 8 views, with the FP64 oracle; then 30 CGLS
+    iterations (FP64 inner products), the final iterate compared with the FP32
+    run's and with the phantom."""
+    import numpy as np
+    import torch
+
+    import oracle
+    import tigre_amd
+    from tigre_amd import algorithms as algs
+    from tigre_amd.geometry import cone_geometry, prepared
+    from tigre_amd.phantom import ellipsoid_phantom_torch
+
+    nx, nviews, niter = 1024, 720, 30
+    geo = cone_geometry((nx, nx, nx), (256.0,) * 3, (768, 1024), (0.4, 0.4), 1000.0, 1536.0)
+    rng = np.random.default_rng(20261202 + 3)
+    geo.offDetector = rng.uniform(-2.0, 2.0, (nviews, 2)) * np.asarray(geo.dDetector)
+    angles = np.linspace(0, 2 * np.pi, nviews, endpoint=False)
+    img = ellipsoid_phantom_torch(geo.nVoxel, geo.sVoxel, 30, seed=20261202 + 3, device="cuda")
+    proj = tigre_amd.Ax(img, geo, angles, "Siddon")
+    torch.cuda.synchronize()
+    log("precision: phantom and data ready")
+
+    def rel(a, b):
+        a, b = a.double(), b.double()
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    def clock(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    sub = np.array([k * nviews // 8 for k in range(8)])
+    img_h = img.cpu().numpy()
+    gp = prepared(copy_geo(geo, sub), angles[sub])
+    t = time.perf_counter()
+    ref_ax = oracle.ax(img_h, gp, "Siddon")
+    ref_atb = oracle.atb(proj[torch.as_tensor(sub, device="cuda")].cpu().numpy(), gp, "matched")
+    t_oracle = time.perf_counter() - t
+    log("oracle on 8 views: %.1f s" % t_oracle)
+    res = {}
+    x32 = None
+    for name, dt in (("f32", torch.float32), ("bf16", torch.bfloat16), ("f16", torch.float16)):
+        imgd, projd = img.to(dt), proj.to(dt)
+        ax, t_ax = clock(lambda: tigre_amd.Ax(imgd, geo, angles, "Siddon"))
+        atb, t_atb = clock(lambda: tigre_amd.Atb(projd, geo, angles, "matched"))
+        idx = torch.as_tensor(sub, device="cuda")
+        subgeo = copy_geo(geo, sub)
+        atb_sub = tigre_amd.Atb(projd[idx].contiguous(), subgeo, angles[sub], "matched")
+        entry = {"ax_ms": round(t_ax * 1e3, 2), "atb_matched_ms": round(t_atb * 1e3, 2),
+                 "ax_rel_l2_vs_f32": rel(ax.float(), proj), "ax_rel_l2_vs_fp64_8views":
+                     oracle.rel_l2(ax[idx].float().cpu().numpy(), ref_ax),
+                 "atb_rel_l2_vs_fp64_8views": oracle.rel_l2(atb_sub.float().cpu().numpy(), ref_atb)}
+        del ax, atb, atb_sub
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x = algs.cgls(proj, geo, angles, niter, storage=dt, return_tensor=True)
+        torch.cuda.synchronize()
+        entry["cgls_s"] = round(time.perf_counter() - t, 2)
+        xf = x.float()
+        if name == "f32":
+            x32 = xf.clone()
+        entry["cgls_rel_l2_vs_f32_run"] = rel(xf, x32)
+        entry["cgls_rel_l2_vs_phantom"] = rel(xf, img)
+        res[name] = entry
+        log("%s: %s" % (name, entry))
+        del x, xf, imgd, projd
+        torch.cuda.empty_cache()
+    V, n = float(nx) ** 3, nviews
+    result = {
+        "metric": "storage precision at 1024^3 / 720 views: rel-L2 of Ax, Atb and CGLS(30) per storage type",
+        "value": res["bf16"]["cgls_rel_l2_vs_f32_run"], "unit": "rel-L2 (BF16 CGLS iterate vs FP32 run)",
+        "n_gpus": 1, "steps": niter, "warmup": 1, "ms_per_step": round(res["f32"]["cgls_s"] * 1e3 / niter, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f16/f32 storage, f32 math",
+        "data": "synthetic (seeded 30-ellipsoid phantom, projections from Ax)",
+        "config": {"workload": "1024^3 (0.25 mm), 720 views, 1024x768 detector at 0.4 mm, jittered offsets, "
+                               "CGLS 30 iterations", "oracle_views": [int(i) for i in sub],
+                   "oracle_seconds": round(t_oracle, 1)},
+        "by_storage": res,
+        "gupdates_per_s_ax_f32": round(V * n / (res["f32"]["ax_ms"] * 1e-3) / 1e9, 1),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def copy_geo(geo, idx):
+    """The geometry restricted to views idx (per-view arrays sliced)."""
+    import copy
+
+    import numpy as np
+
+    g = copy.deepcopy(geo)
+    off = np.asarray(g.offDetector)
+    if off.ndim == 2:
+        g.offDetector = off[idx]
+    return g
 
 
 if __name__ == "__main__":
