@@ -98,6 +98,22 @@ int tigre_atb(const float* projections, const TigreGeometry* geo, float* result,
               const float* angles, int nangles, int backprojection_type, int layout,
               const int* gpuids, int ngpus);
 
+/* Parallel beam.  Replace siddon_ray_projection_parallel
+ * (Common/CUDA/Siddon_projection_parallel.hpp:56) and
+ * interpolation_projection_parallel (ray_interpolated_projection_parallel.hpp:57),
+ * same arguments as tigre_ax: every pixel has its own source, so the rays of a
+ * view are parallel (geometry mode 'parallel', _Ax.pyx:67-87). */
+int tigre_ax_parallel(const float* img, const TigreGeometry* geo, float** result,
+                      const float* angles, int nangles, int projection_type, int layout,
+                      const int* gpuids, int ngpus);
+
+/* Replaces voxel_backprojection_parallel (voxel_backprojection_parallel.hpp:53):
+ * each voxel is projected orthogonally onto the detector and sampled
+ * bilinearly, weight 1.  The reference uses it for 'FDK' and 'matched' alike
+ * (_Atb.pyx:56-76), so there is no backprojector argument. */
+int tigre_atb_parallel(const float* projections, const TigreGeometry* geo, float* result,
+                       const float* angles, int nangles, int layout, const int* gpuids, int ngpus);
+
 /* Replaces pocs_tv (Common/CUDA/GD_TV.hpp:58).  image_size = {X, Y, Z}. */
 int tigre_pocs_tv(const float* img, float* dst, float alpha, const long* image_size,
                   int maxIter, const int* gpuids, int ngpus);
@@ -142,6 +158,13 @@ int tigre_ax_device_typed(const void* d_img, const TigreGeometry* geo, void* d_p
 int tigre_atb_device_typed(const void* d_proj, const TigreGeometry* geo, void* d_vol,
                            const float* angles, int nangles, int backprojection_type, int layout,
                            int dtype, void* stream);
+
+/* Device-pointer forms of tigre_ax_parallel / tigre_atb_parallel (storage type dtype). */
+int tigre_ax_parallel_device(const void* d_img, const TigreGeometry* geo, void* d_proj,
+                             const float* angles, int nangles, int projection_type, int layout,
+                             int dtype, void* stream);
+int tigre_atb_parallel_device(const void* d_proj, const TigreGeometry* geo, void* d_vol,
+                              const float* angles, int nangles, int layout, int dtype, void* stream);
 
 /* maxIter TV steepest-descent steps on a device volume (X fastest); d_dst may
  * equal d_img.  image_size = {X, Y, Z}. */

@@ -55,11 +55,13 @@ def lib():
         L = ctypes.CDLL(LIB)
         vp = ctypes.c_void_p
         g = ctypes.POINTER(OgGeo)
-        for name in ("og_ax_siddon", "og_ax_interp", "og_atb_fdk", "og_atb_matched"):
+        for name in ("og_ax_siddon", "og_ax_interp", "og_atb_fdk", "og_atb_matched", "og_ax_siddon_parallel",
+                     "og_ax_interp_parallel", "og_atb_parallel"):
             fn = getattr(L, name)
             fn.argtypes = [vp, g, ctypes.c_int, vp]
             fn.restype = None
         L.og_forward_frame.argtypes = [g, ctypes.c_int, ctypes.c_int, vp]
+        L.og_forward_frame_parallel.argtypes = [g, ctypes.c_int, ctypes.c_int, vp]
         L.og_backward_frame.argtypes = [g, ctypes.c_int, vp]
         L.og_maxdistance_cuboid.argtypes = [g, ctypes.c_int]
         L.og_maxdistance_cuboid.restype = ctypes.c_double
@@ -129,13 +131,21 @@ def _vp(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _parallel(geo):
+    return getattr(geo, "mode", "cone") == "parallel"
+
+
 def ax(img, geo, projection_type="Siddon", layout_matlab=False):
-    """FP64 forward projection; returns (n, nV, nU) (or (n, nU, nV) for MATLAB layout)."""
+    """FP64 forward projection; returns (n, nV, nU) (or (n, nU, nV) for MATLAB layout).
+    geo.mode 'parallel' selects the parallel-beam restatements."""
     G = _Geo(geo)
     img = np.ascontiguousarray(img, dtype=np.float32)
     shape = (G.n, G.nd[1], G.nd[0]) if layout_matlab else (G.n, G.nd[0], G.nd[1])
     out = np.empty(shape, dtype=np.float64)
-    fn = lib().og_ax_interp if projection_type == "interpolated" else lib().og_ax_siddon
+    if _parallel(geo):
+        fn = lib().og_ax_interp_parallel if projection_type == "interpolated" else lib().og_ax_siddon_parallel
+    else:
+        fn = lib().og_ax_interp if projection_type == "interpolated" else lib().og_ax_siddon
     fn(_vp(img), ctypes.byref(G.s), int(layout_matlab), _vp(out))
     return out
 
@@ -145,7 +155,10 @@ def atb(proj, geo, backprojection_type="FDK", layout_matlab=False):
     G = _Geo(geo)
     proj = np.ascontiguousarray(proj, dtype=np.float32)
     out = np.empty((G.nv[0], G.nv[1], G.nv[2]), dtype=np.float64)
-    fn = lib().og_atb_matched if backprojection_type == "matched" else lib().og_atb_fdk
+    if _parallel(geo):
+        fn = lib().og_atb_parallel  # one parallel-beam backprojector for 'FDK' and 'matched'
+    else:
+        fn = lib().og_atb_matched if backprojection_type == "matched" else lib().og_atb_fdk
     fn(_vp(proj), ctypes.byref(G.s), int(layout_matlab), _vp(out))
     return out
 
@@ -153,7 +166,8 @@ def atb(proj, geo, backprojection_type="FDK", layout_matlab=False):
 def forward_frame(geo, view, voxel_centre_origin):
     G = _Geo(geo)
     out = np.empty(12)
-    lib().og_forward_frame(ctypes.byref(G.s), int(view), int(voxel_centre_origin), _vp(out))
+    fn = lib().og_forward_frame_parallel if _parallel(geo) else lib().og_forward_frame
+    fn(ctypes.byref(G.s), int(view), int(voxel_centre_origin), _vp(out))
     return out.reshape(4, 3)
 
 

@@ -396,6 +396,186 @@ void og_ax_interp(const float* img, const og_geo* g, int layout_matlab, double* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Parallel beam                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Parallel-beam frames.  Siddon (voxel_centre_origin = 0):
+ * computeDeltas_Siddon_parallel, Siddon_projection_parallel.cu:388-470 --
+ * detector points with x = 0 go through roll/pitch/yaw, x -= DSD - DSO, the
+ * detector offset is added once, then a rotation by alpha alone.  Interpolated
+ * (voxel_centre_origin = 1): computeDeltas_parallel,
+ * ray_interpolated_projection_parallel.cu:313-390 -- the detector offset is
+ * added twice and the full ZYZ rotation is applied.  The source is pixel
+ * (0, 0)'s detector point at x = 0 through roll/pitch/yaw, then x += DSO, with
+ * no detector offset; every pixel's source is this source plus the pixel's
+ * u / v steps.  Output as og_forward_frame. */
+void og_forward_frame_parallel(const og_geo* g, int i, int voxel_centre_origin, double out[12]) {
+    double a = g->angles[3 * i], t = g->angles[3 * i + 1], p = g->angles[3 * i + 2];
+    double DSO = g->DSO[i], DSD = g->DSD[i];
+    d3 P = {0.0, g->dDetecU * (-((double)g->nDetecU / 2.0) + 0.5),
+            g->dDetecV * (((double)g->nDetecV / 2.0) - 0.5)};
+    d3 S = P;
+    d3 Pu = {0.0, g->dDetecU, 0.0};
+    d3 Pv = {0.0, 0.0, -g->dDetecV};
+    P = roll_pitch_yaw(g->dRoll[i], g->dPitch[i], g->dYaw[i], P);
+    Pu = roll_pitch_yaw(g->dRoll[i], g->dPitch[i], g->dYaw[i], Pu);
+    Pv = roll_pitch_yaw(g->dRoll[i], g->dPitch[i], g->dYaw[i], Pv);
+    S = roll_pitch_yaw(g->dRoll[i], g->dPitch[i], g->dYaw[i], S);
+    P.x -= (DSD - DSO);
+    S.x += DSO;
+    int times = voxel_centre_origin ? 2 : 1;
+    P.y += times * g->offDetecU[i];
+    P.z += times * g->offDetecV[i];
+    if (voxel_centre_origin) {
+        P = euler_zyz(a, t, p, P);
+        Pu = euler_zyz(a, t, p, Pu);
+        Pv = euler_zyz(a, t, p, Pv);
+        S = euler_zyz(a, t, p, S);
+    } else {
+        P = euler_zyz(a, 0.0, 0.0, P);
+        Pu = euler_zyz(a, 0.0, 0.0, Pu);
+        Pv = euler_zyz(a, 0.0, 0.0, Pv);
+        S = euler_zyz(a, 0.0, 0.0, S);
+    }
+    double shx = g->sVoxelX / 2.0 - (voxel_centre_origin ? g->dVoxelX / 2.0 : 0.0);
+    double shy = g->sVoxelY / 2.0 - (voxel_centre_origin ? g->dVoxelY / 2.0 : 0.0);
+    double shz = g->sVoxelZ / 2.0 - (voxel_centre_origin ? g->dVoxelZ / 2.0 : 0.0);
+    P.x = (P.x - g->offOrigX[i] + shx) / g->dVoxelX;
+    P.y = (P.y - g->offOrigY[i] + shy) / g->dVoxelY;
+    P.z = (P.z - g->offOrigZ[i] + shz) / g->dVoxelZ;
+    S.x = (S.x - g->offOrigX[i] + shx) / g->dVoxelX;
+    S.y = (S.y - g->offOrigY[i] + shy) / g->dVoxelY;
+    S.z = (S.z - g->offOrigZ[i] + shz) / g->dVoxelZ;
+    Pu.x /= g->dVoxelX; Pu.y /= g->dVoxelY; Pu.z /= g->dVoxelZ;
+    Pv.x /= g->dVoxelX; Pv.y /= g->dVoxelY; Pv.z /= g->dVoxelZ;
+    double corx = -g->COR[i] * sin(a) / g->dVoxelX;
+    double cory = g->COR[i] * cos(a) / g->dVoxelY;
+    P.x += corx; P.y += cory;
+    S.x += corx; S.y += cory;
+    out[0] = P.x;  out[1] = P.y;  out[2] = P.z;
+    out[3] = Pu.x; out[4] = Pu.y; out[5] = Pu.z;
+    out[6] = Pv.x; out[7] = Pv.y; out[8] = Pv.z;
+    out[9] = S.x;  out[10] = S.y; out[11] = S.z;
+}
+
+/* One parallel-beam Siddon ray, kernelPixelDetector_parallel
+ * (Siddon_projection_parallel.cu:150-268): crossings of the x and y planes
+ * only, the z index fixed at the first segment's midpoint, the ray length
+ * without its z component.  For the usual horizontal rays (no detector pitch
+ * or roll) this is the full 3D Siddon sum. */
+static double siddon_ray_parallel(const float* img, const og_geo* g, d3 S, d3 P, double* work) {
+    const int nx = g->nVoxelX, ny = g->nVoxelY, nz = g->nVoxelZ;
+    d3 r = {P.x - S.x, P.y - S.y, P.z - S.z};
+    double lo, hi, am = -INFINITY, aM = INFINITY;
+    if (!slab_limits(S.x, r.x, nx, &lo, &hi)) return 0.0;
+    am = fmax(am, lo); aM = fmin(aM, hi);
+    if (!slab_limits(S.y, r.y, ny, &lo, &hi)) return 0.0;
+    am = fmax(am, lo); aM = fmin(aM, hi);
+    if (!(am < aM)) return 0.0;
+    double* ax = work;
+    double* ay = ax + (nx + 1);
+    int cx = axis_crossings(S.x, r.x, nx, am, aM, ax);
+    int cy = axis_crossings(S.y, r.y, ny, am, aM, ay);
+    double first = aM;
+    if (cx > 0 && ax[0] < first) first = ax[0];
+    if (cy > 0 && ay[0] < first) first = ay[0];
+    long k = (long)floor(S.z + 0.5 * (am + first) * r.z);
+    if (k < 0 || k >= nz) return 0.0;
+    int ix = 0, iy = 0;
+    double prev = am, sum = 0.0;
+    for (;;) {
+        double nxt = aM;
+        int which = -1;
+        if (ix < cx && ax[ix] < nxt) { nxt = ax[ix]; which = 0; }
+        if (iy < cy && ay[iy] < nxt) { nxt = ay[iy]; which = 1; }
+        if (nxt > prev) {
+            double mid = 0.5 * (prev + nxt);
+            long i = (long)floor(S.x + mid * r.x);
+            long j = (long)floor(S.y + mid * r.y);
+            if (i >= 0 && i < nx && j >= 0 && j < ny)
+                sum += (nxt - prev) * (double)img[((size_t)k * ny + j) * nx + i];
+            prev = nxt;
+        }
+        if (which == 0) ++ix;
+        else if (which == 1) ++iy;
+        else break;
+    }
+    return sum * sqrt(r.x * r.x * g->dVoxelX * g->dVoxelX + r.y * r.y * g->dVoxelY * g->dVoxelY);
+}
+
+void og_ax_siddon_parallel(const float* img, const og_geo* g, int layout_matlab, double* out) {
+    const int nU = g->nDetecU, nV = g->nDetecV;
+    for (int view = 0; view < g->nangles; ++view) {
+        double f[12];
+        og_forward_frame_parallel(g, view, 0, f);
+        d3 O = {f[0], f[1], f[2]}, DU = {f[3], f[4], f[5]}, DV = {f[6], f[7], f[8]}, S0 = {f[9], f[10], f[11]};
+#pragma omp parallel
+        {
+            double* work = (double*)malloc(sizeof(double) * (size_t)(g->nVoxelX + g->nVoxelY + 2));
+#pragma omp for schedule(dynamic, 4)
+            for (int v = 0; v < nV; ++v) {
+                double pv = (double)(nV - v - 1);
+                for (int u = 0; u < nU; ++u) {
+                    d3 sh = {u * DU.x + pv * DV.x, u * DU.y + pv * DV.y, u * DU.z + pv * DV.z};
+                    d3 P = {O.x + sh.x, O.y + sh.y, O.z + sh.z};
+                    d3 S = {S0.x + sh.x, S0.y + sh.y, S0.z + sh.z};
+                    out[proj_index(g, view, u, v, layout_matlab)] = siddon_ray_parallel(img, g, S, P, work);
+                }
+            }
+            free(work);
+        }
+    }
+}
+
+/* kernelPixelDetector_parallel of ray_interpolated_projection_parallel.cu
+ * :117-191: samples from the pixel's own source, the first at
+ * floor(crop / accuracy), the last at min(n, ceil((2 DSO / dVoxelX + crop) /
+ * accuracy)). */
+void og_ax_interp_parallel(const float* img, const og_geo* g, int layout_matlab, double* out) {
+    const int nU = g->nDetecU, nV = g->nDetecV;
+    const int nx = g->nVoxelX, ny = g->nVoxelY, nz = g->nVoxelZ;
+    const double acc = g->accuracy;
+    const int point = acc > 1.0;
+    for (int view = 0; view < g->nangles; ++view) {
+        double f[12];
+        og_forward_frame_parallel(g, view, 1, f);
+        d3 O = {f[0], f[1], f[2]}, DU = {f[3], f[4], f[5]}, DV = {f[6], f[7], f[8]}, S0 = {f[9], f[10], f[11]};
+        double crop = floor(og_maxdistance_cuboid(g, view));
+        double cap = (2.0 * g->DSO[view] / g->dVoxelX + crop) / acc;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int v = 0; v < nV; ++v) {
+            double pv = (double)(nV - v - 1);
+            for (int u = 0; u < nU; ++u) {
+                d3 sh = {u * DU.x + pv * DV.x, u * DU.y + pv * DV.y, u * DU.z + pv * DV.z};
+                d3 S = {S0.x + sh.x, S0.y + sh.y, S0.z + sh.z};
+                /* the pixel shifts of source and pixel cancel: every ray is O - S0 */
+                d3 d = {O.x - S0.x, O.y - S0.y, O.z - S0.z};
+                /* |d| / acc is often an exact integer here (|d| = DSD / dVoxel); values
+                 * within 1e-9 of an integer count as that integer */
+                double n = ceil(sqrt(d.x * d.x + d.y * d.y + d.z * d.z) / acc - 1e-9);
+                d3 st = {d.x / n, d.y / n, d.z / n};
+                double last = (cap < n) ? ceil(cap) : n;
+                double first = floor(crop / acc);
+                double lo, hi, wlo = -INFINITY, whi = INFINITY;
+                sample_window(S.x, st.x, nx, &lo, &hi); wlo = fmax(wlo, lo); whi = fmin(whi, hi);
+                sample_window(S.y, st.y, ny, &lo, &hi); wlo = fmax(wlo, lo); whi = fmin(whi, hi);
+                sample_window(S.z, st.z, nz, &lo, &hi); wlo = fmax(wlo, lo); whi = fmin(whi, hi);
+                double i0 = fmax(first, floor(wlo) - 1.0);
+                double i1 = fmin(last, ceil(whi) + 1.0);
+                double sum = 0.0;
+                for (double i = i0; i <= i1; i += 1.0) {
+                    double x = S.x + i * st.x, y = S.y + i * st.y, z = S.z + i * st.z;
+                    sum += point ? nearest0(img, nx, ny, nz, x, y, z) : trilinear0(img, nx, ny, nz, x, y, z);
+                }
+                double dl = sqrt(st.x * g->dVoxelX * st.x * g->dVoxelX + st.y * g->dVoxelY * st.y * g->dVoxelY +
+                                 st.z * g->dVoxelZ * st.z * g->dVoxelZ);
+                out[proj_index(g, view, u, v, layout_matlab)] = sum * dl;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Backprojectors                                                            */
 /* ------------------------------------------------------------------------ */
 
@@ -477,6 +657,33 @@ static void atb_common(const float* proj, const og_geo* g, int layout_matlab, do
         double c = g->dVoxelX * g->dVoxelY * g->dVoxelZ / (g->dDetecU * g->dDetecV);
 #pragma omp parallel for schedule(static)
         for (long q = 0; q < (long)nvox; ++q) out[q] *= c;
+    }
+}
+
+/* kernelPixelBackprojection_parallel, voxel_backprojection_parallel.cu:138-281:
+ * the voxel (frame of computeDeltasCubeParallel, :510-585, the same as the
+ * cone backprojectors' frame) projects orthogonally, texture coordinates
+ * (y + nU / 2, z + nV / 2) with y shifted by -COR / dDetecU; weight 1, no
+ * final scaling. */
+void og_atb_parallel(const float* proj, const og_geo* g, int layout_matlab, double* out) {
+    const int nx = g->nVoxelX, ny = g->nVoxelY, nz = g->nVoxelZ;
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t P = (size_t)g->nDetecU * g->nDetecV;
+    memset(out, 0, nvox * sizeof(double));
+    for (int view = 0; view < g->nangles; ++view) {
+        double f[15];
+        og_backward_frame(g, view, f);
+        const double COR = g->COR[view];
+        const float* pj = proj + (size_t)view * P;
+#pragma omp parallel for schedule(static)
+        for (long kz = 0; kz < nz; ++kz)
+            for (long jy = 0; jy < ny; ++jy)
+                for (long ix = 0; ix < nx; ++ix) {
+                    double Py = f[1] + ix * f[4] + jy * f[7] + kz * f[10] - COR / g->dDetecU;
+                    double Pz = f[2] + ix * f[5] + jy * f[8] + kz * f[11];
+                    out[((size_t)kz * ny + jy) * nx + ix] +=
+                        bilinear0(pj, g, layout_matlab, Py + (double)g->nDetecU * 0.5, Pz + (double)g->nDetecV * 0.5);
+                }
     }
 }
 

@@ -42,6 +42,12 @@ typedef struct og_geo {
     const double* dYaw;
 } og_geo;
 
+/* Parallel-beam forms (the reference's *_parallel functions). */
+void og_forward_frame_parallel(const og_geo* g, int view, int voxel_centre_origin, double out[12]);
+void og_ax_siddon_parallel(const float* img, const og_geo* g, int layout_matlab, double* out);
+void og_ax_interp_parallel(const float* img, const og_geo* g, int layout_matlab, double* out);
+void og_atb_parallel(const float* proj, const og_geo* g, int layout_matlab, double* out);
+
 /* Per-view frames (exposed so tests can pin them against closed forms). */
 void og_forward_frame(const og_geo* g, int view, int voxel_centre_origin,
                       double out[12]);  /* uvOrigin, deltaU, deltaV, source */

@@ -63,6 +63,17 @@ SIGNATURES = {
     "tigre_atb": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_void_p, ctypes.c_int]),
+    "tigre_ax_parallel": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.POINTER(_fp),
+                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int]),
+    "tigre_atb_parallel": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
+    "tigre_ax_parallel_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_void_p]),
+    "tigre_atb_parallel_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(TigreGeometry), ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_void_p]),
     "tigre_pocs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_int]),
     "tigre_aw_pocs_tv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,

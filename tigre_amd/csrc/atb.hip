@@ -34,7 +34,11 @@ __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // volumes).  With ep.x set, the result is not stored but applied as the SART
 // update x + inv_v[iy, ix] * result, clamped at 0 when ep.noneg, written to
 // vol (which may alias ep.x: each voxel is read and written by one lane).
-template <bool kMatched, bool kAxial, int kZ>
+// kW: 0 FDK weight (voxel_backprojection.cu:241-249), 1 matched weight
+// (voxel_backprojection2.cu:201-279), 2 parallel beam: the voxel projects
+// orthogonally, u = y + nU/2 - 0.5, v = z + nV/2 - 0.5, weight 1
+// (voxel_backprojection_parallel.cu:221-251).
+template <int kW, bool kAxial, int kZ>
 __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
                                                   const BwdView* __restrict__ views, float* vol, int nx,
                                                   int ny, int nz, float dvx, float dvy, float dvz, float dU, float dV,
@@ -62,9 +66,10 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
         const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
         const float bz = fmaf(fy, w.dY[2], fmaf(fx, w.dX[2], w.org[2]));
+        constexpr bool kMatched = kW == 1, kParallel = kW == 2;
         // FDK distance weight, constant along the column (voxel_backprojection.cu:241-249)
-        float wfdk = 0.0f;
-        if (!kMatched) {
+        float wfdk = 1.0f;
+        if (kW == 0) {
             const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
             const float ww = (w.DSO + ry * w.sina - rx * w.cosa) * w.invDSO;
             wfdk = rcp(ww * ww);
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             Axy = ex * ex + ey * ey;
         }
         if (kAxial) {
-            const float t = w.tnum * rcp(bx);
+            const float t = kParallel ? 1.0f : w.tnum * rcp(bx);
             const float pu_raw = fmaf(by, t, w.uoff);
             const float pu = clampf(pu_raw, hu);
             const float flu = floorf(pu);
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             for (int k = 0; k < kZ; ++k) {
                 const float fz = (float)(z0 + k);
                 const float vx = fmaf(fz, w.dZ[0], bx), vy = fmaf(fz, w.dZ[1], by), vz = fmaf(fz, w.dZ[2], bz);
-                const float t = w.tnum * rcp(vx);
+                const float t = kParallel ? 1.0f : w.tnum * rcp(vx);
                 const float pu_raw = fmaf(vy, t, w.uoff), pv_raw = fmaf(vz, t, w.voff);
                 const float pu = clampf(pu_raw, hu);
                 const float pv = clampf(pv_raw, hv);
@@ -203,15 +208,18 @@ int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nvi
 
 int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views, bool matched, bool axial,
                float* vol, int zper, const AtbUpdate& ep, hipStream_t stream) {
+    // the parallel-beam backprojector serves both 'FDK' and 'matched' (_Atb.pyx:56-76)
+    const int kind = g.parallel ? 2 : (matched ? 1 : 0);
     TIGRE_REQUIRE(zper == 4 || zper == 8 || zper == 16, "slices per lane must be 4, 8 or 16");
     const int nx = g.nVoxel[0], ny = g.nVoxel[1], nz = g.nVoxel[2];
     dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + zper - 1) / zper);
     TIGRE_REQUIRE(grid.z <= 65535u && grid.y <= 65535u, "volume too large for one launch");
     // matched: final scaling by dVx dVy dVz / (dU dV) (voxel_backprojection2.cu:585)
-    const float scale = matched ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
+    const float scale =
+        kind == 1 ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
     const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
                         (float)g.dDetec[1], scale, 0.0f};
-    KernelTimerScope timer(matched ? TIGRE_KERNEL_ATB_MATCHED : TIGRE_KERNEL_ATB_FDK, stream);
+    KernelTimerScope timer(kind == 1 ? TIGRE_KERNEL_ATB_MATCHED : TIGRE_KERNEL_ATB_FDK, stream);
 #define TIGRE_ATB_LAUNCH(M, A, Z)                                                                                \
     hipLaunchKernelGGL((atb_kernel<M, A, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],     \
                        g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5], ep)
@@ -221,12 +229,15 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
         else if (zper == 8) TIGRE_ATB_LAUNCH(M, A, 8);     \
         else TIGRE_ATB_LAUNCH(M, A, 4);                    \
     } while (0)
-    if (matched) {
-        if (axial) TIGRE_ATB_Z(true, true);
-        else TIGRE_ATB_Z(true, false);
+    if (kind == 1) {
+        if (axial) TIGRE_ATB_Z(1, true);
+        else TIGRE_ATB_Z(1, false);
+    } else if (kind == 0) {
+        if (axial) TIGRE_ATB_Z(0, true);
+        else TIGRE_ATB_Z(0, false);
     } else {
-        if (axial) TIGRE_ATB_Z(false, true);
-        else TIGRE_ATB_Z(false, false);
+        if (axial) TIGRE_ATB_Z(2, true);
+        else TIGRE_ATB_Z(2, false);
     }
 #undef TIGRE_ATB_Z
 #undef TIGRE_ATB_LAUNCH

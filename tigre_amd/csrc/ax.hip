@@ -86,9 +86,12 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     if (c.u >= nU || c.v >= nV || c.view >= nviews) return;
     const FwdView& w = views[c.view];
     const double pv = (double)(nV - 1 - c.v);  // detector row flip, :141
-    double rx = w.org[0] + c.u * w.du[0] + pv * w.dv[0] - w.src[0];
-    double ry = w.org[1] + c.u * w.du[1] + pv * w.dv[1] - w.src[1];
-    double rz = w.org[2] + c.u * w.du[2] + pv * w.dv[2] - w.src[2];
+    double sp[3];
+    ray_source(w, c.u, pv, sp);
+    double rx = ray_dir(w, c.u, pv, 0);
+    double ry = ray_dir(w, c.u, pv, 1);
+    double rz = ray_dir(w, c.u, pv, 2);
+    if (w.pad[1] != 0.0) parallel_siddon_height(sp, rx, ry, rz, nx, ny);
     // components below 1e-3 voxel are treated as exactly zero (:155-159)
     if (fabs(rx) < 1e-3) rx = 0.0;
     if (fabs(ry) < 1e-3) ry = 0.0;
@@ -96,16 +99,16 @@ __global__ __launch_bounds__(256) void ax_siddon_kernel(const float* __restrict_
     float* dst = out + (long long)w.pad[0] * sview + (long long)c.u * su + (long long)c.v * sv;
 
     double am = -INFINITY, aM = INFINITY;
-    bool hit = slab(w.src[0], rx, nx, am, aM);
-    hit = hit && slab(w.src[1], ry, ny, am, aM);
-    hit = hit && slab(w.src[2], rz, nz, am, aM);
+    bool hit = slab(sp[0], rx, nx, am, aM);
+    hit = hit && slab(sp[1], ry, ny, am, aM);
+    hit = hit && slab(sp[2], rz, nz, am, aM);
     if (!hit || !(am < aM)) {  // a ray that misses the volume projects to 0 (:179-180)
         *dst = 0.0f;
         return;
     }
-    const Axis ax = axis_setup(w.src[0], rx, am);
-    const Axis ay = axis_setup(w.src[1], ry, am);
-    const Axis az = axis_setup(w.src[2], rz, am);
+    const Axis ax = axis_setup(sp[0], rx, am);
+    const Axis ay = axis_setup(sp[1], ry, am);
+    const Axis az = axis_setup(sp[2], rz, am);
     const float T = (float)(aM - am);
 
     // padded coordinates (index + 1) and their brick offsets
@@ -179,13 +182,15 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     if (c.u >= nU || c.v >= nV || c.view >= nviews) return;
     const FwdView& w = views[c.view];
     const double pv = (double)(nV - 1 - c.v);
-    const double dx = w.org[0] + c.u * w.du[0] + pv * w.dv[0] - w.src[0];
-    const double dy = w.org[1] + c.u * w.du[1] + pv * w.dv[1] - w.src[1];
-    const double dz = w.org[2] + c.u * w.du[2] + pv * w.dv[2] - w.src[2];
+    double sp[3];
+    ray_source(w, c.u, pv, sp);
+    const double dx = ray_dir(w, c.u, pv, 0);
+    const double dy = ray_dir(w, c.u, pv, 1);
+    const double dz = ray_dir(w, c.u, pv, 2);
     float* dst = out + (long long)w.pad[0] * sview + (long long)c.u * su + (long long)c.v * sv;
 
     // samples per ray and step (:164-169)
-    const double n = ceil(sqrt(dx * dx + dy * dy + dz * dz) / acc);
+    const double n = sample_count(w, sqrt(dx * dx + dy * dy + dz * dz), acc);
     if (!(n >= 1.0 && n < 1e9)) {  // degenerate ray (pixel at the source)
         *dst = 0.0f;
         return;
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     // the clipped-away samples are exactly zero.
     const double last = (w.cap < n) ? ceil(w.cap) : n;
     double lo = -INFINITY, hi = INFINITY;
-    const double sxyz[3] = {w.src[0], w.src[1], w.src[2]};
+    const double sxyz[3] = {sp[0], sp[1], sp[2]};
     const double st[3] = {stx, sty, stz};
     const int nn[3] = {nx, ny, nz};
     bool empty = false;
@@ -219,8 +224,8 @@ __global__ __launch_bounds__(256) void ax_interp_kernel(const float* __restrict_
     }
     // Positions relative to the first sample keep the FP32 loop accurate;
     // +1 moves them to padded coordinates (index -1 -> 0).
-    const float qx = (float)(w.src[0] + i0 * stx + 1.0), qy = (float)(w.src[1] + i0 * sty + 1.0),
-                qz = (float)(w.src[2] + i0 * stz + 1.0);
+    const float qx = (float)(sp[0] + i0 * stx + 1.0), qy = (float)(sp[1] + i0 * sty + 1.0),
+                qz = (float)(sp[2] + i0 * stz + 1.0);
     const float fx = (float)stx, fy = (float)sty, fz = (float)stz;
     const int count = (int)(i1 - i0) + 1;
 

@@ -297,23 +297,26 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     double len = 0.0;
     if (active) {
         const double pv = (double)(nV - 1 - v);  // detector row flip, Siddon_projection.cu:141
-        double rx = w.org[0] + u * w.du[0] + pv * w.dv[0] - w.src[0];
-        double ry = w.org[1] + u * w.du[1] + pv * w.dv[1] - w.src[1];
-        double rz = w.org[2] + u * w.du[2] + pv * w.dv[2] - w.src[2];
+        double sp[3];
+        ray_source(w, u, pv, sp);
+        double rx = ray_dir(w, u, pv, 0);
+        double ry = ray_dir(w, u, pv, 1);
+        double rz = ray_dir(w, u, pv, 2);
         if (kMode == 0) {
+            if (w.pad[1] != 0.0) parallel_siddon_height(sp, rx, ry, rz, nx, ny);
             // components below 1e-3 voxel are treated as exactly zero (:155-159)
             if (fabs(rx) < 1e-3) rx = 0.0;
             if (fabs(ry) < 1e-3) ry = 0.0;
             if (fabs(rz) < 1e-3) rz = 0.0;
             len = sqrt(rx * rx * dvx * dvx + ry * ry * dvy * dvy + rz * rz * dvz * dvz);  // mm (:241, :267)
             double am = -INFINITY, aM = INFINITY;
-            bool hit = slab(w.src[0], rx, nx, am, aM);
-            hit = hit && slab(w.src[1], ry, ny, am, aM);
-            hit = hit && slab(w.src[2], rz, nz, am, aM);
+            bool hit = slab(sp[0], rx, nx, am, aM);
+            hit = hit && slab(sp[1], ry, ny, am, aM);
+            hit = hit && slab(sp[2], rz, nz, am, aM);
             if (hit && am < aM) {  // a ray that misses the volume projects to 0 (:179-180)
-                const Axis ax = axis_setup(w.src[0], rx, am);
-                const Axis ay = axis_setup(w.src[1], ry, am);
-                const Axis az = axis_setup(w.src[2], rz, am);
+                const Axis ax = axis_setup(sp[0], rx, am);
+                const Axis ay = axis_setup(sp[1], ry, am);
+                const Axis az = axis_setup(sp[2], rz, am);
                 r.tx = r.t0x = ax.t0;
                 r.ty = r.t0y = ay.t0;
                 r.tz = r.t0z = az.t0;
@@ -333,7 +336,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
             }
         } else {
             // samples per ray and step (ray_interpolated_projection.cu:164-169)
-            const double n = ceil(sqrt(rx * rx + ry * ry + rz * rz) / acc);
+            const double n = sample_count(w, sqrt(rx * rx + ry * ry + rz * rz), acc);
             if (n >= 1.0 && n < 1e9) {  // else: degenerate ray (pixel at the source), 0
                 const double st[3] = {rx / n, ry / n, rz / n};
                 // sample range [first, last] of the reference (:182-193), clipped to
@@ -346,9 +349,9 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     if (st[a] == 0.0) {
-                        if (!(w.src[a] > -1.0 && w.src[a] < (double)nn[a])) empty = true;
+                        if (!(sp[a] > -1.0 && sp[a] < (double)nn[a])) empty = true;
                     } else {
-                        const double a0 = (-1.0 - w.src[a]) / st[a], a1 = ((double)nn[a] - w.src[a]) / st[a];
+                        const double a0 = (-1.0 - sp[a]) / st[a], a1 = ((double)nn[a] - sp[a]) / st[a];
                         lo = fmax(lo, fmin(a0, a1));
                         hi = fmin(hi, fmax(a0, a1));
                     }
@@ -358,9 +361,9 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 len = sqrt(st[0] * dvx * st[0] * dvx + st[1] * dvy * st[1] * dvy + st[2] * dvz * st[2] * dvz);
                 if (!empty && i0 <= i1) {
                     // positions relative to the first sample keep the FP32 loop accurate
-                    rs.qP = (float)(w.src[kP] + i0 * st[kP]);
-                    rs.qA = (float)(w.src[1 - kP] + i0 * st[1 - kP]);
-                    rs.qz = (float)(w.src[2] + i0 * st[2]);
+                    rs.qP = (float)(sp[kP] + i0 * st[kP]);
+                    rs.qA = (float)(sp[1 - kP] + i0 * st[1 - kP]);
+                    rs.qz = (float)(sp[2] + i0 * st[2]);
                     rs.fP = (float)st[kP];
                     rs.fA = (float)st[1 - kP];
                     rs.fz = (float)st[2];
@@ -381,14 +384,16 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
         const int cu = min(tile_u * kBeamTile + ((c & 1) ? kBeamTile - 1 : 0), nU - 1);
         const int cvv = min(tile_v * kBeamTile + ((c & 2) ? kBeamTile - 1 : 0), nV - 1);
         const double pv = (double)(nV - 1 - cvv);
-        const double dP = cw.org[kP] + cu * cw.du[kP] + pv * cw.dv[kP] - cw.src[kP];
-        const double dA = cw.org[1 - kP] + cu * cw.du[1 - kP] + pv * cw.dv[1 - kP] - cw.src[1 - kP];
-        const double dZ = cw.org[2] + cu * cw.du[2] + pv * cw.dv[2] - cw.src[2];
+        double csrc[3];
+        ray_source(cw, cu, pv, csrc);
+        const double dP = cw.org[kP] + cu * cw.du[kP] + pv * cw.dv[kP] - csrc[kP];
+        const double dA = cw.org[1 - kP] + cu * cw.du[1 - kP] + pv * cw.dv[1 - kP] - csrc[1 - kP];
+        const double dZ = cw.org[2] + cu * cw.du[2] + pv * cw.dv[2] - csrc[2];
         const double ka = dA / dP, kzz = dZ / dP;
         kA = (float)ka;
-        cA = (float)(cw.src[1 - kP] - cw.src[kP] * ka);
+        cA = (float)(csrc[1 - kP] - csrc[kP] * ka);
         kz = (float)kzz;
-        cz = (float)(cw.src[2] - cw.src[kP] * kzz);
+        cz = (float)(csrc[2] - csrc[kP] * kzz);
     }
 
     // ---- plan: box origins of every slab (in parallel over the wavefronts)

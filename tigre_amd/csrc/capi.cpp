@@ -44,16 +44,13 @@ size_t volume_elems(const TigreGeometry* g) {
     return (size_t)g->nVoxelX * (size_t)g->nVoxelY * (size_t)g->nVoxelZ;
 }
 
-}  // namespace
-
-extern "C" {
-
-int tigre_ax(const float* img, const TigreGeometry* geo, float** result, const float* angles, int nangles,
-             int projection_type, int layout, const int* gpuids, int ngpus) {
+int ax_host(const float* img, const TigreGeometry* geo, float** result, const float* angles, int nangles,
+            int projection_type, int layout, const int* gpuids, int ngpus, bool parallel) {
     set_error("");
     TIGRE_REQUIRE(img != nullptr && result != nullptr, "null image or result pointer");
     HostGeometry g;
     TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    g.parallel = parallel;
     HostCall call;
     TIGRE_CHECK(call.begin(gpuids, ngpus));
     const size_t nvox = volume_elems(geo);
@@ -69,12 +66,13 @@ int tigre_ax(const float* img, const TigreGeometry* geo, float** result, const f
     return call.finish();
 }
 
-int tigre_atb(const float* projections, const TigreGeometry* geo, float* result, const float* angles, int nangles,
-              int backprojection_type, int layout, const int* gpuids, int ngpus) {
+int atb_host(const float* projections, const TigreGeometry* geo, float* result, const float* angles, int nangles,
+             int backprojection_type, int layout, const int* gpuids, int ngpus, bool parallel) {
     set_error("");
     TIGRE_REQUIRE(projections != nullptr && result != nullptr, "null projection or result pointer");
     HostGeometry g;
     TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    g.parallel = parallel;
     HostCall call;
     TIGRE_CHECK(call.begin(gpuids, ngpus));
     const size_t nvox = volume_elems(geo);
@@ -86,6 +84,30 @@ int tigre_atb(const float* projections, const TigreGeometry* geo, float* result,
     TIGRE_CHECK(atb_device(d_proj, g, d_vol, backprojection_type, layout, TIGRE_DTYPE_F32, call.stream));
     TIGRE_HIP_TRY(hipMemcpyAsync(result, d_vol, nvox * sizeof(float), hipMemcpyDeviceToHost, call.stream));
     return call.finish();
+}
+
+}  // namespace
+
+extern "C" {
+
+int tigre_ax(const float* img, const TigreGeometry* geo, float** result, const float* angles, int nangles,
+             int projection_type, int layout, const int* gpuids, int ngpus) {
+    return ax_host(img, geo, result, angles, nangles, projection_type, layout, gpuids, ngpus, false);
+}
+
+int tigre_ax_parallel(const float* img, const TigreGeometry* geo, float** result, const float* angles, int nangles,
+                      int projection_type, int layout, const int* gpuids, int ngpus) {
+    return ax_host(img, geo, result, angles, nangles, projection_type, layout, gpuids, ngpus, true);
+}
+
+int tigre_atb(const float* projections, const TigreGeometry* geo, float* result, const float* angles, int nangles,
+              int backprojection_type, int layout, const int* gpuids, int ngpus) {
+    return atb_host(projections, geo, result, angles, nangles, backprojection_type, layout, gpuids, ngpus, false);
+}
+
+int tigre_atb_parallel(const float* projections, const TigreGeometry* geo, float* result, const float* angles,
+                       int nangles, int layout, const int* gpuids, int ngpus) {
+    return atb_host(projections, geo, result, angles, nangles, TIGRE_BACKPROJ_FDK, layout, gpuids, ngpus, true);
 }
 
 int tigre_pocs_tv(const float* img, float* dst, float alpha, const long* image_size, int maxIter, const int* gpuids,
@@ -199,6 +221,26 @@ int tigre_atb_device_typed(const void* d_proj, const TigreGeometry* geo, void* d
     HostGeometry g;
     TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
     return atb_device(d_proj, g, d_vol, backprojection_type, layout, dtype, (hipStream_t)stream);
+}
+
+int tigre_ax_parallel_device(const void* d_img, const TigreGeometry* geo, void* d_proj, const float* angles,
+                             int nangles, int projection_type, int layout, int dtype, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_img != nullptr && d_proj != nullptr, "null device pointer");
+    HostGeometry g;
+    TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    g.parallel = true;
+    return ax_device(d_img, g, d_proj, projection_type, layout, dtype, (hipStream_t)stream);
+}
+
+int tigre_atb_parallel_device(const void* d_proj, const TigreGeometry* geo, void* d_vol, const float* angles,
+                              int nangles, int layout, int dtype, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_proj != nullptr && d_vol != nullptr, "null device pointer");
+    HostGeometry g;
+    TIGRE_CHECK(import_geometry(geo, angles, nangles, &g));
+    g.parallel = true;
+    return atb_device(d_proj, g, d_vol, TIGRE_BACKPROJ_FDK, layout, dtype, (hipStream_t)stream);
 }
 
 int tigre_pocs_tv_device(const float* d_img, float* d_dst, float alpha, const long* image_size, int maxIter,

@@ -62,9 +62,34 @@ struct FwdView {
     double dv[3];      // step per row counted from the top (pv = nV - 1 - v)
     double first;      // interpolated: first sample index floor(crop / accuracy)
     double cap;        // interpolated: (2 DSO / dmin + crop) / accuracy
-    double pad[2];     // pad[0]: index of this view's projection in the output
+    double pad[2];     // pad[0]: index of this view's projection in the output;
+                       // pad[1]: 1 for a parallel beam (every pixel has its own source), else 0
 };
 static_assert(sizeof(FwdView) == 16 * sizeof(double), "FwdView layout");
+
+// Start of the ray of pixel (u, pv): the view's source (cone beam) or, for a
+// parallel beam, the source moved with the pixel (Siddon_projection_parallel.cu
+// :144-146), so all rays of the view are parallel.  For a cone beam the shift
+// is exactly zero.
+__host__ __device__ inline void ray_source(const FwdView& w, double u, double pv, double s[3]) {
+    const double k = w.pad[1];
+    for (int c = 0; c < 3; ++c) s[c] = w.src[c] + k * (u * w.du[c] + pv * w.dv[c]);
+}
+
+// Source-to-pixel vector of pixel (u, pv); for a parallel beam the pixel
+// shifts of source and pixel cancel, so it is taken as org - src exactly.
+__host__ __device__ inline double ray_dir(const FwdView& w, double u, double pv, int c) {
+    return w.pad[1] != 0.0 ? w.org[c] - w.src[c] : w.org[c] + u * w.du[c] + pv * w.dv[c] - w.src[c];
+}
+
+// Interpolated projector's sample count ceil(|d| / accuracy)
+// (ray_interpolated_projection.cu:164-166).  A parallel beam's |d| is DSD in
+// voxel units, often an exact multiple of the accuracy, where the count hangs
+// on the last bit of the geometry; values within 1e-9 of an integer count as
+// that integer.
+__host__ __device__ inline double sample_count(const FwdView& w, double len, double acc) {
+    return ceil(len / acc - (w.pad[1] != 0.0 ? 1e-9 : 0.0));
+}
 
 // Backprojectors.  x in mm, y and z in detector-pixel units (the frame of
 // the reference's computeDeltasCube).
@@ -106,6 +131,7 @@ struct HostGeometry {
     std::vector<double> offOrig;   // 3 per view (X, Y, Z)
     std::vector<double> offDetec;  // 2 per view (U, V)
     std::vector<double> rot;       // 3 per view (roll, pitch, yaw)
+    bool parallel = false;         // parallel-beam operators (the reference's *_parallel functions)
 };
 
 int import_geometry(const TigreGeometry* geo, const float* angles, int nangles, HostGeometry* out);

@@ -143,6 +143,27 @@ void forward_views(const HostGeometry& g, bool centre, std::vector<FwdView>* out
         V3 S = mul(E, V3{DSO, 0.0, 0.0});
         V3 u = mul(ED, V3{0.0, dU, 0.0});
         V3 v = mul(ED, V3{0.0, 0.0, -dV});
+        if (g.parallel) {
+            // Parallel beam: the source of pixel (0, 0) is that pixel's detector
+            // point moved to x = DSO before the offsets (no detector offset on
+            // the source), and every pixel carries its own source.  Siddon
+            // rotates by alpha alone (Siddon_projection_parallel.cu:388-470);
+            // the interpolated projector applies the full ZYZ rotation and the
+            // detector offset twice (ray_interpolated_projection_parallel.cu
+            // :313-390), both kept as the reference does.
+            V3 s0 = mul(D, V3{0.0, dU * (0.5 - nU / 2.0), dV * (nV / 2.0 - 0.5)});
+            s0.x += DSO;
+            const M3 R = centre ? E : euler_zyz(al, 0.0, 0.0);
+            if (centre) {
+                corner.y += g.offDetec[2 * i + 0];
+                corner.z += g.offDetec[2 * i + 1];
+            }
+            P = mul(R, corner);
+            S = mul(R, s0);
+            const M3 RD = mul(R, D);
+            u = mul(RD, V3{0.0, dU, 0.0});
+            v = mul(RD, V3{0.0, 0.0, -dV});
+        }
         FwdView& r = (*out)[i];
         const double corx = -g.COR[i] * std::sin(al), cory = g.COR[i] * std::cos(al);
         const double* off = &g.offOrig[3 * i];
@@ -183,9 +204,10 @@ void forward_views(const HostGeometry& g, bool centre, std::vector<FwdView>* out
         const double dmin = planar ? std::fmin(g.dVoxel[0], g.dVoxel[1])
                                    : std::fmin(std::fmin(g.dVoxel[0], g.dVoxel[1]), g.dVoxel[2]);
         r.first = std::floor(crop / g.accuracy);
-        r.cap = (2.0 * DSO / dmin + crop) / g.accuracy;
+        // parallel beam: the cap uses dVoxelX (ray_interpolated_projection_parallel.cu:175-176)
+        r.cap = (2.0 * DSO / (g.parallel ? g.dVoxel[0] : dmin) + crop) / g.accuracy;
         r.pad[0] = (double)i;
-        r.pad[1] = 0.0;
+        r.pad[1] = g.parallel ? 1.0 : 0.0;
     }
 }
 

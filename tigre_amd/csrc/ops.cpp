@@ -21,7 +21,11 @@ struct BeamClass {
 
 BeamClass beam_class(const HostGeometry& g, const FwdView& w) {
     const int nU = g.nDetec[0], nV = g.nDetec[1];
-    auto ray = [&](double u, double pv, int c) { return w.org[c] + u * w.du[c] + pv * w.dv[c] - w.src[c]; };
+    auto ray = [&](double u, double pv, int c) {
+        double s[3];
+        ray_source(w, u, pv, s);
+        return w.org[c] + u * w.du[c] + pv * w.dv[c] - s[c];
+    };
     const double cu = 0.5 * (nU - 1), cv = 0.5 * (nV - 1);
     BeamClass k;
     k.principal = std::fabs(ray(cu, cv, 0)) >= std::fabs(ray(cu, cv, 1)) ? 0 : 1;
@@ -49,7 +53,15 @@ void build_forward_records(const HostGeometry& g, bool interp, int views_per_gro
     std::vector<int> cls(views.size());
     std::vector<int> beam, rest;
     for (size_t i = 0; i < views.size(); ++i) {
-        const BeamClass k = beam_class(g, views[i]);
+        BeamClass k = beam_class(g, views[i]);
+        // parallel-beam Siddon walks a ray tilted in z at a fixed height, outside
+        // the slab boxes planned from the tilted beam: such views take the
+        // gather kernel (horizontal rays, the usual parallel geometry, do not)
+        if (g.parallel && !interp) {
+            double s[3];
+            ray_source(views[i], 0.0, 0.0, s);
+            if (std::fabs(views[i].org[2] - s[2]) >= 1e-3) k.ok = false;
+        }
         cls[i] = k.ok ? k.principal * 2 + (k.dir > 0 ? 1 : 0) : -1;
         (k.ok ? beam : rest).push_back((int)i);
     }

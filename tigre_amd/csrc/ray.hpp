@@ -61,4 +61,22 @@ __device__ __forceinline__ Axis axis_setup(double s, double r, double am) {
     return a;
 }
 
+// Parallel-beam Siddon (Siddon_projection_parallel.cu:173-227): entry and exit
+// come from the x and y planes only, and the z index stays at the midpoint of
+// the first segment, so a ray that moves in z (a detector offset tilts the
+// rays) is summed as the horizontal ray at that height, with the length of its
+// x-y part.  Applied to the ray start s and direction before the 3D set-up.
+__device__ inline void parallel_siddon_height(double s[3], double rx, double ry, double& rz, int nx, int ny) {
+    double am = -INFINITY, aM = INFINITY;
+    if (!slab(s[0], rx, nx, am, aM) || !slab(s[1], ry, ny, am, aM) || !(am < aM)) {
+        rz = 0.0;
+        return;
+    }
+    const Axis ax = axis_setup(s[0], rx, am), ay = axis_setup(s[1], ry, am);
+    double first = am + fmin((double)ax.t0, (double)ay.t0);
+    if (!(first <= aM)) first = aM;
+    s[2] += 0.5 * (am + first) * rz;
+    rz = 0.0;
+}
+
 }  // namespace tigre

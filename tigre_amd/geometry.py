@@ -170,13 +170,41 @@ class ConeGeometryDefault(Geometry):
         self.filter = None
 
 
+class ParallelGeo(Geometry):
+    """Parallel-beam geometry of geometry.py:274-300: 1 mm voxels, detector pixels
+    of 1 mm, nDetector = (nVoxel[0], max(nVoxel[1], nVoxel[2])), DSO = nVoxel[0],
+    DSD = 2 nVoxel[0]."""
+
+    def __init__(self, nVoxel):  # noqa: N803
+        if nVoxel is None:
+            raise ValueError("nVoxel needs to be given for initialisation of parallel beam")
+        Geometry.__init__(self)
+        self.mode = "parallel"
+        self.nVoxel = np.asarray(nVoxel)
+        self.dVoxel = np.array([1, 1, 1])
+        self.sVoxel = self.nVoxel
+        self.DSO = np.float32(self.nVoxel[0])
+        self.DSD = np.float32(self.nVoxel[0] * 2)
+        self.dDetector = np.array([1, 1])
+        self.nDetector = np.zeros((2,), dtype=np.int64)
+        self.nDetector[0] = self.nVoxel[0]
+        self.nDetector[1] = max(self.nVoxel[1], self.nVoxel[2])
+        self.sDetector = self.nDetector
+        self.accuracy = 0.5
+        self.offOrigin = np.array([0, 0, 0])
+        self.offDetector = np.array([0, 0])
+        self.rotDetector = np.array([0, 0, 0])
+
+
 def geometry(mode="cone", nVoxel=None, default=False, high_resolution=True):
     """Geometry factory with the signature of tigre.geometry (geometry.py:302-352)."""
     if mode == "cone":
         if default:
             return ConeGeometryDefault(high_resolution=high_resolution, nVoxel=nVoxel)
         return Geometry()
-    raise NotImplementedError("tigre_amd implements the cone-beam operators; mode=%r is not available" % (mode,))
+    if mode == "parallel":
+        return ParallelGeo(nVoxel)
+    raise NotImplementedError("tigre_amd implements cone and parallel beams; mode=%r is not available" % (mode,))
 
 
 def cone_geometry(n_voxel, s_voxel, n_detector, d_detector, DSO, DSD, accuracy=0.5):

@@ -61,9 +61,13 @@ def _backprojection_kind(backprojection_type):
     return _native.BACKPROJ_MATCHED
 
 
-def _check_mode(geo):
+def _is_parallel(geo):
+    """Parallel or cone beam; an unknown mode warns and means cone (_Ax.pyx:67-73)."""
+    if geo.mode == "parallel":
+        return True
     if geo.mode not in ("cone", None):
-        raise NotImplementedError("tigre_amd implements cone-beam operators only (geo.mode=%r)" % (geo.mode,))
+        print("Warning: Unknown mode, using default cone beam")
+    return False
 
 
 def _storage_code(t):
@@ -100,7 +104,7 @@ def Ax(img, geo, angles, projection_type="Siddon", **kwargs):
     if tuple(shape) != tuple(int(v) for v in np.asarray(geo.nVoxel)):
         raise ValueError("Input data should be of shape geo.nVoxel: " + str(geo.nVoxel) + " not:" + str(shape))
     g = prepared(geo, angles)
-    _check_mode(g)
+    parallel = _is_parallel(g)
     kind = _projection_kind(projection_type)
     n = g.angles.shape[0]
     cg = _native.CGeometry(g, n)
@@ -111,10 +115,10 @@ def Ax(img, geo, angles, projection_type="Siddon", **kwargs):
 
         src = img.contiguous()
         out = torch.empty((n, nv, nu), dtype=img.dtype, device=img.device)
+        fn = lib.tigre_ax_parallel_device if parallel else lib.tigre_ax_device_typed
         with torch.cuda.device(img.device):
-            st = lib.tigre_ax_device_typed(ctypes.c_void_p(src.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                           ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON,
-                                           code, _stream_handle())
+            st = fn(ctypes.c_void_p(src.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
+                    ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON, code, _stream_handle())
         _native.check(st, "TIGRE: Call to Ax failed")
         return out
     src = np.ascontiguousarray(img)
@@ -124,8 +128,9 @@ def Ax(img, geo, angles, projection_type="Siddon", **kwargs):
     ptrs = (ctypes.POINTER(ctypes.c_float) * n)(
         *[ctypes.cast(base + 4 * P * i, ctypes.POINTER(ctypes.c_float)) for i in range(n)])
     ids, nids = _gpu_ids(kwargs.get("gpuids"))
-    st = lib.tigre_ax(src.ctypes.data_as(ctypes.c_void_p), cg.ptr(), ptrs, ang.ctypes.data_as(ctypes.c_void_p), n,
-                      kind, _native.LAYOUT_PYTHON, ids, nids)
+    fn = lib.tigre_ax_parallel if parallel else lib.tigre_ax
+    st = fn(src.ctypes.data_as(ctypes.c_void_p), cg.ptr(), ptrs, ang.ctypes.data_as(ctypes.c_void_p), n, kind,
+            _native.LAYOUT_PYTHON, ids, nids)
     _native.check(st, "TIGRE: Call to Ax failed")
     return out
 
@@ -144,7 +149,7 @@ def Atb(projections, geo, angles, backprojection_type="FDK", **kwargs):
         if not np.isreal(projections).all():
             raise ValueError("Complex types not compatible for back projection.")
     g = prepared(geo, angles)
-    _check_mode(g)
+    parallel = _is_parallel(g)
     expect = (g.angles.shape[0], int(g.nDetector[0]), int(g.nDetector[1]))
     if tuple(projections.shape) != expect:
         raise ValueError("Expected data shape: " + str(expect) + " not compatible with: "
@@ -160,15 +165,25 @@ def Atb(projections, geo, angles, backprojection_type="FDK", **kwargs):
         src = projections.contiguous()
         out = torch.empty(vshape, dtype=projections.dtype, device=projections.device)
         with torch.cuda.device(projections.device):
-            st = lib.tigre_atb_device_typed(ctypes.c_void_p(src.data_ptr()), cg.ptr(),
-                                            ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p), n,
-                                            kind, _native.LAYOUT_PYTHON, code, _stream_handle())
+            if parallel:
+                st = lib.tigre_atb_parallel_device(ctypes.c_void_p(src.data_ptr()), cg.ptr(),
+                                                   ctypes.c_void_p(out.data_ptr()),
+                                                   ang.ctypes.data_as(ctypes.c_void_p), n, _native.LAYOUT_PYTHON,
+                                                   code, _stream_handle())
+            else:
+                st = lib.tigre_atb_device_typed(ctypes.c_void_p(src.data_ptr()), cg.ptr(),
+                                                ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p),
+                                                n, kind, _native.LAYOUT_PYTHON, code, _stream_handle())
         _native.check(st, "Atb:")
         return out
     src = np.ascontiguousarray(projections)
     out = np.empty(vshape, dtype=np.float32)
     ids, nids = _gpu_ids(kwargs.get("gpuids"))
-    st = lib.tigre_atb(src.ctypes.data_as(ctypes.c_void_p), cg.ptr(), out.ctypes.data_as(ctypes.c_void_p),
-                       ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON, ids, nids)
+    if parallel:
+        st = lib.tigre_atb_parallel(src.ctypes.data_as(ctypes.c_void_p), cg.ptr(), out.ctypes.data_as(ctypes.c_void_p),
+                                    ang.ctypes.data_as(ctypes.c_void_p), n, _native.LAYOUT_PYTHON, ids, nids)
+    else:
+        st = lib.tigre_atb(src.ctypes.data_as(ctypes.c_void_p), cg.ptr(), out.ctypes.data_as(ctypes.c_void_p),
+                           ang.ctypes.data_as(ctypes.c_void_p), n, kind, _native.LAYOUT_PYTHON, ids, nids)
     _native.check(st, "Atb:")
     return out
