@@ -70,5 +70,10 @@ def test_default_geometries():
     g = geometry(mode="cone", default=True, high_resolution=False)
     assert list(g.nVoxel) == [64, 64, 64]
     assert isinstance(geometry(mode="cone"), Geometry)
-    with pytest.raises(NotImplementedError):
+    # parallel beam needs nVoxel (geometry.py:274-278) and follows ParallelGeo's defaults
+    with pytest.raises(ValueError):
         geometry(mode="parallel")
+    p = geometry(mode="parallel", nVoxel=np.array([8, 10, 12]))
+    assert p.mode == "parallel" and list(p.nDetector) == [8, 12] and float(p.DSD) == 16.0
+    with pytest.raises(NotImplementedError):
+        geometry(mode="fan")
