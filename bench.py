@@ -59,10 +59,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size once (counter calibration)")
-    ap.add_argument("--workload", choices=["pair", "batched", "learned", "precision"], default="pair",
+    ap.add_argument("--workload", choices=["pair", "small", "batched", "learned", "precision"], default="pair",
                     help="pair: Ax+Atb at 512^3 (default); batched: 64 uneven OS-SART cases in one graph; "
                          "learned: ASD-POCS with the learned proximal step vs plain TV at 256^3; "
-                         "precision: BF16 / FP16 / FP32 storage at 1024^3 with CGLS")
+                         "precision: BF16 / FP16 / FP32 storage at 1024^3 with CGLS; "
+                         "small: every operator at 128^3 beside the full FP64 oracle")
     ap.add_argument("--train-steps", type=int, default=1000, help="learned: Adam steps")
     ap.add_argument("--cases", type=int, default=64, help="batched: number of cases")
     ap.add_argument("--iters", type=int, default=10, help="batched: OS-SART iterations per case")
@@ -80,6 +81,9 @@ def main():
         return
     if args.workload == "precision":
         bench_precision(args)
+        return
+    if args.workload == "small":
+        bench_small(args)
         return
 
     import numpy as np
@@ -532,6 +536,86 @@ def bench_precision(args):
                    "oracle_seconds": round(t_oracle, 1)},
         "by_storage": res,
         "gupdates_per_s_ax_f32": round(V * n / (res["f32"]["ax_ms"] * 1e-3) / 1e9, 1),
+    }
+    print(json.dumps(result), flush=True)
+
+
+def bench_small(args):
+    """Every operator at 128^3 (1 mm voxels), 128 views over 2*pi, 128 x 128
+    detector at 1.6 mm, DSO 1000 / DSD 1536 mm, seeded 10-ellipsoid phantom:
+    Ax 'Siddon' and 'interpolated', Atb 'FDK' and 'matched', and FDK with the
+    ram-lak filter, each timed on the GPU (device-resident, median of 10 after
+    3 warm-ups) and in full by the FP64 OpenMP oracle on the host cores, with
+    the relative L2 error between the two."""
+    import numpy as np
+    import torch
+
+    import oracle
+    import oracle.algorithms as oalg
+    import tigre_amd
+    from tigre_amd import algorithms as algs
+    from tigre_amd.geometry import cone_geometry, prepared
+    from tigre_amd.phantom import ellipsoid_phantom
+
+    geo = cone_geometry((128, 128, 128), (128.0,) * 3, (128, 128), (1.6, 1.6), 1000.0, 1536.0)
+    angles = np.linspace(0, 2 * np.pi, 128, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 10, seed=20261202 + 1)
+    gp = prepared(geo, angles)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    dimg = torch.as_tensor(img, device="cuda")
+    proj_ref = oracle.ax(img, gp, "Siddon").astype(np.float32)
+    dproj = torch.as_tensor(proj_ref, device="cuda")
+
+    def gpu_ms(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(10):
+            s_, e_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s_.record()
+            fn()
+            e_.record()
+            e_.synchronize()
+            ts.append(s_.elapsed_time(e_))
+        return float(np.median(ts))
+
+    def cpu_s(fn):
+        t = time.perf_counter()
+        out = fn()
+        return out, time.perf_counter() - t
+
+    ops = {
+        "ax_siddon": (lambda: tigre_amd.Ax(dimg, geo, angles, "Siddon"), lambda: oracle.ax(img, gp, "Siddon")),
+        "ax_interpolated": (lambda: tigre_amd.Ax(dimg, geo, angles, "interpolated"),
+                            lambda: oracle.ax(img, gp, "interpolated")),
+        "atb_fdk": (lambda: tigre_amd.Atb(dproj, geo, angles, "FDK"), lambda: oracle.atb(proj_ref, gp, "FDK")),
+        "atb_matched": (lambda: tigre_amd.Atb(dproj, geo, angles, "matched"),
+                        lambda: oracle.atb(proj_ref, gp, "matched")),
+        "fdk_ram_lak": (lambda: algs.fdk(dproj, geo, angles, filter="ram_lak", return_tensor=True),
+                        lambda: oalg.fdk(proj_ref, geo, angles, "ram_lak")[0]),
+    }
+    res = {}
+    for name, (g_fn, c_fn) in ops.items():
+        ms = gpu_ms(g_fn)
+        got = g_fn().float().cpu().numpy()
+        ref, t_cpu = cpu_s(c_fn)
+        res[name] = {"gpu_ms": round(ms, 3), "cpu_s": round(t_cpu, 3), "rel_l2": oracle.rel_l2(got, ref),
+                     "speedup": round(t_cpu * 1e3 / ms, 1)}
+        log("%s: %s" % (name, res[name]))
+    V, n = 128.0 ** 3, 128
+    pair_ms = res["ax_siddon"]["gpu_ms"] + res["atb_fdk"]["gpu_ms"]
+    result = {
+        "metric": "Ax+Atb Gupdates/s (Siddon + FDK, cone beam), 128^3 / 128 views",
+        "value": round(2 * V * n / (pair_ms * 1e-3) / 1e9, 2), "unit": "Gupdates/s", "n_gpus": 1, "steps": 10,
+        "warmup": 3, "ms_per_step": round(pair_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded 10-ellipsoid phantom)",
+        "config": {"workload": "cone beam 128^3 at 1 mm, 128 views, 128x128 detector at 1.6 mm, DSO 1000 / DSD 1536"},
+        "operators": res,
+        "cpu_baseline": {"value": round(2 * V * n / (res["ax_siddon"]["cpu_s"] + res["atb_fdk"]["cpu_s"]) / 1e9, 4),
+                         "unit": "Gupdates/s", "cores": oracle.get_threads(), "kind": "port",
+                         "sample": "FP64 OpenMP oracle, every operator in full"},
     }
     print(json.dumps(result), flush=True)
 
