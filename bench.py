@@ -324,6 +324,16 @@ def bench_batched(args):
     same = all(torch.equal(a, b) for a, b in zip(bo.results(), eager))
     log("graph replay: median %.3f s over %d (capture %.2f s), bit-identical to sequential: %s" % (
         t_graph, len(reps), t_capture, same))
+    # 4) one graph per stream, replayed on concurrent streams
+    bo.graph = None
+    bo.capture_lanes()
+    bo.reset()
+    bo.replay_lanes()
+    torch.cuda.synchronize()
+    lane_reps = [clock(bo.replay_lanes) for _ in range(max(1, args.steps))]
+    t_lanes = float(np.median(lane_reps))
+    same_lanes = all(torch.equal(a, b) for a, b in zip(bo.results(), eager))
+    log("per-stream graphs: median %.3f s, bit-identical: %s" % (t_lanes, same_lanes))
     result = {
         "metric": "batched OS-SART wall-clock, %d cases x %d iterations" % (len(cases), args.iters),
         "value": round(t_graph, 4), "unit": "s", "n_gpus": 1, "steps": len(reps), "warmup": 1,
@@ -333,6 +343,7 @@ def bench_batched(args):
                                "10 iterations, 20-view subsets, one graph replay",
                    "cases": [list(d) for d in desc], "streams": args.streams},
         "gupdates_per_s": round(updates / t_graph / 1e9, 2),
+        "per_stream_graphs_s": round(t_lanes, 4), "per_stream_graphs_bit_identical": same_lanes,
         "sequential_api_s": round(t_api, 4), "sequential_plans_s": round(t_eager, 4),
         "speedup_vs_api": round(t_api / t_graph, 2), "speedup_vs_sequential_plans": round(t_eager / t_graph, 2),
         "bit_identical_to_sequential": same, "setup_s": round(t_setup, 2), "capture_s": round(t_capture, 2),

@@ -157,6 +157,14 @@ def test_batched_matches_ossart(cuda, batch):
         assert float(scratch) == float(scratch)
         for r, s in zip(got, seq):
             assert torch.equal(r, s), "replay %d differs" % rep
+    bo.capture_lanes()
+    for rep in range(2):
+        bo.reset()
+        torch.cuda.synchronize()
+        got = bo.replay_lanes()
+        torch.cuda.synchronize()
+        for r, s in zip(got, seq):
+            assert torch.equal(r, s), "per-stream replay %d differs" % rep
     ora = oalg.os_sart(batch[0][0], batch[0][1], batch[0][2], 2, blocksize=8)
     err = oracle.rel_l2(got[0].cpu().numpy(), ora)
     print("batched OS-SART case 0 rel-L2 vs oracle %.3e" % err)

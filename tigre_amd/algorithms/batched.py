@@ -10,10 +10,12 @@ subset costs a handful of kernel launches and no host work.
 ``run_sequential`` launches case after case from the host (the same kernels,
 one Python call per launch); ``capture`` records every case's iterations into
 one CUDA/HIP graph with the cases spread over a few streams (independent
-branches of the graph, so small cases overlap), and ``replay`` runs it with
-one host call.  Both give the same bits as :func:`tigre_amd.algorithms.ossart`
-when the plans use the default parameters, since the kernels and their
-rounding steps are the same.
+branches of the graph), and ``replay`` runs it with one host call;
+``capture_lanes`` / ``replay_lanes`` record one graph per stream instead and
+replay them on concurrent streams, which lets small cases overlap (the
+branches of one graph execute one after another on this runtime).  All give
+the same bits as :func:`tigre_amd.algorithms.ossart` when the plans use the
+default parameters, since the kernels and their rounding steps are the same.
 """
 from __future__ import annotations
 
@@ -133,8 +135,45 @@ class BatchedOSSART:
         self.graph.replay()
         return self.results()
 
+    def capture_lanes(self):
+        """One graph per stream (its share of the cases), so that replays on
+        different streams can run at the same time (the branches of a single
+        graph run one after another on this runtime)."""
+        torch = _torch()
+        self.run_sequential()
+        torch.cuda.synchronize()
+        self.lanes = []
+        for idx in self._assign():
+            if not idx:
+                continue
+            s = torch.cuda.Stream()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s):
+                for i in idx:
+                    self.cases[i].res.zero_()
+                    for it in range(self.niter):
+                        self.cases[i].iteration(it)
+            self.lanes.append((s, g))
+        torch.cuda.synchronize()
+        return self.lanes
+
+    def replay_lanes(self):
+        """Replay every lane's graph on its own stream, joined to the current stream."""
+        torch = _torch()
+        if not getattr(self, "lanes", None):
+            self.capture_lanes()
+        main = torch.cuda.current_stream()
+        for s, g in self.lanes:
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                g.replay()
+        for s, _ in self.lanes:
+            main.wait_stream(s)
+        return self.results()
+
     def close(self):
         self.graph = None
+        self.lanes = []
         for c in self.cases:
             for p in c.plans:
                 p.close()
