@@ -236,30 +236,76 @@ __device__ __forceinline__ float box_sample(const float* __restrict__ src, int S
 
 // Samples of one slab (cells lo .. hi along P, box origin (a0, z0), LDS or
 // global `src`); stops at the first sample whose cell lies beyond the slab.
-// Returns true when the ray has no samples left.
-template <int kMode>
+// Returns true when the ray has no samples left.  The samples of the slab are
+// counted once (P advances monotonically with the sample index), so the loop
+// tests one bound per sample.  kLds: offsets inside a box are below the slab
+// capacity (far below 2^24) and are formed from box-relative cell indices in
+// float arithmetic, exactly; the staging copy in global memory needs the
+// integer form.
+template <int kMode, bool kLds>
 __device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ src, int SP, int SZ, int P0, int Z0,
-                                   int A0, int lo, int hi, int dir, float hP, float hA, float hz, int guard) {
+                                   int A0, int lo, int hi, int dir, float hP, float hA, float hz) {
     int m = r.m;
     const int count = r.count;
     const float qP = r.qP, qA = r.qA, qz = r.qz, fP = r.fP, fA = r.fA, fz = r.fz;
     float sum = r.sum;
-    bool done = false;
-    for (; guard > 0; --guard) {
-        if (m >= count) {
-            done = true;
-            break;
-        }
+    // positions stay in [-1, n] (rounding at the clipped ends), so the taps
+    // stay in the zero-bordered range
+    auto posP = [&](int k) { return __builtin_amdgcn_fmed3f(fmaf((float)k, fP, qP), -1.0f, hP); };
+    auto inside = [&](int k) {
+        const int pc = (int)floorf(kMode == 2 ? posP(k) + 0.5f : posP(k));
+        return dir > 0 ? pc <= hi : pc >= lo;
+    };
+    // first sample beyond the slab: an estimate, then corrected with the
+    // per-sample test itself
+    int me = count;
+    if (fP != 0.0f) {
+        const float lim = (dir > 0 ? (float)hi + 1.0f : (float)lo) - (kMode == 2 ? 0.5f : 0.0f);
+        const float est = ceilf((lim - qP) / fP);
+        me = est < (float)m ? m : (est > (float)count ? count : (int)est);
+        while (me > m && !inside(me - 1)) --me;
+        while (me < count && inside(me)) ++me;
+    }
+    const bool done = me >= count;
+    // trilinear: the 8 taps of the last cell stay in registers; consecutive
+    // samples (half a voxel apart at the default accuracy) mostly share a
+    // cell, so only the lanes that moved to a new cell read LDS again
+    const float fP0 = (float)P0, fZ0 = (float)Z0, fA0 = (float)A0, fSP = (float)SP, fSZ = (float)SZ;
+    unsigned cached = 0xffffffffu;
+    float c000 = 0.f, c001 = 0.f, c010 = 0.f, c011 = 0.f, c100 = 0.f, c101 = 0.f, c110 = 0.f, c111 = 0.f;
+    for (; m < me; ++m) {
         const float fm = (float)m;
-        // positions stay in [-1, n] (rounding at the clipped ends), so the
-        // taps stay in the zero-bordered range
-        const float P = __builtin_amdgcn_fmed3f(fmaf(fm, fP, qP), -1.0f, hP);
-        const int pc = (int)floorf(kMode == 2 ? P + 0.5f : P);
-        if (dir > 0 ? pc > hi : pc < lo) break;
+        const float P = posP(m);
         const float A = __builtin_amdgcn_fmed3f(fmaf(fm, fA, qA), -1.0f, hA);
         const float Z = __builtin_amdgcn_fmed3f(fmaf(fm, fz, qz), -1.0f, hz);
-        sum += box_sample<kMode>(src, SP, SZ, P, A, Z, P0, Z0, A0);
-        ++m;
+        if (kMode == 2) {
+            sum += box_sample<kMode>(src, SP, SZ, P, A, Z, P0, Z0, A0);
+        } else {
+            const float flP = floorf(P), flA = floorf(A), flZ = floorf(Z);
+            const float wP = P - flP, wA = A - flA, wZ = Z - flZ;
+            const unsigned o =
+                kLds ? (unsigned)fmaf(flP - fP0, fSP, fmaf(flZ - fZ0, fSZ, flA - fA0))
+                     : __umul24((unsigned)((int)flP - P0), (unsigned)SP) +
+                           __umul24((unsigned)((int)flZ - Z0), (unsigned)SZ) + (unsigned)((int)flA - A0);
+            if (o != cached) {
+                c000 = src[o];
+                c001 = src[o + 1];
+                c010 = src[o + SZ];
+                c011 = src[o + SZ + 1];
+                c100 = src[o + SP];
+                c101 = src[o + SP + 1];
+                c110 = src[o + SP + SZ];
+                c111 = src[o + SP + SZ + 1];
+                cached = o;
+            }
+            // the arithmetic of box_sample, tap for tap
+            const float a00 = fmaf(wA, c001 - c000, c000);
+            const float a10 = fmaf(wA, c011 - c010, c010);
+            const float a01 = fmaf(wA, c101 - c100, c100);
+            const float a11 = fmaf(wA, c111 - c110, c110);
+            const float b0 = fmaf(wZ, a10 - a00, a00), b1 = fmaf(wZ, a11 - a01, a01);
+            sum += fmaf(wP, b1 - b0, b0);
+        }
     }
     r.m = m;
     r.sum = sum;
@@ -442,8 +488,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
             }
         } else if (state == 2) {
             // box origin (-1, -1, -4): the bordered staging copy itself
-            sample_slab<kMode>(rs, stage, sg.NZ * sg.NA, sg.NA, -1, -1, -4, -1, last_cell, dir, (float)nP, (float)nA,
-                               (float)nz, 1 << 30);
+            sample_slab<kMode, false>(rs, stage, sg.NZ * sg.NA, sg.NA, -1, -1, -4, -1, last_cell, dir, (float)nP,
+                                      (float)nA, (float)nz);
         }
     } else {
         const int SP = nzb * na;
@@ -508,8 +554,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                     if (walk<kP, true>(r, sh.buf, slot, (planes + na + nzb + 5) / 2)) state = 0;
                 }
             } else if (state == 2) {
-                if (sample_slab<kMode>(rs, cur, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP, (float)nA, (float)nz,
-                                       1 << 30))
+                if (sample_slab<kMode, true>(rs, cur, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP, (float)nA,
+                                             (float)nz))
                     state = 0;
             }
             plo = lo;
