@@ -52,6 +52,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
     const long long QU = quad_det(nU);
     const long long QP = QU * quad_det(nV);
     const float4* __restrict__ quads = reinterpret_cast<const float4*>(proj);
+    const unsigned rowb = (unsigned)QU * 16u;  // bytes per quad row (< 2^24, checked by the launcher)
     const float hu = (float)nU, hv = (float)nV;
     const float fx = (float)ix, fy = (float)iy;
 
@@ -62,6 +63,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
     for (int p = 0; p < nviews; ++p) {
         const BwdView& w = views[p];
         const float4* pj = quads + (long long)p * QP;
+        const char* pjb = reinterpret_cast<const char*>(pj);
         // source -> voxel (ix, iy, z0) vector, frame of computeDeltasCube
         const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
         const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
@@ -87,7 +89,9 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             const float pu = clampf(pu_raw, hu);
             const float flu = floorf(pu);
             const float wu = pu - flu;
-            const float4* col = pj + ((int)flu + 1);
+            // 32-bit byte offsets from the view's (uniform) base: a 24-bit
+            // multiply-add per sample instead of 64-bit index arithmetic
+            const unsigned colb = (unsigned)((int)flu + 1) * 16u;
             const float dpv = w.dZ[2] * t;
             const float pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
             float Day2 = 0.0f;
@@ -101,7 +105,8 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                 const float pv = clampf(pv_raw, hv);
                 const float flv = floorf(pv);
                 const float wv = pv - flv;
-                const float4 q = col[(long long)((int)flv + 1) * QU];
+                const float4 q = *reinterpret_cast<const float4*>(
+                    pjb + (__umul24((unsigned)((int)flv + 1), rowb) + colb));
                 const float a0 = fmaf(wu, q.y - q.x, q.x);
                 const float a1 = fmaf(wu, q.w - q.z, q.z);
                 const float s = fmaf(wv, a1 - a0, a0);
@@ -126,7 +131,8 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                 const float pv = clampf(pv_raw, hv);
                 const float flu = floorf(pu), flv = floorf(pv);
                 const float wu = pu - flu, wv = pv - flv;
-                const float4 q = pj[(long long)((int)flv + 1) * QU + ((int)flu + 1)];
+                const float4 q = *reinterpret_cast<const float4*>(
+                    pjb + (__umul24((unsigned)((int)flv + 1), rowb) + (unsigned)((int)flu + 1) * 16u));
                 const float a0 = fmaf(wu, q.y - q.x, q.x);
                 const float a1 = fmaf(wu, q.w - q.z, q.z);
                 const float s = fmaf(wv, a1 - a0, a0);
@@ -211,6 +217,10 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
     // the parallel-beam backprojector serves both 'FDK' and 'matched' (_Atb.pyx:56-76)
     const int kind = g.parallel ? 2 : (matched ? 1 : 0);
     TIGRE_REQUIRE(zper == 4 || zper == 8 || zper == 16, "slices per lane must be 4, 8 or 16");
+    // per-view byte offsets are 32-bit and the row stride enters a 24-bit multiply
+    TIGRE_REQUIRE(quad_det(g.nDetec[0]) * 16 < (1LL << 24) &&
+                      quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * 16 < (1LL << 32),
+                  "detector too large for one backprojection launch");
     const int nx = g.nVoxel[0], ny = g.nVoxel[1], nz = g.nVoxel[2];
     dim3 grid((nx + 15) / 16, (ny + 15) / 16, (nz + zper - 1) / zper);
     TIGRE_REQUIRE(grid.z <= 65535u && grid.y <= 65535u, "volume too large for one launch");
