@@ -8,12 +8,18 @@
 // hit a few cache lines.  Per-view parameters are uniform across the
 // workgroup and are fetched with scalar loads.
 //
-// Projections are read from a "quad" copy: element (u, v) holds the four
-// bilinear taps p(u,v), p(u+1,v), p(u,v+1), p(u+1,v+1) (zero outside the
-// detector) for u in [-1, nU], v in [-1, nV].  One 16-byte load per voxel
-// update replaces four scattered taps (the texture-address unit, not the ALU,
-// bounds this kernel), and the zero border replaces the border-addressed
-// linear texture of the reference without bounds checks.
+// Projections are read from a "quad" copy: element (u, v) holds
+// p(u,v), p(u+1,v) - p(u,v), p(u,v+1), p(u+1,v+1) - p(u,v+1) (zero outside
+// the detector) for tap origins u in [-1, nU], v in [-1, nV], stored at index
+// (u + 1, v + 1).  One 16-byte load per voxel update replaces four scattered
+// taps (the texture-address unit bounds this kernel alongside the ALU), the
+// zero border replaces the border-addressed linear texture of the reference
+// without bounds checks, and the two u differences of the bilinear lerp are
+// computed once per detector pixel instead of once per voxel update (the same
+// float subtractions, so the same bits).  Detector coordinates are carried in
+// that storage frame (pixel index + 1, the offset folded into the per-view
+// records), clamped with one med3 to [0, n + 1]: the tap origin is then the
+// truncating conversion and the weight the hardware fraction.
 //
 // kAxial: when no view tilts the rotation axis (dZ has no x/y component, the
 // case of every planar orbit without detector pitch/yaw) the column projects
@@ -53,17 +59,28 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
     const long long QP = QU * quad_det(nV);
     const float4* __restrict__ quads = reinterpret_cast<const float4*>(proj);
     const unsigned rowb = (unsigned)QU * 16u;  // bytes per quad row (< 2^24, checked by the launcher)
-    const float hu = (float)nU, hv = (float)nV;
+    const float hu = (float)(nU + 1), hv = (float)(nV + 1);  // storage-frame clamp bounds
     const float fx = (float)ix, fy = (float)iy;
 
     float acc[kZ];
 #pragma unroll
     for (int k = 0; k < kZ; ++k) acc[k] = 0.0f;
 
+    // one voxel update from a storage-frame detector position (u, v) already
+    // clamped to [0, n + 1]: bilinear sample of the quad copy.  Offsets are
+    // 32-bit bytes from the view's (uniform) base, formed with one 24-bit
+    // multiply-add.
+    auto sample = [&](const char* pjb, unsigned colb, float wu, float pv) {
+        const float wv = __builtin_amdgcn_fractf(pv);
+        const float4 q = *reinterpret_cast<const float4*>(pjb + (__umul24((unsigned)pv, rowb) + colb));
+        const float a0 = fmaf(wu, q.y, q.x);
+        const float a1 = fmaf(wu, q.w, q.z);
+        return fmaf(wv, a1 - a0, a0);
+    };
+
     for (int p = 0; p < nviews; ++p) {
         const BwdView& w = views[p];
-        const float4* pj = quads + (long long)p * QP;
-        const char* pjb = reinterpret_cast<const char*>(pj);
+        const char* pjb = reinterpret_cast<const char*>(quads + (long long)p * QP);
         // source -> voxel (ix, iy, z0) vector, frame of computeDeltasCube
         const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
         const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
@@ -86,12 +103,9 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         if (kAxial) {
             const float t = kParallel ? 1.0f : w.tnum * rcp(bx);
             const float pu_raw = fmaf(by, t, w.uoff);
-            const float pu = clampf(pu_raw, hu);
-            const float flu = floorf(pu);
-            const float wu = pu - flu;
-            // 32-bit byte offsets from the view's (uniform) base: a 24-bit
-            // multiply-add per sample instead of 64-bit index arithmetic
-            const unsigned colb = (unsigned)((int)flu + 1) * 16u;
+            const float pu = __builtin_amdgcn_fmed3f(pu_raw, 0.0f, hu);
+            const float wu = __builtin_amdgcn_fractf(pu);
+            const unsigned colb = (unsigned)pu * 16u;
             const float dpv = w.dZ[2] * t;
             const float pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
             float Day2 = 0.0f;
@@ -102,14 +116,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
 #pragma unroll
             for (int k = 0; k < kZ; ++k) {
                 const float pv_raw = fmaf((float)k, dpv, pv0);
-                const float pv = clampf(pv_raw, hv);
-                const float flv = floorf(pv);
-                const float wv = pv - flv;
-                const float4 q = *reinterpret_cast<const float4*>(
-                    pjb + (__umul24((unsigned)((int)flv + 1), rowb) + colb));
-                const float a0 = fmaf(wu, q.y - q.x, q.x);
-                const float a1 = fmaf(wu, q.w - q.z, q.z);
-                const float s = fmaf(wv, a1 - a0, a0);
+                const float s = sample(pjb, colb, wu, __builtin_amdgcn_fmed3f(pv_raw, 0.0f, hv));
                 if (kMatched) {
                     const float Dz = fmaf(pv_raw, dV, w.dz0);
                     const float L2 = fmaf(Dz, Dz, Day2);
@@ -127,15 +134,9 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                 const float vx = fmaf(fz, w.dZ[0], bx), vy = fmaf(fz, w.dZ[1], by), vz = fmaf(fz, w.dZ[2], bz);
                 const float t = kParallel ? 1.0f : w.tnum * rcp(vx);
                 const float pu_raw = fmaf(vy, t, w.uoff), pv_raw = fmaf(vz, t, w.voff);
-                const float pu = clampf(pu_raw, hu);
-                const float pv = clampf(pv_raw, hv);
-                const float flu = floorf(pu), flv = floorf(pv);
-                const float wu = pu - flu, wv = pv - flv;
-                const float4 q = *reinterpret_cast<const float4*>(
-                    pjb + (__umul24((unsigned)((int)flv + 1), rowb) + (unsigned)((int)flu + 1) * 16u));
-                const float a0 = fmaf(wu, q.y - q.x, q.x);
-                const float a1 = fmaf(wu, q.w - q.z, q.z);
-                const float s = fmaf(wv, a1 - a0, a0);
+                const float pu = __builtin_amdgcn_fmed3f(pu_raw, 0.0f, hu);
+                const float s = sample(pjb, (unsigned)pu * 16u, __builtin_amdgcn_fractf(pu),
+                                       __builtin_amdgcn_fmed3f(pv_raw, 0.0f, hv));
                 if (kMatched) {
                     const float Day = fmaf(pu_raw, dU, w.day0);
                     const float Dz = fmaf(pv_raw, dV, w.dz0);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(256) void quad_projections_kernel(const void* __res
                                    dtype);
             t[k] = val;
         }
-        quads[q] = make_float4(t[0], t[1], t[2], t[3]);
+        quads[q] = make_float4(t[0], t[1] - t[0], t[2], t[3] - t[2]);
     }
 }
 

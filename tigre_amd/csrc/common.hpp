@@ -100,8 +100,8 @@ struct BwdView {
     float dZ[3];
     float src[3];      // source
     float tnum;        // DSO - DSD - src.x
-    float uoff;        // nDetecU/2 - 0.5  (texture coordinate -> pixel index)
-    float voff;        // nDetecV/2 - 0.5
+    float uoff;        // nDetecU/2 - 0.5 + 1 (texture coordinate -> pixel index + 1)
+    float voff;        // nDetecV/2 - 0.5 + 1
     // FDK weight (voxel_backprojection.cu:243-249): W = 1/((DSO + ry sin - rx cos)/DSO)^2
     float sina, cosa, DSO, DSD, invDSO;
     float realx0, realy0;   // rx = realx0 + ix*dVx, ry = realy0 + iy*dVy
@@ -109,7 +109,7 @@ struct BwdView {
     float rvx0, rvy0, rvz0; // real position of voxel (0,0,0)
     float Sx, Sy;           // real source position
     float dax;              // -(DSD - DSO)
-    float day0, dz0;        // detector real-position offsets
+    float day0, dz0;        // detector real-position offsets (for coordinates pixel + 1)
     float pad[7];
 };
 static_assert(sizeof(BwdView) == 40 * sizeof(float), "BwdView layout");
@@ -266,10 +266,6 @@ inline bool valid_dtype(int dtype) {
 }
 // float -> dtype, round to nearest even (storage.hip)
 int launch_store_converted(const float* src, void* dst, long long n, int dtype, hipStream_t stream);
-
-// Clamp into [-1, hi]; NaN maps to -1 (fmaxf returns the non-NaN operand), so
-// a clamped coordinate always addresses the zero-padded border or the data.
-__device__ inline float clampf(float x, float hi) { return fminf(fmaxf(x, -1.0f), hi); }
 
 // Quad projection dimensions: tap origins -1 .. n on each detector axis.
 __host__ __device__ inline long quad_det(int n) { return (long)n + 2; }

@@ -257,8 +257,9 @@ void backward_views(const HostGeometry& g, std::vector<BwdView>* out) {
         r.dZ[0] = (float)dz.x; r.dZ[1] = (float)dz.y; r.dZ[2] = (float)dz.z;
         r.src[0] = (float)S.x; r.src[1] = (float)S.y; r.src[2] = (float)S.z;
         r.tnum = (float)(DSO - DSD - S.x);
-        r.uoff = (float)(S.y + g.nDetec[0] * 0.5 - 0.5);
-        r.voff = (float)(S.z + g.nDetec[1] * 0.5 - 0.5);
+        // detector coordinates in the quad copy's storage frame: pixel index + 1
+        r.uoff = (float)(S.y + g.nDetec[0] * 0.5 - 0.5 + 1.0);
+        r.voff = (float)(S.z + g.nDetec[1] * 0.5 - 0.5 + 1.0);
         const double sa = std::sin(al), ca = std::cos(al);
         r.sina = (float)sa;
         r.cosa = (float)ca;
@@ -273,9 +274,10 @@ void backward_views(const HostGeometry& g, std::vector<BwdView>* out) {
         r.Sx = (float)(DSO * ca);
         r.Sy = (float)(-DSO * sa);
         r.dax = (float)(-(DSD - DSO));
-        // realD uses the texture coordinate u = pixel + 0.5 (voxel_backprojection2.cu:259-260)
-        r.day0 = (float)((-g.sDetec[0] + dU) * 0.5 + offU + 0.5 * dU);
-        r.dz0 = (float)((-g.sDetec[1] + dV) * 0.5 + offV + 0.5 * dV);
+        // realD uses the texture coordinate u = pixel + 0.5 (voxel_backprojection2.cu:259-260);
+        // the kernel's coordinate is pixel + 1, hence the further - dU, - dV
+        r.day0 = (float)((-g.sDetec[0] + dU) * 0.5 + offU + 0.5 * dU - dU);
+        r.dz0 = (float)((-g.sDetec[1] + dV) * 0.5 + offV + 0.5 * dV - dV);
         for (float& x : r.pad) x = 0.0f;
     }
 }
