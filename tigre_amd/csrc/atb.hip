@@ -36,6 +36,37 @@ namespace {
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 }
 
+// Writes a lane's kZ accumulated voxels: the plain result acc * scale or,
+// with ep.x set, the SART update x + inv_v * (acc * scale) (clamped at 0 when
+// ep.noneg) with the rounding steps of that tensor expression.
+template <int kZ>
+__device__ __forceinline__ void store_column(const float (&acc)[kZ], float* vol, int ix, int iy, int z0, int nx,
+                                             int ny, int nz, float scale, const AtbUpdate& ep) {
+    const long long plane = (long long)nx * ny;
+    const long long at = (long long)z0 * plane + (long long)iy * nx + ix;
+    float* out = vol + at;
+    if (ep.x == nullptr) {
+#pragma unroll
+        for (int k = 0; k < kZ; ++k)
+            if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
+    } else {
+        // three roundings, no fused multiply-add (contraction is off in this block)
+#pragma clang fp contract(off)
+        const float inv = ep.inv_v[(long long)iy * nx + ix];
+        const float* xin = ep.x + at;
+#pragma unroll
+        for (int k = 0; k < kZ; ++k) {
+            if (z0 + k < nz) {
+                const float b = acc[k] * scale;
+                const float t = inv * b;
+                float v = xin[(long long)k * plane] + t;
+                if (ep.noneg) v = fmaxf(v, 0.0f);
+                out[(long long)k * plane] = v;
+            }
+        }
+    }
+}
+
 // kZ: slices per lane (4, 8 or 16; fewer gives more workgroups on small
 // volumes).  With ep.x set, the result is not stored but applied as the SART
 // update x + inv_v[iy, ix] * result, clamped at 0 when ep.noneg, written to
@@ -150,30 +181,192 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
             }
         }
     }
-    const long long plane = (long long)nx * ny;
-    const long long at = (long long)z0 * plane + (long long)iy * nx + ix;
-    float* out = vol + at;
-    if (ep.x == nullptr) {
+    store_column<kZ>(acc, vol, ix, iy, z0, nx, ny, nz, scale, ep);
+}
+
+// LDS-staged variant for axial views with the FDK or parallel-beam weight.
+// The gather above moves 16 bytes through the texture-address path per voxel
+// update, and that path (about 64 bytes per clock per CU) bounds it.  Here
+// every wavefront copies the detector window its 8 x 8 x kZ voxel patch
+// projects to (at most kWinRows x kWinCols quads, about 14 x 21 at unit
+// magnification) into its own LDS region with direct global-to-LDS loads, and
+// gathers from LDS (128 bytes per clock).  The window of view p + 1 is
+// computed and its copy issued before the updates of view p, into the other
+// half of a double buffer, so the copy overlaps the arithmetic; each
+// wavefront only touches its own region, so no barrier is needed, only a wait
+// for its own loads.  The window is the projection of the patch's corners:
+// u and v are linear-fractional in the voxel position (the source is never
+// inside the patch), so over the patch their extremes lie at corners, and
+// one quad of margin on each side covers the rounding of the other lanes.
+// A view whose window does not fit is gathered from global memory as in
+// atb_kernel.  Both paths read the same quads with the same arithmetic, so
+// the results do not depend on which one a view takes.
+namespace {
+constexpr int kWinCols = 16;                    // quads per staged row
+constexpr int kWinLoads = 6;                    // direct loads per window, 4 rows each
+constexpr int kWinRows = 4 * kWinLoads;
+constexpr int kWinQuads = kWinCols * kWinRows;  // 384 quads, 6 KiB
+}  // namespace
+
+// The kZ samples of one column and view, from a staged window (LDS) or from
+// the quad copy in global memory: every read is issued before any arithmetic
+// needs it.  The LDS pointer is restrict-qualified: its reads then carry
+// alias-scope information, and the compiler does not make them wait for the
+// direct LDS loads of the next view (into the other half of the double
+// buffer) that are still in flight; the kernel waits for a view's own loads
+// explicitly before its updates.
+template <int kZ>
+__device__ __forceinline__ void samples_lds(const char* __restrict__ lds, int lb, float pv0, float dpv, float wu,
+                                            float hv, float (&smp)[kZ]) {
+    float4 q[kZ];
+    float wv[kZ];
 #pragma unroll
-        for (int k = 0; k < kZ; ++k)
-            if (z0 + k < nz) out[(long long)k * plane] = acc[k] * scale;
-    } else {
-        // the rounding steps of x + (inv_v * (acc * scale)): three roundings,
-        // no fused multiply-add (contraction is off in this block)
-#pragma clang fp contract(off)
-        const float inv = ep.inv_v[(long long)iy * nx + ix];
-        const float* xin = ep.x + at;
+    for (int k = 0; k < kZ; ++k) {
+        const float pv = __builtin_amdgcn_fmed3f(fmaf((float)k, dpv, pv0), 0.0f, hv);
+        wv[k] = __builtin_amdgcn_fractf(pv);
+        q[k] = *reinterpret_cast<const float4*>(lds + ((int)((unsigned)pv << 8) + lb));
+    }
 #pragma unroll
-        for (int k = 0; k < kZ; ++k) {
-            if (z0 + k < nz) {
-                const float b = acc[k] * scale;
-                const float t = inv * b;
-                float v = xin[(long long)k * plane] + t;
-                if (ep.noneg) v = fmaxf(v, 0.0f);
-                out[(long long)k * plane] = v;
+    for (int k = 0; k < kZ; ++k) {
+        const float a0 = fmaf(wu, q[k].y, q[k].x);
+        const float a1 = fmaf(wu, q[k].w, q[k].z);
+        smp[k] = fmaf(wv[k], a1 - a0, a0);
+    }
+}
+
+template <int kZ>
+__device__ __forceinline__ void samples_global(const char* pjb, unsigned rowb, unsigned colb, float pv0, float dpv,
+                                               float wu, float hv, float (&smp)[kZ]) {
+    float4 q[kZ];
+    float wv[kZ];
+#pragma unroll
+    for (int k = 0; k < kZ; ++k) {
+        const float pv = __builtin_amdgcn_fmed3f(fmaf((float)k, dpv, pv0), 0.0f, hv);
+        wv[k] = __builtin_amdgcn_fractf(pv);
+        q[k] = *reinterpret_cast<const float4*>(pjb + (__umul24((unsigned)pv, rowb) + colb));
+    }
+#pragma unroll
+    for (int k = 0; k < kZ; ++k) {
+        const float a0 = fmaf(wu, q[k].y, q[k].x);
+        const float a1 = fmaf(wu, q[k].w, q[k].z);
+        smp[k] = fmaf(wv[k], a1 - a0, a0);
+    }
+}
+
+template <int kW, int kZ>
+__global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
+                                                      const BwdView* __restrict__ views, float* vol, int nx,
+                                                      int ny, int nz, float dvx, float dvy, float scale,
+                                                      AtbUpdate ep) {
+    static_assert(kW == 0 || kW == 2, "FDK or parallel-beam weight");
+    __shared__ float4 win[4][2][kWinQuads];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int z0 = blockIdx.z * kZ;
+    // no early exit: every lane takes part in the copies, and the window is
+    // read from the corner lanes 0, 7, 56, 63; lanes outside the volume only
+    // skip the store
+
+    const long long QU = quad_det(nU);
+    const long long QP = QU * quad_det(nV);
+    const char* qbase = reinterpret_cast<const char*>(proj);
+    const unsigned rowb = (unsigned)QU * 16u;
+    const float hu = (float)(nU + 1), hv = (float)(nV + 1);
+    const float fx = (float)ix, fy = (float)iy;
+    const int lrow = lane >> 4, lcol = lane & 15;  // this lane's quad in each 4 x 16 copy
+    const char* lds = reinterpret_cast<const char*>(&win[0][0][0]);
+
+    // per view and column: u weight, quad column (bytes), v at slice z0 and
+    // its step, distance weight; the window and the column's LDS offset
+    struct Col {
+        float wu, pv0, dpv, wt;
+        unsigned colb;
+        int lb;      // LDS byte offset of (row 0, this column) in the view's window; may be negative
+        bool staged;
+    };
+    auto column = [&](int p, Col& c) {
+        const BwdView& w = views[p];
+        const float bx = fmaf(fy, w.dY[0], fmaf(fx, w.dX[0], w.org[0]));
+        const float by = fmaf(fy, w.dY[1], fmaf(fx, w.dX[1], w.org[1]));
+        const float bz = fmaf(fy, w.dY[2], fmaf(fx, w.dX[2], w.org[2]));
+        c.wt = 1.0f;
+        if (kW == 0) {  // voxel_backprojection.cu:241-249
+            const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
+            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) * w.invDSO;
+            c.wt = rcp(ww * ww);
+        }
+        const float t = kW == 2 ? 1.0f : w.tnum * rcp(bx);
+        const float pu = __builtin_amdgcn_fmed3f(fmaf(by, t, w.uoff), 0.0f, hu);
+        c.wu = __builtin_amdgcn_fractf(pu);
+        c.colb = (unsigned)pu * 16u;
+        c.dpv = w.dZ[2] * t;
+        c.pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
+        // rows of the first and last slice (v is monotonic along the column)
+        const unsigned ra = (unsigned)__builtin_amdgcn_fmed3f(c.pv0, 0.0f, hv);
+        const unsigned rb = (unsigned)__builtin_amdgcn_fmed3f(fmaf((float)(kZ - 1), c.dpv, c.pv0), 0.0f, hv);
+        const unsigned rpack = min(ra, rb) | (max(ra, rb) << 16);
+        unsigned rlo = 0xffffu, rhi = 0u, clo = 0xffffffffu, chi = 0u;
+        const int corner[4] = {0, 7, 56, 63};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const unsigned r = (unsigned)__builtin_amdgcn_readlane((int)rpack, corner[j]);
+            const unsigned cb = (unsigned)__builtin_amdgcn_readlane((int)c.colb, corner[j]);
+            rlo = min(rlo, r & 0xffffu);
+            rhi = max(rhi, r >> 16);
+            clo = min(clo, cb >> 4);
+            chi = max(chi, cb >> 4);
+        }
+        const int r0 = max((int)rlo - 1, 0), r1 = min((int)rhi + 1, nV + 1);
+        const int c0 = max((int)clo - 1, 0), c1 = min((int)chi + 1, nU + 1);
+        c.staged = r1 - r0 < kWinRows && c1 - c0 < kWinCols;
+        const int buf = p & 1;
+        c.lb = (int)c.colb - c0 * 16 - r0 * (kWinCols * 16) + (wave * 2 + buf) * (kWinQuads * 16);
+        if (c.staged) {
+            // rows r0 .. r0 + 4 nl - 1, columns c0 .. c0 + 15 (clamped to the
+            // quad copy: the extra quads are never read)
+            const char* pjb = qbase + (long long)p * QP * 16;
+            const unsigned gcol = (unsigned)min(c0 + lcol, nU + 1) * 16u;
+            float4* dst = &win[wave][buf][0];
+            const int nl = (r1 - r0 + 4) >> 2;
+#pragma unroll
+            for (int j = 0; j < kWinLoads; ++j) {
+                if (j < nl) {
+                    const unsigned row = (unsigned)min(r0 + 4 * j + lrow, nV + 1);
+                    __builtin_amdgcn_global_load_lds(pjb + (__umul24(row, rowb) + gcol), dst + 64 * j, 16, 0, 0);
+                }
             }
         }
+    };
+
+    float acc[kZ];
+#pragma unroll
+    for (int k = 0; k < kZ; ++k) acc[k] = 0.0f;
+
+    // one view: wait for its window, set up and issue the next view's, then
+    // the kZ updates (two Col records alternate, so nothing is copied)
+    auto step = [&](int p, const Col& c, Col& next) {
+        __builtin_amdgcn_s_waitcnt(0);  // this view's window has landed (nothing else is in flight)
+        if (p + 1 < nviews) column(p + 1, next);
+        // the bilinear arithmetic sits in each branch: with it after the
+        // branch, the two gathers would be merged into generic (flat) loads
+        float smp[kZ];
+        if (c.staged)
+            samples_lds<kZ>(lds, c.lb, c.pv0, c.dpv, c.wu, hv, smp);
+        else
+            samples_global<kZ>(qbase + (long long)p * QP * 16, rowb, c.colb, c.pv0, c.dpv, c.wu, hv, smp);
+#pragma unroll
+        for (int k = 0; k < kZ; ++k) acc[k] = fmaf(smp[k], c.wt, acc[k]);
+    };
+    Col ca, cb;
+    if (nviews > 0) column(0, ca);
+    int p = 0;
+    for (; p + 1 < nviews; p += 2) {
+        step(p, ca, cb);
+        step(p + 1, cb, ca);
     }
+    if (p < nviews) step(p, ca, cb);
+    if (ix < nx && iy < ny) store_column<kZ>(acc, vol, ix, iy, z0, nx, ny, nz, scale, ep);
 }
 
 // Quad copy of the projections (see above), view-major, rows of quad_det(nU)
@@ -240,16 +433,31 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
         else if (zper == 8) TIGRE_ATB_LAUNCH(M, A, 8);     \
         else TIGRE_ATB_LAUNCH(M, A, 4);                    \
     } while (0)
+#define TIGRE_ATB_LDS_LAUNCH(M, Z)                                                                            \
+    hipLaunchKernelGGL((atb_lds_kernel<M, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1], \
+                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[5], ep)
+#define TIGRE_ATB_LDS_Z(M)                                 \
+    do {                                                   \
+        if (zper == 16) TIGRE_ATB_LDS_LAUNCH(M, 16);       \
+        else if (zper == 8) TIGRE_ATB_LDS_LAUNCH(M, 8);    \
+        else TIGRE_ATB_LDS_LAUNCH(M, 4);                   \
+    } while (0)
+    // the staged kernel packs two row indices into 16 bits each
+    const bool staged = axial && kind != 1 && g.nDetec[1] + 1 < 65536;
     if (kind == 1) {
         if (axial) TIGRE_ATB_Z(1, true);
         else TIGRE_ATB_Z(1, false);
     } else if (kind == 0) {
-        if (axial) TIGRE_ATB_Z(0, true);
+        if (staged) TIGRE_ATB_LDS_Z(0);
+        else if (axial) TIGRE_ATB_Z(0, true);
         else TIGRE_ATB_Z(0, false);
     } else {
-        if (axial) TIGRE_ATB_Z(2, true);
+        if (staged) TIGRE_ATB_LDS_Z(2);
+        else if (axial) TIGRE_ATB_Z(2, true);
         else TIGRE_ATB_Z(2, false);
     }
+#undef TIGRE_ATB_LDS_Z
+#undef TIGRE_ATB_LDS_LAUNCH
 #undef TIGRE_ATB_Z
 #undef TIGRE_ATB_LAUNCH
     TIGRE_HIP_TRY(hipGetLastError());
