@@ -8,15 +8,15 @@
 // hit a few cache lines.  Per-view parameters are uniform across the
 // workgroup and are fetched with scalar loads.
 //
-// Projections are read from a "quad" copy: element (u, v) holds
-// p(u,v), p(u+1,v) - p(u,v), p(u,v+1), p(u+1,v+1) - p(u,v+1) (zero outside
-// the detector) for tap origins u in [-1, nU], v in [-1, nV], stored at index
-// (u + 1, v + 1).  One 16-byte load per voxel update replaces four scattered
-// taps (the texture-address unit bounds this kernel alongside the ALU), the
-// zero border replaces the border-addressed linear texture of the reference
-// without bounds checks, and the two u differences of the bilinear lerp are
-// computed once per detector pixel instead of once per voxel update (the same
-// float subtractions, so the same bits).  Detector coordinates are carried in
+// Projections are read from a "quad" copy: with the four taps a = p(u,v),
+// b = p(u+1,v), c = p(u,v+1), d = p(u+1,v+1) (zero outside the detector),
+// element (u, v) holds (a, b - a, c - a, (d - c) - (b - a)) for tap origins
+// u in [-1, nU], v in [-1, nV], stored at index (u + 1, v + 1).  One 16-byte
+// load per voxel update replaces four scattered taps, the zero border
+// replaces the border-addressed linear texture of the reference without
+// bounds checks, and the bilinear sample is three fused multiply-adds
+// (bilerp below): the differences are formed once per detector pixel instead
+// of once per voxel update.  Detector coordinates are carried in
 // that storage frame (pixel index + 1, the offset folded into the per-view
 // records), clamped with one med3 to [0, n + 1]: the tap origin is then the
 // truncating conversion and the weight the hardware fraction.
@@ -34,6 +34,31 @@ namespace tigre {
 namespace {
 // 1/x with the hardware reciprocal (1 ulp), instead of a correctly rounded division.
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Waves per SIMD the gather kernel is compiled for: the register budget that
+// keeps all kZ gathers of a column in flight (the compiler otherwise trades
+// them for occupancy and serialises the loads).
+constexpr int atb_waves(int kW, int kZ) { return kZ >= 16 ? (kW == 1 ? 3 : 4) : kZ == 8 ? (kW == 1 ? 5 : 6) : 8; }
+
+// Minimum / maximum of two wave-uniform values on the scalar unit.
+__device__ __forceinline__ unsigned smin(unsigned a, unsigned b) {
+    unsigned r;
+    asm("s_min_u32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+__device__ __forceinline__ unsigned smax(unsigned a, unsigned b) {
+    unsigned r;
+    asm("s_max_u32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+
+// Bilinear sample from a quad (a, b - a, c - a, (d - c) - (b - a)) at weights
+// (wu, wv): s = a0 + wv (a1 - a0) with a0 = a + wu (b - a) and
+// a1 - a0 = (c - a) + wu ((d - c) - (b - a)).
+__device__ __forceinline__ float bilerp(const float4& q, float wu, float wv) {
+    const float a0 = fmaf(wu, q.y, q.x);
+    return fmaf(wv, fmaf(wu, q.w, q.z), a0);
+}
 }
 
 // Writes a lane's kZ accumulated voxels: the plain result acc * scale or,
@@ -76,7 +101,7 @@ __device__ __forceinline__ void store_column(const float (&acc)[kZ], float* vol,
 // orthogonally, u = y + nU/2 - 0.5, v = z + nV/2 - 0.5, weight 1
 // (voxel_backprojection_parallel.cu:221-251).
 template <int kW, bool kAxial, int kZ>
-__global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(kW, kZ), atb_waves(kW, kZ)))) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
                                                   const BwdView* __restrict__ views, float* vol, int nx,
                                                   int ny, int nz, float dvx, float dvy, float dvz, float dU, float dV,
                                                   float scale, AtbUpdate ep) {
@@ -104,9 +129,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
     auto sample = [&](const char* pjb, unsigned colb, float wu, float pv) {
         const float wv = __builtin_amdgcn_fractf(pv);
         const float4 q = *reinterpret_cast<const float4*>(pjb + (__umul24((unsigned)pv, rowb) + colb));
-        const float a0 = fmaf(wu, q.y, q.x);
-        const float a1 = fmaf(wu, q.w, q.z);
-        return fmaf(wv, a1 - a0, a0);
+        return bilerp(q, wu, wv);
     };
 
     for (int p = 0; p < nviews; ++p) {
@@ -120,8 +143,7 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
         // FDK distance weight, constant along the column (voxel_backprojection.cu:241-249)
         float wfdk = 1.0f;
         if (kW == 0) {
-            const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
-            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) * w.invDSO;
+            const float ww = fmaf(fy, w.wfdk[2], fmaf(fx, w.wfdk[1], w.wfdk[0]));
             wfdk = rcp(ww * ww);
         }
         // matched weight: squared in-plane source-voxel distance, constant along z
@@ -144,10 +166,20 @@ __global__ __launch_bounds__(256) void atb_kernel(const float* __restrict__ proj
                 const float Day = fmaf(pu_raw, dU, w.day0);
                 Day2 = fmaf(Day, Day, w.DSD * w.DSD);
             }
+            // every gather of the column first, then the arithmetic (keeps the
+            // kZ loads in flight together)
+            float4 q[kZ];
+            float wv[kZ];
+#pragma unroll
+            for (int k = 0; k < kZ; ++k) {
+                const float pv = __builtin_amdgcn_fmed3f(fmaf((float)k, dpv, pv0), 0.0f, hv);
+                wv[k] = __builtin_amdgcn_fractf(pv);
+                q[k] = *reinterpret_cast<const float4*>(pjb + (__umul24((unsigned)pv, rowb) + colb));
+            }
 #pragma unroll
             for (int k = 0; k < kZ; ++k) {
                 const float pv_raw = fmaf((float)k, dpv, pv0);
-                const float s = sample(pjb, colb, wu, __builtin_amdgcn_fmed3f(pv_raw, 0.0f, hv));
+                const float s = bilerp(q[k], wu, wv[k]);
                 if (kMatched) {
                     const float Dz = fmaf(pv_raw, dV, w.dz0);
                     const float L2 = fmaf(Dz, Dz, Day2);
@@ -227,11 +259,7 @@ __device__ __forceinline__ void samples_lds(const char* __restrict__ lds, int lb
         q[k] = *reinterpret_cast<const float4*>(lds + ((int)((unsigned)pv << 8) + lb));
     }
 #pragma unroll
-    for (int k = 0; k < kZ; ++k) {
-        const float a0 = fmaf(wu, q[k].y, q[k].x);
-        const float a1 = fmaf(wu, q[k].w, q[k].z);
-        smp[k] = fmaf(wv[k], a1 - a0, a0);
-    }
+    for (int k = 0; k < kZ; ++k) smp[k] = bilerp(q[k], wu, wv[k]);
 }
 
 template <int kZ>
@@ -246,21 +274,17 @@ __device__ __forceinline__ void samples_global(const char* pjb, unsigned rowb, u
         q[k] = *reinterpret_cast<const float4*>(pjb + (__umul24((unsigned)pv, rowb) + colb));
     }
 #pragma unroll
-    for (int k = 0; k < kZ; ++k) {
-        const float a0 = fmaf(wu, q[k].y, q[k].x);
-        const float a1 = fmaf(wu, q[k].w, q[k].z);
-        smp[k] = fmaf(wv[k], a1 - a0, a0);
-    }
+    for (int k = 0; k < kZ; ++k) smp[k] = bilerp(q[k], wu, wv[k]);
 }
 
 template <int kW, int kZ>
 __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
                                                       const BwdView* __restrict__ views, float* vol, int nx,
-                                                      int ny, int nz, float dvx, float dvy, float scale,
-                                                      AtbUpdate ep) {
+                                                      int ny, int nz, float scale, AtbUpdate ep) {
     static_assert(kW == 0 || kW == 2, "FDK or parallel-beam weight");
     __shared__ float4 win[4][2][kWinQuads];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: LDS bases stay scalar
     const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
     const int z0 = blockIdx.z * kZ;
@@ -274,7 +298,8 @@ __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict
     const unsigned rowb = (unsigned)QU * 16u;
     const float hu = (float)(nU + 1), hv = (float)(nV + 1);
     const float fx = (float)ix, fy = (float)iy;
-    const int lrow = lane >> 4, lcol = lane & 15;  // this lane's quad in each 4 x 16 copy
+    // this lane's quad in each 4 x 16 block of a copy: row lane / 16, column lane % 16
+    const unsigned laneoff = (unsigned)(lane >> 4) * rowb + (unsigned)(lane & 15) * 16u;
     const char* lds = reinterpret_cast<const char*>(&win[0][0][0]);
 
     // per view and column: u weight, quad column (bytes), v at slice z0 and
@@ -292,8 +317,7 @@ __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict
         const float bz = fmaf(fy, w.dY[2], fmaf(fx, w.dX[2], w.org[2]));
         c.wt = 1.0f;
         if (kW == 0) {  // voxel_backprojection.cu:241-249
-            const float rx = fmaf(fx, dvx, w.realx0), ry = fmaf(fy, dvy, w.realy0);
-            const float ww = (w.DSO + ry * w.sina - rx * w.cosa) * w.invDSO;
+            const float ww = fmaf(fy, w.wfdk[2], fmaf(fx, w.wfdk[1], w.wfdk[0]));
             c.wt = rcp(ww * ww);
         }
         const float t = kW == 2 ? 1.0f : w.tnum * rcp(bx);
@@ -306,16 +330,17 @@ __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict
         const unsigned ra = (unsigned)__builtin_amdgcn_fmed3f(c.pv0, 0.0f, hv);
         const unsigned rb = (unsigned)__builtin_amdgcn_fmed3f(fmaf((float)(kZ - 1), c.dpv, c.pv0), 0.0f, hv);
         const unsigned rpack = min(ra, rb) | (max(ra, rb) << 16);
+        // corner values, reduced on the scalar unit
         unsigned rlo = 0xffffu, rhi = 0u, clo = 0xffffffffu, chi = 0u;
         const int corner[4] = {0, 7, 56, 63};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const unsigned r = (unsigned)__builtin_amdgcn_readlane((int)rpack, corner[j]);
-            const unsigned cb = (unsigned)__builtin_amdgcn_readlane((int)c.colb, corner[j]);
-            rlo = min(rlo, r & 0xffffu);
-            rhi = max(rhi, r >> 16);
-            clo = min(clo, cb >> 4);
-            chi = max(chi, cb >> 4);
+            const unsigned cb = (unsigned)__builtin_amdgcn_readlane((int)c.colb, corner[j]) >> 4;
+            rlo = smin(rlo, r & 0xffffu);
+            rhi = smax(rhi, r >> 16);
+            clo = smin(clo, cb);
+            chi = smax(chi, cb);
         }
         const int r0 = max((int)rlo - 1, 0), r1 = min((int)rhi + 1, nV + 1);
         const int c0 = max((int)clo - 1, 0), c1 = min((int)chi + 1, nU + 1);
@@ -323,17 +348,18 @@ __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict
         const int buf = p & 1;
         c.lb = (int)c.colb - c0 * 16 - r0 * (kWinCols * 16) + (wave * 2 + buf) * (kWinQuads * 16);
         if (c.staged) {
-            // rows r0 .. r0 + 4 nl - 1, columns c0 .. c0 + 15 (clamped to the
-            // quad copy: the extra quads are never read)
-            const char* pjb = qbase + (long long)p * QP * 16;
-            const unsigned gcol = (unsigned)min(c0 + lcol, nU + 1) * 16u;
+            // rows r0 .. r0 + 4 nl - 1, columns c0 .. c0 + 15: the quads past
+            // the window belong to the next row or view or to the slack after
+            // the copy, and are never read.  The address is a uniform base
+            // plus the lane's fixed offset.
+            const char* blk = qbase + ((long long)p * QP + (long long)r0 * QU + c0) * 16;
             float4* dst = &win[wave][buf][0];
             const int nl = (r1 - r0 + 4) >> 2;
 #pragma unroll
             for (int j = 0; j < kWinLoads; ++j) {
                 if (j < nl) {
-                    const unsigned row = (unsigned)min(r0 + 4 * j + lrow, nV + 1);
-                    __builtin_amdgcn_global_load_lds(pjb + (__umul24(row, rowb) + gcol), dst + 64 * j, 16, 0, 0);
+                    const char* bj = blk + (long long)(4 * j) * rowb;
+                    __builtin_amdgcn_global_load_lds(bj + laneoff, dst + 64 * j, 16, 0, 0);
                 }
             }
         }
@@ -390,7 +416,8 @@ __global__ __launch_bounds__(256) void quad_projections_kernel(const void* __res
                                    dtype);
             t[k] = val;
         }
-        quads[q] = make_float4(t[0], t[1] - t[0], t[2], t[3] - t[2]);
+        const float du0 = t[1] - t[0], du1 = t[3] - t[2];
+        quads[q] = make_float4(t[0], du0, t[2] - t[0], du1 - du0);
     }
 }
 
@@ -435,7 +462,7 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
     } while (0)
 #define TIGRE_ATB_LDS_LAUNCH(M, Z)                                                                            \
     hipLaunchKernelGGL((atb_lds_kernel<M, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1], \
-                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[5], ep)
+                       g.nangles, d_views, vol, nx, ny, nz, a[5], ep)
 #define TIGRE_ATB_LDS_Z(M)                                 \
     do {                                                   \
         if (zper == 16) TIGRE_ATB_LDS_LAUNCH(M, 16);       \

@@ -102,15 +102,17 @@ struct BwdView {
     float tnum;        // DSO - DSD - src.x
     float uoff;        // nDetecU/2 - 0.5 + 1 (texture coordinate -> pixel index + 1)
     float voff;        // nDetecV/2 - 0.5 + 1
-    // FDK weight (voxel_backprojection.cu:243-249): W = 1/((DSO + ry sin - rx cos)/DSO)^2
-    float sina, cosa, DSO, DSD, invDSO;
-    float realx0, realy0;   // rx = realx0 + ix*dVx, ry = realy0 + iy*dVy
+    // FDK weight (voxel_backprojection.cu:243-249): W = 1/U^2 with
+    // U = (DSO + ry sin - rx cos)/DSO, affine in the voxel index:
+    // U = wfdk[0] + ix wfdk[1] + iy wfdk[2]
+    float wfdk[3];
+    float DSD;
     // matched weight (voxel_backprojection2.cu:201-279)
     float rvx0, rvy0, rvz0; // real position of voxel (0,0,0)
     float Sx, Sy;           // real source position
     float dax;              // -(DSD - DSO)
     float day0, dz0;        // detector real-position offsets (for coordinates pixel + 1)
-    float pad[7];
+    float pad[10];
 };
 static_assert(sizeof(BwdView) == 40 * sizeof(float), "BwdView layout");
 
@@ -269,5 +271,10 @@ int launch_store_converted(const float* src, void* dst, long long n, int dtype, 
 
 // Quad projection dimensions: tap origins -1 .. n on each detector axis.
 __host__ __device__ inline long quad_det(int n) { return (long)n + 2; }
+// Rows of quads allocated after the last view of a quad copy: the staged
+// backprojector copies whole 4 x 16-quad blocks of a detector window, which
+// may run up to 23 rows past a view's last row and 15 quads past a row's end
+// (into the next view or row, never read; past the last view, into this slack).
+constexpr long kQuadSlackRows = 25;
 
 }  // namespace tigre

@@ -261,13 +261,13 @@ void backward_views(const HostGeometry& g, std::vector<BwdView>* out) {
         r.uoff = (float)(S.y + g.nDetec[0] * 0.5 - 0.5 + 1.0);
         r.voff = (float)(S.z + g.nDetec[1] * 0.5 - 0.5 + 1.0);
         const double sa = std::sin(al), ca = std::cos(al);
-        r.sina = (float)sa;
-        r.cosa = (float)ca;
-        r.DSO = (float)DSO;
         r.DSD = (float)DSD;
-        r.invDSO = (float)(1.0 / DSO);
-        r.realx0 = (float)(-(g.sVoxel[0] - g.dVoxel[0]) * 0.5 + off[0]);
-        r.realy0 = (float)(-(g.sVoxel[1] - g.dVoxel[1]) * 0.5 + off[1] + COR);
+        // real position of voxel (ix, iy): rx = realx0 + ix dVx, ry = realy0 + iy dVy
+        const double realx0 = -(g.sVoxel[0] - g.dVoxel[0]) * 0.5 + off[0];
+        const double realy0 = -(g.sVoxel[1] - g.dVoxel[1]) * 0.5 + off[1] + COR;
+        r.wfdk[0] = (float)((DSO + realy0 * sa - realx0 * ca) / DSO);
+        r.wfdk[1] = (float)(-g.dVoxel[0] * ca / DSO);
+        r.wfdk[2] = (float)(g.dVoxel[1] * sa / DSO);
         r.rvx0 = (float)(-g.sVoxel[0] / 2 + g.dVoxel[0] / 2 + off[0]);
         r.rvy0 = (float)(-g.sVoxel[1] / 2 + g.dVoxel[1] / 2 + off[1]);
         r.rvz0 = (float)(-g.sVoxel[2] / 2 + g.dVoxel[2] / 2 + off[2]);

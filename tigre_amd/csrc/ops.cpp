@@ -181,7 +181,8 @@ int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int pr
 }
 
 size_t atb_workspace_bytes(const HostGeometry& g) {
-    return align256((size_t)quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * (size_t)g.nangles * 4 * sizeof(float));
+    const size_t QU = (size_t)quad_det(g.nDetec[0]);
+    return align256((QU * quad_det(g.nDetec[1]) * (size_t)g.nangles + QU * kQuadSlackRows) * 4 * sizeof(float));
 }
 
 int atb_launch(const void* d_proj, int dtype, const HostGeometry& g, const BwdView* d_views, bool axial,
