@@ -252,8 +252,8 @@ __device__ __forceinline__ float box_sample(const float* __restrict__ src, int S
 // float arithmetic, exactly; the staging copy in global memory needs the
 // integer form.
 template <int kMode, bool kLds>
-__device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ src, int SP, int SZ, int P0, int Z0,
-                                   int A0, int lo, int hi, int dir, float hP, float hA, float hz) {
+__device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ base, unsigned slot, int SP, int SZ,
+                                   int P0, int Z0, int A0, int lo, int hi, int dir, float hP, float hA, float hz) {
     int m = r.m;
     const int count = r.count;
     const float qP = r.qP, qA = r.qA, qz = r.qz, fP = r.fP, fA = r.fA, fz = r.fz;
@@ -276,47 +276,53 @@ __device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ src,
         while (me < count && inside(me)) ++me;
     }
     const bool done = me >= count;
-    // trilinear: the 8 taps of the last cell stay in registers; consecutive
-    // samples (half a voxel apart at the default accuracy) mostly share a
-    // cell, so only the lanes that moved to a new cell read LDS again
-    const float fP0 = (float)P0, fZ0 = (float)Z0, fA0 = (float)A0, fSP = (float)SP, fSZ = (float)SZ;
+    // trilinear: the 8 taps of the last cell and their four differences along
+    // A stay in registers; consecutive samples (half a voxel apart at the
+    // default accuracy) mostly share a cell, so only the lanes that moved to a
+    // new cell read again.  The sample index runs as an exact float.
+    const float fP0 = (float)P0, fSP = (float)SP, fSZ = (float)SZ;
+    // A0 + Z0 SZ - slot, so that fma(flZ, SZ, flA - cza) = (flZ - Z0) SZ + flA - A0 + slot
+    // (exact: every term is an integer of magnitude below 2^24)
+    const float cza = (float)(A0 + Z0 * SZ - (int)slot);
     unsigned cached = 0xffffffffu;
-    float c000 = 0.f, c001 = 0.f, c010 = 0.f, c011 = 0.f, c100 = 0.f, c101 = 0.f, c110 = 0.f, c111 = 0.f;
-    for (; m < me; ++m) {
-        const float fm = (float)m;
-        const float P = posP(m);
+    float c000 = 0.f, c010 = 0.f, c100 = 0.f, c110 = 0.f, d00 = 0.f, d10 = 0.f, d01 = 0.f, d11 = 0.f;
+    const float fme = (float)me;
+    float fm = (float)m;
+    for (; fm < fme; fm += 1.0f) {
+        const float P = __builtin_amdgcn_fmed3f(fmaf(fm, fP, qP), -1.0f, hP);
         const float A = __builtin_amdgcn_fmed3f(fmaf(fm, fA, qA), -1.0f, hA);
         const float Z = __builtin_amdgcn_fmed3f(fmaf(fm, fz, qz), -1.0f, hz);
         if (kMode == 2) {
-            sum += box_sample<kMode>(src, SP, SZ, P, A, Z, P0, Z0, A0);
+            sum += box_sample<kMode>(base + slot, SP, SZ, P, A, Z, P0, Z0, A0);
         } else {
             const float flP = floorf(P), flA = floorf(A), flZ = floorf(Z);
             const float wP = P - flP, wA = A - flA, wZ = Z - flZ;
             const unsigned o =
-                kLds ? (unsigned)fmaf(flP - fP0, fSP, fmaf(flZ - fZ0, fSZ, flA - fA0))
+                kLds ? (unsigned)fmaf(flP - fP0, fSP, fmaf(flZ, fSZ, flA - cza))
                      : __umul24((unsigned)((int)flP - P0), (unsigned)SP) +
                            __umul24((unsigned)((int)flZ - Z0), (unsigned)SZ) + (unsigned)((int)flA - A0);
             if (o != cached) {
-                c000 = src[o];
-                c001 = src[o + 1];
-                c010 = src[o + SZ];
-                c011 = src[o + SZ + 1];
-                c100 = src[o + SP];
-                c101 = src[o + SP + 1];
-                c110 = src[o + SP + SZ];
-                c111 = src[o + SP + SZ + 1];
+                const float* t = base + o;
+                c000 = t[0];
+                c010 = t[SZ];
+                c100 = t[SP];
+                c110 = t[SP + SZ];
+                d00 = t[1] - c000;
+                d10 = t[SZ + 1] - c010;
+                d01 = t[SP + 1] - c100;
+                d11 = t[SP + SZ + 1] - c110;
                 cached = o;
             }
             // the arithmetic of box_sample, tap for tap
-            const float a00 = fmaf(wA, c001 - c000, c000);
-            const float a10 = fmaf(wA, c011 - c010, c010);
-            const float a01 = fmaf(wA, c101 - c100, c100);
-            const float a11 = fmaf(wA, c111 - c110, c110);
+            const float a00 = fmaf(wA, d00, c000);
+            const float a10 = fmaf(wA, d10, c010);
+            const float a01 = fmaf(wA, d01, c100);
+            const float a11 = fmaf(wA, d11, c110);
             const float b0 = fmaf(wZ, a10 - a00, a00), b1 = fmaf(wZ, a11 - a01, a01);
             sum += fmaf(wP, b1 - b0, b0);
         }
     }
-    r.m = m;
+    r.m = (int)fm;
     r.sum = sum;
     return done;
 }
@@ -497,8 +503,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
             }
         } else if (state == 2) {
             // box origin (-1, -1, -4): the bordered staging copy itself
-            sample_slab<kMode, false>(rs, stage, sg.NZ * sg.NA, sg.NA, -1, -1, -4, -1, last_cell, dir, (float)nP,
-                                      (float)nA, (float)nz);
+            sample_slab<kMode, false>(rs, stage, 0u, sg.NZ * sg.NA, sg.NA, -1, -1, -4, -1, last_cell, dir,
+                                      (float)nP, (float)nA, (float)nz);
         }
     } else {
         const int SP = nzb * na;
@@ -546,7 +552,6 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
             int lo, hi;
             slab_range(s, planes, dir, last_cell, lo, hi);
             const int2 o = sh.origin[s];
-            const float* cur = sh.buf + (s & 1) * kSlabCap;
             const unsigned slot = (unsigned)((s & 1) * kSlabCap);
             if (kMode == 0) {
                 if (state == 2) {
@@ -563,8 +568,8 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                     if (walk<kP, true>(r, sh.buf, slot, (planes + na + nzb + 5) / 2)) state = 0;
                 }
             } else if (state == 2) {
-                if (sample_slab<kMode, true>(rs, cur, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP, (float)nA,
-                                             (float)nz))
+                if (sample_slab<kMode, true>(rs, sh.buf, slot, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP,
+                                             (float)nA, (float)nz))
                     state = 0;
             }
             plo = lo;
