@@ -58,3 +58,36 @@ def test_parallel_atb(cuda, name):
     print("parallel Atb %s rel-L2 %.3e" % (name, err))
     assert err <= TOL
     assert np.array_equal(tigre_amd.Atb(proj, geo, angles, "matched"), got)
+
+
+def test_interpolated_ignores_stale_lds(cuda):
+    """Regression: a sample clamped to the far face (P = nP) of a sweep towards -P
+    lies beyond every slab; it must be skipped, not read with its upper tap
+    outside the slab box.  That read had weight 0, so it only showed when the
+    LDS still held NaN from an earlier kernel: dirty the LDS first (weight
+    gradients of NaN activations stage them there), then project."""
+    import ctypes
+
+    import torch
+
+    from tigre_amd import _native
+    from tigre_amd.operators import _stream_handle
+
+    lib = _native.lib()
+    shape = (2, 32, 32, 64)
+    n = ctypes.c_size_t()
+    _native.check(lib.tigre_cnn_wgrad_workspace_size(*shape, ctypes.byref(n)), "ws")
+    ws = torch.empty(n.value, dtype=torch.uint8, device="cuda")
+    nan = torch.full(shape + (32,), float("nan"), dtype=torch.bfloat16, device="cuda")
+    out = torch.empty((28, 32, 32), device="cuda")
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    for _ in range(3):
+        _native.check(lib.tigre_cnn_wgrad(p(nan), _native.CNN_IN_BF16X32, p(nan), _native.CNN_IN_BF16X32, p(out),
+                                          *shape, p(ws), n.value, _stream_handle()), "wgrad")
+    geo = _geometries()["default"]
+    angles = np.linspace(0.05, np.pi, 33, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 6, seed=23)
+    got = tigre_amd.Ax(torch.from_numpy(img).to(cuda), geo, angles, "interpolated").cpu().numpy()
+    assert np.isfinite(got).all()
+    ref = oracle.ax(img, prepared(geo, angles), "interpolated")
+    assert oracle.rel_l2(got, ref) <= TOL

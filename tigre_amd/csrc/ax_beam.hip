@@ -11,9 +11,13 @@
 // walk; the slab copies are coalesced and are shared by the 1024 rays.
 //
 // The staging copy ("stage") of the volume is [p][z][a], a = the other
-// horizontal axis fastest, with a zero border: p in -1 .. nP, z in -1 .. nz,
-// a in -4 .. round_up(nA, 4) + 3.  Every voxel a walk can reach (indices -1 ..
-// n) therefore exists, and rows start on 16-byte boundaries.
+// horizontal axis fastest, with a zero border: p in -1 .. nP + 1, z in -1 ..
+// nz, a in -4 .. round_up(nA, 4) + 3.  Every voxel a walk can reach (indices
+// -1 .. n) therefore exists, rows start on 16-byte boundaries, and so does the
+// upper interpolation tap of a sample clamped to the far face (index n + 1:
+// row nz + 1 of plane p is row -1 of plane p + 1, and the extra zero plane
+// nP + 1 ends the copy), as do the whole rows of the last plane a slab box
+// copies.
 //
 // Planning: every wavefront evaluates the 16 corner rays of the tile (four
 // corners in each view) and computes the box origins of a share of the slabs;
@@ -63,8 +67,9 @@ __device__ __forceinline__ float corner_max(float x) {
 // A(p) = cA + kA p, z(p) = cz + kz p (corner ray c in lane c; the other rows of
 // the wavefront hold copies), the voxels of the planes span P coordinates
 // [lo, hi + 1], and a margin of one voxel covers rounding; `tap` = 1 adds the
-// upper interpolation tap.  Clipped to -1 .. n (no walk or tap leaves it); a0
-// is aligned down to 4 for 16-byte rows.
+// upper interpolation tap.  Clipped to -1 .. n + tap (no walk or tap leaves
+// it: a sample clamped to the face n reads the zero tap n + 1 with weight 0);
+// a0 is aligned down to 4 for 16-byte rows.
 struct BoxExtent {
     int a0, a1, z0, z1;
 };
@@ -81,9 +86,9 @@ __device__ inline BoxExtent beam_extent(int lo, int hi, float cA, float kA, floa
     zmax = fminf(fmaxf(zmax, -2.0f), (float)nz + 1.0f);
     BoxExtent e;
     e.a0 = max((int)floorf(amin) - 1, -1) & ~3;
-    e.a1 = min((int)floorf(amax) + 1 + tap, nA);
+    e.a1 = min((int)floorf(amax) + 1 + tap, nA + tap);
     e.z0 = max((int)floorf(zmin) - 1, -1);
-    e.z1 = min((int)floorf(zmax) + 1 + tap, nz);
+    e.z1 = min((int)floorf(zmax) + 1 + tap, nz + tap);
     return e;
 }
 
@@ -261,10 +266,15 @@ __device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ base
     // positions stay in [-1, n] (rounding at the clipped ends), so the taps
     // stay in the zero-bordered range
     auto posP = [&](int k) { return __builtin_amdgcn_fmed3f(fmaf((float)k, fP, qP), -1.0f, hP); };
-    auto inside = [&](int k) {
-        const int pc = (int)floorf(kMode == 2 ? posP(k) + 0.5f : posP(k));
-        return dir > 0 ? pc <= hi : pc >= lo;
-    };
+    auto cell = [&](int k) { return (int)floorf(kMode == 2 ? posP(k) + 0.5f : posP(k)); };
+    auto inside = [&](int k) { return dir > 0 ? cell(k) <= hi : cell(k) >= lo; };
+    // A sample clamped to the far face P = nP lies in cell nP, beyond every
+    // slab (both its taps are on the zero border, so it contributes exactly
+    // 0).  A sweep towards +P meets it last and stops before it; a sweep
+    // towards -P meets it first, and would take it into the first slab with
+    // its upper tap outside the box: skip it.
+    if (dir < 0)
+        while (m < count && cell(m) > hi) ++m;
     // first sample beyond the slab: an estimate, then corrected with the
     // per-sample test itself
     int me = count;
@@ -615,7 +625,7 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
 }
 
 // Staging copy for principal axis P: out[p][z][a] over the bordered ranges
-// p -1 .. nP, z -1 .. nz, a -4 .. na_pad + 3 (zero outside the volume), from the
+// p -1 .. nP + 1, z -1 .. nz, a -4 .. na_pad + 3 (zero outside the volume), from the
 // x-fastest volume.  64 x 64 tiles of (p, a) per z row go through LDS, so both
 // the reads (x fastest) and the writes (a fastest) are coalesced whichever of
 // p, a is x.
@@ -623,7 +633,7 @@ __global__ __launch_bounds__(256) void stage_volume_kernel(const void* __restric
                                                            int nz, int principal, int na_pad,
                                                            float* __restrict__ out) {
     __shared__ float tile[64][65];
-    const int NP = (principal == 0 ? nx : ny) + 2, NZ = nz + 2, NA = na_pad + 8;
+    const int NP = (principal == 0 ? nx : ny) + 3, NZ = nz + 2, NA = na_pad + 8;
     const int a_base = blockIdx.x * 64 - 4, p_base = blockIdx.y * 64 - 1, z = (int)blockIdx.z - 1;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // read: x fastest across threads
@@ -660,7 +670,9 @@ int stage_pad(int n) { return (n + 3) & ~3; }
 long long stage_elements(const HostGeometry& g, int principal) {
     const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
     const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
-    return (long long)(nP + 2) * (g.nVoxel[2] + 2) * (stage_pad(nA) + 8);
+    // planes -1 .. nP + 1 written, plus one unwritten plane of slack: the
+    // last rows a slab box copies may run past the copy by less than a plane
+    return (long long)(nP + 4) * (g.nVoxel[2] + 2) * (stage_pad(nA) + 8);
 }
 
 int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int principal, float* stage,
@@ -668,7 +680,7 @@ int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int p
     const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
     const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
     const int NA = stage_pad(nA) + 8;
-    dim3 grid((unsigned)((NA + 63) / 64), (unsigned)((nP + 2 + 63) / 64), (unsigned)(g.nVoxel[2] + 2));
+    dim3 grid((unsigned)((NA + 63) / 64), (unsigned)((nP + 3 + 63) / 64), (unsigned)(g.nVoxel[2] + 2));
     TIGRE_REQUIRE(grid.y <= 65535u && grid.z <= 65535u, "volume too large for the staging copy");
     KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
     hipLaunchKernelGGL(stage_volume_kernel, grid, dim3(256), 0, stream, vol, dtype, g.nVoxel[0], g.nVoxel[1],
