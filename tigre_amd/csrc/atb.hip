@@ -40,6 +40,15 @@ __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 // them for occupancy and serialises the loads).
 constexpr int atb_waves(int kW, int kZ) { return kZ >= 16 ? (kW == 1 ? 3 : 4) : kZ == 8 ? (kW == 1 ? 5 : 6) : 8; }
 
+// Matched weight l2^(3/2) / s2 from the DSD-normalised squares l2 (source to
+// detector point, about 1) and s2 (source to voxel, about (DSO/DSD)^2): one
+// reciprocal square root, l2^2 / sqrt(l2 s2^2), instead of a square root and a
+// reciprocal.  The normalised operands keep every product near 1, whatever
+// the length unit of the geometry.
+__device__ __forceinline__ float matched_weight(float l2, float s2) {
+    return (l2 * l2) * __builtin_amdgcn_rsqf(l2 * s2 * s2);
+}
+
 // Minimum / maximum of two wave-uniform values on the scalar unit.
 __device__ __forceinline__ unsigned smin(unsigned a, unsigned b) {
     unsigned r;
@@ -103,8 +112,7 @@ __device__ __forceinline__ void store_column(const float (&acc)[kZ], float* vol,
 template <int kW, bool kAxial, int kZ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(kW, kZ), atb_waves(kW, kZ)))) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
                                                   const BwdView* __restrict__ views, float* vol, int nx,
-                                                  int ny, int nz, float dvx, float dvy, float dvz, float dU, float dV,
-                                                  float scale, AtbUpdate ep) {
+                                                  int ny, int nz, float scale, AtbUpdate ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -146,11 +154,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(k
             const float ww = fmaf(fy, w.wfdk[2], fmaf(fx, w.wfdk[1], w.wfdk[0]));
             wfdk = rcp(ww * ww);
         }
-        // matched weight: squared in-plane source-voxel distance, constant along z
+        // matched weight: squared in-plane source-voxel distance / DSD^2, constant along z
         float Axy = 0.0f;
         if (kMatched) {
-            const float ex = w.Sx - fmaf(fx, dvx, w.rvx0);
-            const float ey = w.Sy - fmaf(fy, dvy, w.rvy0);
+            const float ex = w.mS[0] - fmaf(fx, w.mdv[0], w.mrv0[0]);
+            const float ey = w.mS[1] - fmaf(fy, w.mdv[1], w.mrv0[1]);
             Axy = ex * ex + ey * ey;
         }
         if (kAxial) {
@@ -163,8 +171,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(k
             const float pv0 = fmaf(fmaf((float)z0, w.dZ[2], bz), t, w.voff);
             float Day2 = 0.0f;
             if (kMatched) {
-                const float Day = fmaf(pu_raw, dU, w.day0);
-                Day2 = fmaf(Day, Day, w.DSD * w.DSD);
+                const float Day = fmaf(pu_raw, w.mdet[0], w.mdet[2]);
+                Day2 = fmaf(Day, Day, 1.0f);
             }
             // every gather of the column first, then the arithmetic (keeps the
             // kZ loads in flight together)
@@ -181,11 +189,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(k
                 const float pv_raw = fmaf((float)k, dpv, pv0);
                 const float s = bilerp(q[k], wu, wv[k]);
                 if (kMatched) {
-                    const float Dz = fmaf(pv_raw, dV, w.dz0);
-                    const float L2 = fmaf(Dz, Dz, Day2);
-                    const float rvz = fmaf((float)(z0 + k), dvz, w.rvz0);
-                    const float lsq = fmaf(rvz, rvz, Axy);
-                    acc[k] = fmaf(s, L2 * __builtin_amdgcn_sqrtf(L2) * rcp(w.DSD * lsq), acc[k]);
+                    const float Dz = fmaf(pv_raw, w.mdet[1], w.mdet[3]);
+                    const float rvz = fmaf((float)(z0 + k), w.mdv[2], w.mrv0[2]);
+                    acc[k] = fmaf(s, matched_weight(fmaf(Dz, Dz, Day2), fmaf(rvz, rvz, Axy)), acc[k]);
                 } else {
                     acc[k] = fmaf(s, wfdk, acc[k]);
                 }
@@ -201,12 +207,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(k
                 const float s = sample(pjb, (unsigned)pu * 16u, __builtin_amdgcn_fractf(pu),
                                        __builtin_amdgcn_fmed3f(pv_raw, 0.0f, hv));
                 if (kMatched) {
-                    const float Day = fmaf(pu_raw, dU, w.day0);
-                    const float Dz = fmaf(pv_raw, dV, w.dz0);
-                    const float L2 = fmaf(Dz, Dz, fmaf(Day, Day, w.DSD * w.DSD));
-                    const float rvz = fmaf(fz, dvz, w.rvz0);
-                    const float lsq = fmaf(rvz, rvz, Axy);
-                    acc[k] = fmaf(s, L2 * __builtin_amdgcn_sqrtf(L2) * rcp(w.DSD * lsq), acc[k]);
+                    const float Day = fmaf(pu_raw, w.mdet[0], w.mdet[2]);
+                    const float Dz = fmaf(pv_raw, w.mdet[1], w.mdet[3]);
+                    const float rvz = fmaf(fz, w.mdv[2], w.mrv0[2]);
+                    acc[k] = fmaf(s, matched_weight(fmaf(Dz, Dz, fmaf(Day, Day, 1.0f)), fmaf(rvz, rvz, Axy)),
+                                  acc[k]);
                 } else {
                     acc[k] = fmaf(s, wfdk, acc[k]);
                 }
@@ -448,12 +453,10 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
     // matched: final scaling by dVx dVy dVz / (dU dV) (voxel_backprojection2.cu:585)
     const float scale =
         kind == 1 ? (float)(g.dVoxel[0] * g.dVoxel[1] * g.dVoxel[2] / (g.dDetec[0] * g.dDetec[1])) : 1.0f;
-    const float a[7] = {(float)g.dVoxel[0], (float)g.dVoxel[1], (float)g.dVoxel[2], (float)g.dDetec[0],
-                        (float)g.dDetec[1], scale, 0.0f};
     KernelTimerScope timer(kind == 1 ? TIGRE_KERNEL_ATB_MATCHED : TIGRE_KERNEL_ATB_FDK, stream);
 #define TIGRE_ATB_LAUNCH(M, A, Z)                                                                                \
     hipLaunchKernelGGL((atb_kernel<M, A, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1],     \
-                       g.nangles, d_views, vol, nx, ny, nz, a[0], a[1], a[2], a[3], a[4], a[5], ep)
+                       g.nangles, d_views, vol, nx, ny, nz, scale, ep)
 #define TIGRE_ATB_Z(M, A)                                  \
     do {                                                   \
         if (zper == 16) TIGRE_ATB_LAUNCH(M, A, 16);        \
@@ -462,7 +465,7 @@ int launch_atb(const float* quads, const HostGeometry& g, const BwdView* d_views
     } while (0)
 #define TIGRE_ATB_LDS_LAUNCH(M, Z)                                                                            \
     hipLaunchKernelGGL((atb_lds_kernel<M, Z>), grid, dim3(256), 0, stream, quads, g.nDetec[0], g.nDetec[1], \
-                       g.nangles, d_views, vol, nx, ny, nz, a[5], ep)
+                       g.nangles, d_views, vol, nx, ny, nz, scale, ep)
 #define TIGRE_ATB_LDS_Z(M)                                 \
     do {                                                   \
         if (zper == 16) TIGRE_ATB_LDS_LAUNCH(M, 16);       \

@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kBeamTile = 16;       // detector pixels per tile edge
 constexpr int kBeamViews = 4;       // views per workgroup
-constexpr int kBeamThreads = 1024;  // kBeamViews * kBeamTile * kBeamTile
+constexpr int kBeamThreads = kBeamViews * kBeamTile * kBeamTile;
 constexpr int kSlabCap = 9216;      // floats per LDS slab buffer (two per workgroup)
 constexpr int kSlabPlanesMax = 8;   // principal planes per slab, at most (the tunable limit is <= this)
 

@@ -40,7 +40,6 @@ __device__ __forceinline__ unsigned short bf16_bits(float f) {
     if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
-__device__ __forceinline__ float bf16_value(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
 // A fragment of 8 channels (8 q .. 8 q + 7) of voxel `vox`, or zeros.
 template <int kIn>

@@ -107,12 +107,14 @@ struct BwdView {
     // U = wfdk[0] + ix wfdk[1] + iy wfdk[2]
     float wfdk[3];
     float DSD;
-    // matched weight (voxel_backprojection2.cu:201-279)
-    float rvx0, rvy0, rvz0; // real position of voxel (0,0,0)
-    float Sx, Sy;           // real source position
-    float dax;              // -(DSD - DSO)
-    float day0, dz0;        // detector real-position offsets (for coordinates pixel + 1)
-    float pad[10];
+    // matched weight (voxel_backprojection2.cu:201-279), W = L^3 / (DSD |S - x|^2),
+    // L = |S - D(u, v)|; with every length divided by DSD, W = l2^(3/2) / s2 where
+    // l2 = 1 + (Day/DSD)^2 + (Dz/DSD)^2 and s2 = |S - x|^2 / DSD^2
+    float mrv0[3];          // real position of voxel (0,0,0) / DSD
+    float mdv[3];           // voxel size / DSD
+    float mS[2];            // real source position (x, y) / DSD
+    float mdet[4];          // dU / DSD, dV / DSD, detector offsets day0 / DSD, dz0 / DSD (coordinates pixel + 1)
+    float pad[6];
 };
 static_assert(sizeof(BwdView) == 40 * sizeof(float), "BwdView layout");
 

@@ -268,16 +268,18 @@ void backward_views(const HostGeometry& g, std::vector<BwdView>* out) {
         r.wfdk[0] = (float)((DSO + realy0 * sa - realx0 * ca) / DSO);
         r.wfdk[1] = (float)(-g.dVoxel[0] * ca / DSO);
         r.wfdk[2] = (float)(g.dVoxel[1] * sa / DSO);
-        r.rvx0 = (float)(-g.sVoxel[0] / 2 + g.dVoxel[0] / 2 + off[0]);
-        r.rvy0 = (float)(-g.sVoxel[1] / 2 + g.dVoxel[1] / 2 + off[1]);
-        r.rvz0 = (float)(-g.sVoxel[2] / 2 + g.dVoxel[2] / 2 + off[2]);
-        r.Sx = (float)(DSO * ca);
-        r.Sy = (float)(-DSO * sa);
-        r.dax = (float)(-(DSD - DSO));
+        for (int c = 0; c < 3; ++c) {
+            r.mrv0[c] = (float)((-g.sVoxel[c] / 2 + g.dVoxel[c] / 2 + off[c]) / DSD);
+            r.mdv[c] = (float)(g.dVoxel[c] / DSD);
+        }
+        r.mS[0] = (float)(DSO * ca / DSD);
+        r.mS[1] = (float)(-DSO * sa / DSD);
         // realD uses the texture coordinate u = pixel + 0.5 (voxel_backprojection2.cu:259-260);
         // the kernel's coordinate is pixel + 1, hence the further - dU, - dV
-        r.day0 = (float)((-g.sDetec[0] + dU) * 0.5 + offU + 0.5 * dU - dU);
-        r.dz0 = (float)((-g.sDetec[1] + dV) * 0.5 + offV + 0.5 * dV - dV);
+        r.mdet[0] = (float)(dU / DSD);
+        r.mdet[1] = (float)(dV / DSD);
+        r.mdet[2] = (float)(((-g.sDetec[0] + dU) * 0.5 + offU + 0.5 * dU - dU) / DSD);
+        r.mdet[3] = (float)(((-g.sDetec[1] + dV) * 0.5 + offV + 0.5 * dV - dV) / DSD);
         for (float& x : r.pad) x = 0.0f;
     }
 }
