@@ -63,56 +63,78 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     constexpr bool kPadZ = kKind != 2;
     constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
     constexpr int FW = kTX + 2, FH = kTY + 2, NW = kTX + 1, NH = kTY + 1;
-    __shared__ float F[NF][2][FH][FW];       // f planes p - 1, p (ring), halo of one voxel
-    __shared__ float Nn[NF][2][3][NH][NW];   // n planes p - 1, p (ring)
+    __shared__ float F[NF][2][FH * FW];      // f planes p - 1, p (ring), halo of one voxel
+    __shared__ float Nn[NF][2][3][NH * NW];  // n planes p - 1, p (ring)
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
     const int z1 = min(z0 + kZC, nz);
     const long long plane = (long long)nx * ny;
     const float scale = kStep ? scale_in[0] : 0.0f;
 
+    // This thread's entries of the f tile (with halo) and of the n tile, and
+    // their edge tests, are fixed across the planes of the march: set up once.
+    constexpr int kPerF = (FH * FW + kTvThreads - 1) / kTvThreads;
+    constexpr int kPerN = (NH * NW + kTvThreads - 1) / kTvThreads;
+    int fofs[kPerF];  // in-plane offset of the entry, or -1 outside the tile / volume
+#pragma unroll
+    for (int j = 0; j < kPerF; ++j) {
+        const int i = tid + j * kTvThreads;
+        const int ly = i / FW, lx = i - ly * FW;
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        fofs[j] = (i < FH * FW && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+    }
+    // n entries: bit 0 the voxel's column exists, bit 1 a y neighbour below
+    // (gy >= 1), bit 2 an x neighbour (gx >= 1); -1 outside the tile
+    int nflag[kPerN];
+#pragma unroll
+    for (int j = 0; j < kPerN; ++j) {
+        const int i = tid + j * kTvThreads;
+        const int ly = i / NW, lx = i - ly * NW;
+        const int gx = x0 + lx, gy = y0 + ly;
+        nflag[j] = i < NH * NW ? ((gx < nx && gy < ny) ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0) : -1;
+    }
+
     // A plane (with halo) is fetched into registers one step before it is
     // stored to LDS, so its loads are in flight during the previous plane's
     // arithmetic.
-    constexpr int kPer = (FH * FW + kTvThreads - 1) / kTvThreads;
-    float pre[NF][kPer];
+    float pre[NF][kPerF];
     auto fetch_plane = [&](int p) {
+        const bool pin = p >= 0 && p < nz;
+        const long long base = (long long)p * plane;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int i = tid + j * kTvThreads;
-            const int ly = i / FW, lx = i - ly * FW;
-            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-            const bool ok = i < FH * FW && p >= 0 && p < nz && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-            const long long idx = p * plane + (long long)gy * nx + gx;
-            pre[0][j] = ok ? f[idx] : 0.0f;
-            if (NF == 2) pre[NF - 1][j] = ok ? tp.prior[idx] : 0.0f;
+        for (int j = 0; j < kPerF; ++j) {
+            const bool ok = pin && fofs[j] >= 0;
+            pre[0][j] = ok ? f[base + fofs[j]] : 0.0f;
+            if (NF == 2) pre[NF - 1][j] = ok ? tp.prior[base + fofs[j]] : 0.0f;
         }
     };
     auto store_plane = [&](int slot) {
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
+        for (int j = 0; j < kPerF; ++j) {
             const int i = tid + j * kTvThreads;
             if (i >= FH * FW) continue;
-            const int ly = i / FW, lx = i - ly * FW;
-            F[0][slot][ly][lx] = pre[0][j];
-            if (NF == 2) F[NF - 1][slot][ly][lx] = pre[0][j] - pre[NF - 1][j];
+            F[0][slot][i] = pre[0][j];
+            if (NF == 2) F[NF - 1][slot][i] = pre[0][j] - pre[NF - 1][j];
         }
     };
     auto compute_n = [&](int p, int fs, int ps, int ns) {
-        for (int i = tid; i < NH * NW; i += kTvThreads) {
+        const bool pex = p >= 0 && (kPadZ ? p <= nz : p < nz);
+        const bool in = p < nz, has_below = p >= 1;
+#pragma unroll
+        for (int j = 0; j < kPerN; ++j) {
+            if (nflag[j] < 0) continue;
+            const int i = tid + j * kTvThreads;
             const int ly = i / NW, lx = i - ly * NW;
-            const int gx = x0 + lx, gy = y0 + ly;
-            const bool exists = gx < nx && gy < ny && p >= 0 && (kPadZ ? p <= nz : p < nz);
-            const bool in = p < nz;
+            const int ci = (ly + 1) * FW + lx + 1;  // the voxel in the f tile
 #pragma unroll
             for (int k = 0; k < NF; ++k) {
                 float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
-                if (exists) {
-                    const float c = in ? F[k][fs][ly + 1][lx + 1] : 0.0f;
-                    const float below = p >= 1 ? F[k][ps][ly + 1][lx + 1] : 0.0f;
-                    const float d0 = kPadZ ? c - below : (p >= 1 ? c - below : 0.0f);
-                    const float d1 = (gy >= 1 && in) ? c - F[k][fs][ly][lx + 1] : 0.0f;
-                    const float d2 = (gx >= 1 && in) ? c - F[k][fs][ly + 1][lx] : 0.0f;
+                if (pex && (nflag[j] & 1)) {
+                    const float c = in ? F[k][fs][ci] : 0.0f;
+                    const float below = has_below ? F[k][ps][ci] : 0.0f;
+                    const float d0 = kPadZ ? c - below : (has_below ? c - below : 0.0f);
+                    const float d1 = ((nflag[j] & 2) && in) ? c - F[k][fs][ci - FW] : 0.0f;
+                    const float d2 = ((nflag[j] & 4) && in) ? c - F[k][fs][ci - 1] : 0.0f;
                     float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
                     if (kKind == 1) {
                         const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
@@ -127,12 +149,23 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                     n1 = w1 * d1 * inv;
                     n2 = w2 * d2 * inv;
                 }
-                Nn[k][ns][0][ly][lx] = n0;
-                Nn[k][ns][1][ly][lx] = n1;
-                Nn[k][ns][2][ly][lx] = n2;
+                Nn[k][ns][0][i] = n0;
+                Nn[k][ns][1][i] = n1;
+                Nn[k][ns][2][i] = n2;
             }
         }
     };
+
+    // g: two rows of the tile per thread, column tid % 64
+    const int tx = tid & 63;
+    constexpr int kRows = kTY / (kTvThreads / 64);
+    int gofs[kRows];  // in-plane offset, or -1 outside the volume
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int ty = (tid >> 6) + r * (kTvThreads / 64);
+        const int gx = x0 + tx, gy = y0 + ty;
+        gofs[r] = (gx < nx && gy < ny) ? gy * nx + gx : -1;
+    }
 
     double acc = 0.0;
     fetch_plane(z0 - 1);
@@ -150,21 +183,21 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         __syncthreads();
         compute_n(p, fs, fs ^ 1, ns);
         __syncthreads();
-        // g at plane p - 1 for the tile: two rows per thread
-        const int tx = tid & 63;
-        for (int ty = tid >> 6; ty < kTY; ty += kTvThreads / 64) {
-            const int gx = x0 + tx, gy = y0 + ty;
-            if (gx >= nx || gy >= ny) continue;
-            const int no = ns ^ 1;
+        // g at plane p - 1
+        const int no = ns ^ 1;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (gofs[r] < 0) continue;
+            const int ty = (tid >> 6) + r * (kTvThreads / 64);
+            const int ni = ty * NW + tx;
             float g[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k)
-                g[k] = Nn[k][no][0][ty][tx] + Nn[k][no][1][ty][tx] + Nn[k][no][2][ty][tx] -
-                       Nn[k][no][2][ty][tx + 1] - Nn[k][no][1][ty + 1][tx] - Nn[k][ns][0][ty][tx];
+                g[k] = Nn[k][no][0][ni] + Nn[k][no][1][ni] + Nn[k][no][2][ni] - Nn[k][no][2][ni + 1] -
+                       Nn[k][no][1][ni + NW] - Nn[k][ns][0][ni];
             const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
             if (kStep) {
-                const long long idx = (long long)(p - 1) * plane + (long long)gy * nx + gx;
-                out[idx] = F[0][fs ^ 1][ty + 1][tx + 1] - gv * scale;
+                out[(long long)(p - 1) * plane + gofs[r]] = F[0][fs ^ 1][(ty + 1) * FW + tx + 1] - gv * scale;
             } else {
                 acc += (double)gv * (double)gv;
             }
@@ -193,7 +226,8 @@ template <int kKind>
 int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp, double norm_eps, long nx, long ny,
                    long nz, int maxiter, hipStream_t stream) {
     TIGRE_REQUIRE(nx > 0 && ny > 0 && nz > 0, "empty image");
-    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && ny < 65535L * kTY && nx < 65535L * kTX, "image too large");
+    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && nx * ny < (1LL << 31) && ny < 65535L * kTY && nx < 65535L * kTX,
+                  "image too large");
     const size_t n = (size_t)nx * ny * nz;
     if (maxiter <= 0) {
         if (dst != img) TIGRE_HIP_TRY(hipMemcpyAsync(dst, img, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
