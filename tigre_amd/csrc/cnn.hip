@@ -120,6 +120,121 @@ __global__ __launch_bounds__(256) void conv_kernel(const void* __restrict__ x, c
     }
 }
 
+// Convolution of 32-channel bf16 activations with the input staged in LDS.
+// conv_kernel above reads every input fragment from global memory once per
+// tap (27 times per voxel), which bounds it by the texture-address path.
+// Here a workgroup computes 4 output rows (one per wavefront) of 128 voxels
+// (8 MFMA tiles per wavefront, so each tap's weight fragments serve 16
+// MFMAs) and, for each of the three input planes z - 1, z, z + 1 in turn,
+// copies the 6 x 130 voxel block its taps read into LDS with direct
+// global-to-LDS loads (voxels outside the volume are copied from a zero
+// chunk).  The 16-byte channel groups of a voxel are stored in the order
+// q ^ ((x >> 2) & 3): the 16 lanes of one group read 16 consecutive voxels
+// and then fall on distinct LDS banks.  A fragment address is a per-lane
+// constant for each tile and dx, plus an immediate offset for dy.
+constexpr int kLdsTiles = 8;                    // 16-voxel tiles per wavefront
+constexpr int kLdsBX = 16 * kLdsTiles + 2;      // staged voxels along x (with the halo)
+constexpr int kLdsRows = 4 + 2;                 // staged rows: the workgroup's 4 and the halo
+constexpr int kLdsChunks = kLdsRows * kLdsBX * 4;  // 16-byte chunks of one staged plane
+
+__device__ uint4 g_zero_chunk[4];  // zero-initialised, never written: the source of the zero halo
+
+template <int kEpi>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void conv_lds_kernel(const unsigned short* __restrict__ x,
+                                                       const unsigned short* __restrict__ w,
+                                                       const float* __restrict__ bias, const void* __restrict__ aux,
+                                                       void* __restrict__ y, int N, int D, int H, int W) {
+    __shared__ uint4 plane[kLdsChunks];  // [row][voxel][channel group, swizzled]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
+    const int zz0 = (int)(blockIdx.z % (unsigned)D);
+    const long long n = blockIdx.z / (unsigned)D;
+    const int yo = y0 + wave;  // this wavefront's output row (computed but not stored when >= H)
+
+    // A-fragment byte offsets in the plane for tile 0, dx = d - 1 and dy = -1
+    // (row `wave` of the block).  Tile t adds 16 voxels (1024 bytes; the
+    // swizzle depends on (x >> 2) & 3, which 16 voxels leave unchanged) and
+    // dy + 1 rows kLdsBX * 64 bytes each: both are immediate offsets.
+    unsigned aoff[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int xi = m + d;
+        aoff[d] = (unsigned)(((wave * kLdsBX + xi) * 4 + (q ^ ((xi >> 2) & 3))) * 16);
+    }
+    f32x4 acc[kLdsTiles][2];
+#pragma unroll
+    for (int t = 0; t < kLdsTiles; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const char* lds = reinterpret_cast<const char*>(plane);
+
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zz = zz0 + dz;
+        if (zz < 0 || zz >= D) continue;  // uniform: the plane contributes nothing
+        __syncthreads();                  // the previous plane's reads are done
+        // chunk i of the plane: row i / (4 kLdsBX), voxel (i / 4) % kLdsBX, stored group i % 4
+        const long long zrow = (n * D + zz) * (long long)H;
+        for (int i0 = 0; i0 < kLdsChunks; i0 += 256) {
+            const int i = i0 + tid;
+            if (i < kLdsChunks) {
+                const int yi = i / (4 * kLdsBX), r = i - yi * 4 * kLdsBX, xi = r >> 2, sg = r & 3;
+                const int gy = y0 - 1 + yi, gx = x0 - 1 + xi;
+                const void* src = g_zero_chunk;
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    src = x + ((zrow + gy) * W + gx) * kC + 8 * (sg ^ ((xi >> 2) & 3));
+                __builtin_amdgcn_global_load_lds(src, &plane[i0 + wave * 64], 16, 0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int tap = (dz + 1) * 9 + dy * 3 + d;
+                const unsigned short* wt = w + (long long)tap * kC * kC;
+                const bf16x8 b0 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + m * kC + 8 * q));
+                const bf16x8 b1 =
+                    __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + (16 + m) * kC + 8 * q));
+#pragma unroll
+                for (int t = 0; t < kLdsTiles; ++t) {
+                    const bf16x8 a = __builtin_bit_cast(
+                        bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + dy * (kLdsBX * 64))));
+                    acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[t][0], 0, 0, 0);
+                    acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[t][1], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    if (yo >= H) return;
+    const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
+#pragma unroll
+    for (int t = 0; t < kLdsTiles; ++t) {
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            const int o = hb * 16 + m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xo = x0 + 16 * t + 4 * q + r;
+                if (xo >= W) continue;
+                const long long vox = obase + xo;
+                float v = acc[t][hb][r];
+                if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+                    v += bias[o];
+                    v = v > 0.0f ? v : 0.0f;
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                } else if (kEpi == TIGRE_CNN_EPI_RELU_MASK) {
+                    const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
+                    v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                } else {  // residual output: channel 0 only, FP32
+                    if (o == 0) static_cast<float*>(y)[vox] = static_cast<const float*>(aux)[vox] + (v + bias[0]);
+                }
+            }
+        }
+    }
+}
+
 // Weight gradient.  Block: one tap (blockIdx.x, 27 = bias) and one chunk of
 // x-row segments of 32 voxels (blockIdx.y); wavefront w owns the 16 x 16
 // block (input channels 16 (w & 1) .., output channels 16 (w >> 1) ..).
@@ -229,9 +344,16 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
     dim3 grid((unsigned)((W + 16 * kTilesPerWave - 1) / (16 * kTilesPerWave)), (unsigned)((rows + 3) / 4));
 #define TIGRE_CONV(I, E) hipLaunchKernelGGL((conv_kernel<I, E>), grid, dim3(256), 0, stream, x, w, bias, aux, y, N, D, H, W)
     if (x_mode == TIGRE_CNN_IN_BF16X32) {
-        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_BIAS_RELU);
-        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_RELU_MASK);
-        else TIGRE_CONV(TIGRE_CNN_IN_BF16X32, TIGRE_CNN_EPI_RESIDUAL);
+        TIGRE_REQUIRE((long long)N * D <= 65535, "too many planes for one launch");
+        const unsigned short* xa = static_cast<const unsigned short*>(x);
+        const dim3 lgrid((unsigned)((W + 16 * kLdsTiles - 1) / (16 * kLdsTiles)), (unsigned)((H + 3) / 4),
+                         (unsigned)(N * D));
+#define TIGRE_CONV_LDS(E) \
+    hipLaunchKernelGGL((conv_lds_kernel<E>), lgrid, dim3(256), 0, stream, xa, w, bias, aux, y, N, D, H, W)
+        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV_LDS(TIGRE_CNN_EPI_BIAS_RELU);
+        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV_LDS(TIGRE_CNN_EPI_RELU_MASK);
+        else TIGRE_CONV_LDS(TIGRE_CNN_EPI_RESIDUAL);
+#undef TIGRE_CONV_LDS
     } else {
         if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_BIAS_RELU);
         else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_RELU_MASK);
