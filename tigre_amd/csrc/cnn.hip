@@ -235,6 +235,98 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
 }
 
+// Convolution of a single FP32 channel (the image, or the loss gradient) into
+// 32 channels.  With one input channel the GEMM k dimension of conv_kernel
+// holds 1 useful channel of 32; here k is the tap instead: A[v][k] =
+// x[v + tap k] (k < 27, zero above), B[k][o] = w[k][o][0], so one MFMA covers
+// all taps of 16 voxels and 16 output channels.  The workgroup's 3 x 6 x 130
+// block of the input is staged in LDS (zero outside the volume) and every
+// lane gathers its 8 taps of each tile from there.
+template <int kEpi>
+__global__ __launch_bounds__(256) void conv_taps_kernel(const float* __restrict__ x,
+                                                        const unsigned short* __restrict__ w,
+                                                        const float* __restrict__ bias, const void* __restrict__ aux,
+                                                        void* __restrict__ y, int N, int D, int H, int W) {
+    constexpr int kPlane = kLdsRows * kLdsBX;
+    __shared__ float blk[3 * kPlane];  // [dz][row][voxel]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
+    const int zz0 = (int)(blockIdx.z % (unsigned)D);
+    const long long n = blockIdx.z / (unsigned)D;
+    const int yo = y0 + wave;
+
+    for (int i = tid; i < 3 * kPlane; i += 256) {
+        const int pz = i / kPlane, r = i - pz * kPlane, yi = r / kLdsBX, xi = r - yi * kLdsBX;
+        const int gz = zz0 - 1 + pz, gy = y0 - 1 + yi, gx = x0 - 1 + xi;
+        float v = 0.0f;
+        if (gz >= 0 && gz < D && gy >= 0 && gy < H && gx >= 0 && gx < W)
+            v = x[((n * D + gz) * H + gy) * (long long)W + gx];
+        blk[i] = v;
+    }
+    // B fragments: k = 8 q + j is the tap, column o = 16 hb + m
+    bf16x8 b[2];
+#pragma unroll
+    for (int hb = 0; hb < 2; ++hb) {
+        unsigned short bits[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int tap = 8 * q + j;
+            bits[j] = tap < 27 ? w[((long long)tap * kC + 16 * hb + m) * kC] : (unsigned short)0;
+        }
+        const uint4 raw = make_uint4(bits[0] | ((unsigned)bits[1] << 16), bits[2] | ((unsigned)bits[3] << 16),
+                                     bits[4] | ((unsigned)bits[5] << 16), bits[6] | ((unsigned)bits[7] << 16));
+        b[hb] = __builtin_bit_cast(bf16x8, raw);
+    }
+    // this lane's taps: offsets in the block relative to (row wave, voxel m) of tile 0
+    int toff[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int tap = 8 * q + j;
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;  // each + 1
+        toff[j] = tap < 27 ? (dz * kLdsRows + wave + dy) * kLdsBX + m + dx : -1;
+    }
+    __syncthreads();
+
+    f32x4 acc[kLdsTiles][2];
+#pragma unroll
+    for (int t = 0; t < kLdsTiles; ++t) {
+        unsigned short bits[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bits[j] = toff[j] >= 0 ? bf16_bits(blk[toff[j] + 16 * t]) : (unsigned short)0;
+        const uint4 raw = make_uint4(bits[0] | ((unsigned)bits[1] << 16), bits[2] | ((unsigned)bits[3] << 16),
+                                     bits[4] | ((unsigned)bits[5] << 16), bits[6] | ((unsigned)bits[7] << 16));
+        const bf16x8 a = __builtin_bit_cast(bf16x8, raw);
+        acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[1], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+
+    if (yo >= H) return;
+    const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
+#pragma unroll
+    for (int t = 0; t < kLdsTiles; ++t) {
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            const int o = hb * 16 + m;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xo = x0 + 16 * t + 4 * q + r;
+                if (xo >= W) continue;
+                const long long vox = obase + xo;
+                float v = acc[t][hb][r];
+                if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+                    v += bias[o];
+                    v = v > 0.0f ? v : 0.0f;
+                } else {
+                    const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
+                    v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+                }
+                static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+            }
+        }
+    }
+}
+
 // Weight gradient.  Block: one tap (blockIdx.x, 27 = bias) and one chunk of
 // x-row segments of 32 voxels (blockIdx.y); wavefront w owns the 16 x 16
 // block (input channels 16 (w & 1) .., output channels 16 (w >> 1) ..).
@@ -355,8 +447,16 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
         else TIGRE_CONV_LDS(TIGRE_CNN_EPI_RESIDUAL);
 #undef TIGRE_CONV_LDS
     } else {
-        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_BIAS_RELU);
-        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_RELU_MASK);
+        TIGRE_REQUIRE((long long)N * D <= 65535, "too many planes for one launch");
+        const float* xf = static_cast<const float*>(x);
+        const dim3 lgrid((unsigned)((W + 16 * kLdsTiles - 1) / (16 * kLdsTiles)), (unsigned)((H + 3) / 4),
+                         (unsigned)(N * D));
+        if (epi == TIGRE_CNN_EPI_BIAS_RELU)
+            hipLaunchKernelGGL((conv_taps_kernel<TIGRE_CNN_EPI_BIAS_RELU>), lgrid, dim3(256), 0, stream, xf, w, bias,
+                               aux, y, N, D, H, W);
+        else if (epi == TIGRE_CNN_EPI_RELU_MASK)
+            hipLaunchKernelGGL((conv_taps_kernel<TIGRE_CNN_EPI_RELU_MASK>), lgrid, dim3(256), 0, stream, xf, w, bias,
+                               aux, y, N, D, H, W);
         else TIGRE_CONV(TIGRE_CNN_IN_F32X1, TIGRE_CNN_EPI_RESIDUAL);
     }
 #undef TIGRE_CONV
