@@ -208,7 +208,8 @@ FDK = fdk
 # SART family (iterative_recon_alg.py:131-401, art_family_algorithms.py)
 # ---------------------------------------------------------------------------
 class _SartState:
-    def __init__(self, proj, geo, angles, blocksize, order=None, init=None, lmbda=1.0, noneg=True, verbose=False):
+    def __init__(self, proj, geo, angles, blocksize, order=None, init=None, lmbda=1.0, noneg=True, verbose=False,
+                 use_plans=True):
         torch = _torch()
         self.geo = copy.deepcopy(geo)
         self.geo.check_geo(angles)
@@ -231,6 +232,21 @@ class _SartState:
             self.res = _to_device(init).reshape(shape).clone()
         self.proj_blocks = [self.proj[torch.as_tensor(i, device="cuda")].contiguous() for i in self.blocks]
         self.W_blocks = [self.W[torch.as_tensor(i, device="cuda")].contiguous() for i in self.blocks]
+        # Cone beam: one projection plan per subset (records on the device once)
+        # and the fused residual / update kernels, which keep the rounding
+        # steps of the tensor expressions of the plain loop below (same bits)
+        # and make a subset a handful of launches with no host set-up.
+        self.plans = None
+        if use_plans and getattr(self.geo, "mode", "cone") == "cone" and self.res.dtype == torch.float32:
+            from ..plan import Plan
+
+            self.plans = [Plan(_sub_geometry(self.geo, idx), self.angles[idx]) for idx in self.blocks]
+            self.ws = torch.empty(max(max(p.workspace_bytes() for p in self.plans), 256), dtype=torch.uint8,
+                                  device="cuda")
+            nV, nU = self.plans[0].proj_shape[1:]
+            self.axbuf = torch.empty((max(len(b) for b in self.blocks), nV, nU), dtype=torch.float32,
+                                     device="cuda")
+            self._inv_v = (None, None)  # (lambda, [lambda / V_j])
 
     def _weights_w(self):
         """W = 1 / Ax(ones(2^3)) on an enlarged volume (iterative_recon_alg.py:214-230)."""
@@ -265,6 +281,17 @@ class _SartState:
     def sweep(self):
         """One pass over all subsets (art_data_minimizing, iterative_recon_alg.py:326-345)."""
         torch = _torch()
+        if self.plans is not None:
+            from ..plan import sart_residual
+
+            if self._inv_v[0] != self.lmbda:
+                self._inv_v = (self.lmbda, [(self.lmbda / self.V[j]).contiguous() for j in range(len(self.blocks))])
+            for j, p in enumerate(self.plans):
+                ax = self.axbuf[: p.nviews]
+                p.ax(self.res, ax, self.ws, _native.PROJ_SIDDON)
+                sart_residual(self.proj_blocks[j], self.W_blocks[j], ax)
+                p.sart_update(ax, self.res, self._inv_v[1][j], self.ws, self.noneg, _native.BACKPROJ_FDK)
+            return
         for j in range(len(self.blocks)):
             r = self.W_blocks[j] * (self.proj_blocks[j] - self.views.ax(self.res, j))
             self.res += self.lmbda / self.V[j] * self.views.atb(r, j)

@@ -29,7 +29,7 @@ from . import _SartState, _sub_geometry, _torch
 class _Case:
     def __init__(self, proj, geo, angles, blocksize, lmbda, lmbda_red, niter, noneg, order, tuner, params):
         torch = _torch()
-        st = _SartState(proj, geo, angles, blocksize, order=order, lmbda=lmbda, noneg=noneg)
+        st = _SartState(proj, geo, angles, blocksize, order=order, lmbda=lmbda, noneg=noneg, use_plans=False)
         self.blocks = st.blocks
         self.proj_blocks = st.proj_blocks
         self.W_blocks = st.W_blocks
