@@ -18,14 +18,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def cfg1_geometry():
+def small_cone_geometry():
     # DSO 1000 / DSD 1536 mm, 128^3 at 1 mm, 128^2 detector at 1.6 mm
     return cone_geometry((128, 128, 128), (128.0, 128.0, 128.0), (128, 128), (1.6, 1.6), 1000.0, 1536.0)
 
 
 @pytest.fixture(scope="module")
-def cfg1():
-    geo = cfg1_geometry()
+def small_cone():
+    geo = small_cone_geometry()
     angles = np.linspace(0, 2 * np.pi, 128, endpoint=False)
     img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 10, seed=20261203)
     return geo, angles, img, prepared(geo, angles)
@@ -47,37 +47,37 @@ def tilted_geometry(n=40, nviews=24, seed=5):
 
 
 @pytest.mark.parametrize("ptype", ["Siddon", "interpolated"])
-def test_ax_cfg1(cuda, cfg1, ptype):
+def test_ax_small_cone(cuda, small_cone, ptype):
     import torch
 
-    geo, angles, img, gp = cfg1
+    geo, angles, img, gp = small_cone
     got = tigre_amd.Ax(torch.from_numpy(img).to(cuda), geo, angles, ptype)
     torch.cuda.synchronize()
     ref = oracle.ax(img, gp, ptype)
     err = oracle.rel_l2(got.cpu().numpy(), ref)
-    print("Ax %s cfg1 rel-L2 %.3e" % (ptype, err))
+    print("Ax %s small_cone rel-L2 %.3e" % (ptype, err))
     assert err <= TOL
 
 
 @pytest.mark.parametrize("btype", ["FDK", "matched"])
-def test_atb_cfg1(cuda, cfg1, btype):
+def test_atb_small_cone(cuda, small_cone, btype):
     import torch
 
-    geo, angles, img, gp = cfg1
+    geo, angles, img, gp = small_cone
     proj = oracle.ax(img, gp).astype(np.float32)
     got = tigre_amd.Atb(torch.from_numpy(proj).to(cuda), geo, angles, btype)
     torch.cuda.synchronize()
     ref = oracle.atb(proj, gp, btype)
     err = oracle.rel_l2(got.cpu().numpy(), ref)
-    print("Atb %s cfg1 rel-L2 %.3e" % (btype, err))
+    print("Atb %s small_cone rel-L2 %.3e" % (btype, err))
     assert err <= TOL
 
 
-def test_host_contract_matches_device(cuda, cfg1):
+def test_host_contract_matches_device(cuda, small_cone):
     """NumPy in/out (reference contract) equals the resident device path bit for bit."""
     import torch
 
-    geo, angles, img, gp = cfg1
+    geo, angles, img, gp = small_cone
     sub = angles[:16]
     a = tigre_amd.Ax(img, geo, sub)
     b = tigre_amd.Ax(torch.from_numpy(img).to(cuda), geo, sub).cpu().numpy()
@@ -114,7 +114,7 @@ def test_atb_tilted(cuda, btype):
 
 def _siddon_case(name):
     """Geometries that drive each Siddon code path: the beam-staged cell walk and
-    crossing walk (cfg1 views away from / near 45 degrees), a slab box too large
+    crossing walk (small_cone views away from / near 45 degrees), a slab box too large
     for LDS (coarse detector: walk from global memory), and views whose rays
     barely advance along any horizontal axis (wide fan: gather kernel)."""
     if name == "coarse":
@@ -140,15 +140,15 @@ def test_siddon_paths(cuda, name):
     assert err <= TOL and worst <= 1e-4
 
 
-def test_siddon_cfg1_max_error(cuda, cfg1):
+def test_siddon_small_cone_max_error(cuda, small_cone):
     """Every ray, not only the L2 aggregate: a wrong voxel on a few rays shows here."""
     import torch
 
-    geo, angles, img, gp = cfg1
+    geo, angles, img, gp = small_cone
     got = tigre_amd.Ax(torch.from_numpy(img).to(cuda), geo, angles, "Siddon").cpu().numpy()
     ref = oracle.ax(img, gp, "Siddon")
     worst = float(np.max(np.abs(got - ref))) / float(np.max(np.abs(ref)))
-    print("Ax Siddon cfg1 max-rel %.3e" % worst)
+    print("Ax Siddon small_cone max-rel %.3e" % worst)
     assert worst <= 1e-4
 
 
