@@ -194,10 +194,13 @@ def test_matlab_layout(cuda, kind):
         assert oracle.rel_l2(out, ref) <= TOL
 
 
-def test_minTV(cuda):
+@pytest.mark.parametrize("shape", [(40, 48, 56), (19, 13, 130)])
+def test_minTV(cuda, shape):
+    """minTV against the oracle; the second shape has several x tiles with a
+    ragged last one, a ragged y tile and a 3-plane z chunk."""
     import torch
 
-    img = ellipsoid_phantom((40, 48, 56), (40.0, 48.0, 56.0), 6, seed=9)
+    img = ellipsoid_phantom(shape, tuple(float(v) for v in shape), 6, seed=9)
     img += np.random.default_rng(1).normal(0, 0.05, img.shape).astype(np.float32)
     got = tigre_amd.minTV(torch.from_numpy(img).to(cuda), 0.02, 20).cpu().numpy()
     err = oracle.rel_l2(got, oracle.pocs_tv(img, 0.02, 20))

@@ -14,9 +14,10 @@
 //           every run) into alpha / sqrt(sum + norm_eps);
 //   pass 2  f' = f - scale * g, recomputing g from f.
 // Each pass is a 64 x 8 column tile that marches along z through a chunk of
-// planes: the f planes (with a one-voxel halo) and two planes of n live in
-// LDS, so f is read from HBM about once per pass: 12 bytes of traffic per
-// voxel and iteration, no host round trip for the norm.
+// planes, every lane one column with its z neighbours in registers and its
+// x / y neighbours through LDS, so f is read from HBM about once per pass:
+// 12 bytes of traffic per voxel and iteration, no host round trip for the
+// norm.
 #include <cmath>
 
 #include "common.hpp"
@@ -24,8 +25,12 @@
 namespace tigre {
 
 namespace {
-constexpr int kTX = 64, kTY = 8, kZC = 16;  // tile columns, rows, planes per chunk
-constexpr int kTvThreads = 256;
+constexpr int kTX = 64, kTY = 8, kZC = 16;  // tile columns, rows (one wavefront each), planes per chunk
+constexpr int kAhead = 4;                   // planes of f in flight per lane
+constexpr int kTvThreads = kTX * kTY;
+constexpr int kFW = kTX + 2, kFH = kTY + 2;  // f plane with a one-voxel halo
+constexpr int kNW = kTX + 1, kNH = kTY + 1;  // n plane: the tile plus the column x0 + 64 and row y0 + 4
+constexpr int kFHalo = 2 * kFW + 2 * kTY;    // halo entries of an f plane
 
 struct TvParams {
     const float* prior;  // PICCS prior image
@@ -46,6 +51,70 @@ __device__ inline double block_sum(double v) {
     __syncthreads();
     return s;  // valid in thread 0
 }
+
+// LDS index in the f plane of halo entry e: row -1 (x0 - 1 .. x0 + 64), row
+// kTY, then the columns x0 - 1 and x0 + 64 of rows 0 .. kTY - 1.
+__device__ __forceinline__ void f_halo_pos(int e, int& lx, int& ly) {
+    if (e < kFW) {
+        lx = e - 1;
+        ly = -1;
+    } else if (e < 2 * kFW) {
+        lx = e - kFW - 1;
+        ly = kTY;
+    } else {
+        const int k = e - 2 * kFW;
+        lx = k < kTY ? -1 : kTX;
+        ly = k < kTY ? k : k - kTY;
+    }
+}
+// Halo n entry of lane (tx, ty), or false: row kTY (x0 .. x0 + 63) belongs to
+// the last wavefront, column kTX (rows 0 .. kTY - 1) to the first kTY lanes
+// of the one before (two wavefronts do a second n evaluation per plane).
+__device__ __forceinline__ bool n_halo_pos(int tx, int ty, int& lx, int& ly) {
+    if (ty == kTY - 1) {
+        lx = tx;
+        ly = kTY;
+        return true;
+    }
+    if (ty == kTY - 2 && tx < kTY) {
+        lx = kTX;
+        ly = tx;
+        return true;
+    }
+    return false;
+}
+
+// The normalised difference field at one voxel from its value c, the value
+// below it (z - 1) and its neighbours at x - 1 and y - 1 (GD_TV.cu:104-140:
+// a missing neighbour's difference is 0; `flags` bit 0 the column exists,
+// bit 1 a y neighbour, bit 2 an x neighbour).
+template <int kKind>
+__device__ __forceinline__ void n_field(float c, float below, float cy, float cx, int flags, bool pex, bool in,
+                                        bool has_below, const TvParams& tp, float& n0, float& n1, float& n2) {
+    constexpr bool kPadZ = kKind != 2;
+    constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
+    n0 = n1 = n2 = 0.0f;
+    if (pex && (flags & 1)) {
+        const float cc = in ? c : 0.0f;
+        const float bb = has_below ? below : 0.0f;
+        const float d0 = kPadZ ? cc - bb : (has_below ? cc - bb : 0.0f);
+        const float d1 = ((flags & 2) && in) ? cc - cy : 0.0f;
+        const float d2 = ((flags & 4) && in) ? cc - cx : 0.0f;
+        float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
+        if (kKind == 1) {
+            const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
+            w0 = __expf(-a0 * a0);
+            w1 = __expf(-a1 * a1);
+            w2 = __expf(-a2 * a2);
+        }
+        // hardware square root and reciprocal (1 ulp each)
+        const float inv =
+            __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(w0 * d0 * d0 + w1 * d1 * d1 + w2 * d2 * d2) + eps);
+        n0 = w0 * d0 * inv;
+        n1 = w1 * d1 * inv;
+        n2 = w2 * d2 * inv;
+    }
+}
 }  // namespace
 
 // kKind 0: TV (GD_TV; the image sits between zero slices, so z always has a
@@ -54,6 +123,17 @@ __device__ inline double block_sum(double v) {
 // term dropped at the edges like x and y (PICCS.cu:109-157,322-347).  At the
 // low x / y edge the difference is 0; beyond the high edge n is 0
 // (GD_TV.cu:104-114).  kStep false: pass 1 (partials); true: pass 2.
+//
+// A workgroup is a 64 x 8 column tile, one wavefront per row, marching along
+// z through a chunk of planes; every lane owns one column.  Its f and n values
+// of the previous plane stay in registers (the z neighbours), the f plane goes
+// through LDS for the x - 1 / y - 1 neighbours and the n_x / n_y planes for
+// the x + 1 / y + 1 ones.  f and n planes are double-buffered so one barrier
+// per plane orders everything: plane p's f is written before it, read after
+// it; its n is written after it and read after the next plane's barrier.
+// The first lanes also own the halo entries of the f plane (next plane's value
+// prefetched in a register), and the lanes named by n_halo_pos those of the n
+// plane (their f of the previous plane kept in a register).
 template <int kKind, bool kStep>
 __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __restrict__ f, float* __restrict__ out,
                                                              TvParams tp, int nx, int ny, int nz,
@@ -61,148 +141,143 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                                                              const float* __restrict__ scale_in) {
     constexpr int NF = kKind == 2 ? 2 : 1;
     constexpr bool kPadZ = kKind != 2;
-    constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
-    constexpr int FW = kTX + 2, FH = kTY + 2, NW = kTX + 1, NH = kTY + 1;
-    __shared__ float F[NF][2][FH * FW];      // f planes p - 1, p (ring), halo of one voxel
-    __shared__ float Nn[NF][2][3][NH * NW];  // n planes p - 1, p (ring)
-    const int tid = threadIdx.x;
+    // f planes (ring of two) with a halo of one voxel, then one slot per lane
+    // that lanes without a halo entry store their halo value to
+    __shared__ float F[2][NF][kFH * kFW + kTvThreads];
+    __shared__ float Nn[2][NF][2][kNH * kNW];     // n_y (0) and n_x (1) planes (ring of two)
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
     const int z1 = min(z0 + kZC, nz);
     const long long plane = (long long)nx * ny;
     const float scale = kStep ? scale_in[0] : 0.0f;
 
-    // This thread's entries of the f tile (with halo) and of the n tile, and
-    // their edge tests, are fixed across the planes of the march: set up once.
-    constexpr int kPerF = (FH * FW + kTvThreads - 1) / kTvThreads;
-    constexpr int kPerN = (NH * NW + kTvThreads - 1) / kTvThreads;
-    int fofs[kPerF];  // in-plane offset of the entry, or -1 outside the tile / volume
-#pragma unroll
-    for (int j = 0; j < kPerF; ++j) {
-        const int i = tid + j * kTvThreads;
-        const int ly = i / FW, lx = i - ly * FW;
-        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        fofs[j] = (i < FH * FW && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+    // own column
+    const int gx = x0 + tx, gy = y0 + ty;
+    const bool col = gx < nx && gy < ny;
+    const int own_ofs = col ? gy * nx + gx : -1;
+    const int own_f = (ty + 1) * kFW + tx + 1, own_n = ty * kNW + tx;
+    const int own_flags = (col ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0);
+    // halo entry of the f plane (lanes tid < kFHalo)
+    int lx, ly;
+    int hf_idx = kFH * kFW + tid, hf_ofs = -1;
+    if (tid < kFHalo) {
+        f_halo_pos(tid, lx, ly);
+        hf_idx = (ly + 1) * kFW + lx + 1;
+        const int hx = x0 + lx, hy = y0 + ly;
+        hf_ofs = (hx >= 0 && hx < nx && hy >= 0 && hy < ny) ? hy * nx + hx : -1;
     }
-    // n entries: bit 0 the voxel's column exists, bit 1 a y neighbour below
-    // (gy >= 1), bit 2 an x neighbour (gx >= 1); -1 outside the tile
-    int nflag[kPerN];
-#pragma unroll
-    for (int j = 0; j < kPerN; ++j) {
-        const int i = tid + j * kTvThreads;
-        const int ly = i / NW, lx = i - ly * NW;
-        const int gx = x0 + lx, gy = y0 + ly;
-        nflag[j] = i < NH * NW ? ((gx < nx && gy < ny) ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0) : -1;
+    // halo entry of the n plane
+    int hn_f = -1, hn_n = -1, hn_ofs = -1, hn_flags = 0;
+    if (n_halo_pos(tx, ty, lx, ly)) {
+        hn_f = (ly + 1) * kFW + lx + 1;
+        hn_n = ly * kNW + lx;
+        const int hx = x0 + lx, hy = y0 + ly;
+        const bool hcol = hx < nx && hy < ny;
+        hn_ofs = hcol ? hy * nx + hx : -1;
+        hn_flags = (hcol ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
     }
 
-    // A plane (with halo) is fetched into registers one step before it is
-    // stored to LDS, so its loads are in flight during the previous plane's
-    // arithmetic.
-    float pre[NF][kPerF];
-    auto fetch_plane = [&](int p) {
-        const bool pin = p >= 0 && p < nz;
+    auto load = [&](int k, long long idx) -> float {
+        return k == 0 ? f[idx] : f[idx] - tp.prior[idx];
+    };
+    // The image (and prior) of plane p at this lane's column and f-halo
+    // entry, fetched kAhead planes before its use into a ring of registers
+    // (one plane in flight per lane would leave HBM latency-bound).  The
+    // loads are unconditional, from a clamped in-bounds address, and raw:
+    // the masks and the PICCS difference are applied when the slot is used,
+    // so nothing has to wait for a load before then.
+    const float* src[2] = {f, tp.prior};
+    const int own_ld = max(own_ofs, 0), hf_ld = max(hf_ofs, 0);
+    float ring[kAhead][NF], hring[kAhead][NF];
+    auto fetch = [&](int p, float* v, float* hv) {
+        const long long base = (long long)min(max(p, 0), nz - 1) * plane;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            v[k] = src[k][base + own_ld];
+            hv[k] = src[k][base + hf_ld];
+        }
+    };
+    // field k of a fetched slot: 0 outside the volume, f - prior for PICCS
+    auto field = [&](const float* raw, int k, bool ok) -> float {
+        return ok ? (k == 0 ? raw[0] : raw[0] - raw[1]) : 0.0f;
+    };
+
+    float fprev[NF], hprev[NF];  // f of the previous plane: own column, n-halo column
+    {
+        const int p = z0 - 1;
+        const bool pin = p >= 0;
         const long long base = (long long)p * plane;
 #pragma unroll
-        for (int j = 0; j < kPerF; ++j) {
-            const bool ok = pin && fofs[j] >= 0;
-            pre[0][j] = ok ? f[base + fofs[j]] : 0.0f;
-            if (NF == 2) pre[NF - 1][j] = ok ? tp.prior[base + fofs[j]] : 0.0f;
+        for (int k = 0; k < NF; ++k) {
+            fprev[k] = (pin && own_ofs >= 0) ? load(k, base + own_ofs) : 0.0f;
+            hprev[k] = (pin && hn_ofs >= 0) ? load(k, base + hn_ofs) : 0.0f;
         }
-    };
-    auto store_plane = [&](int slot) {
+    }
 #pragma unroll
-        for (int j = 0; j < kPerF; ++j) {
-            const int i = tid + j * kTvThreads;
-            if (i >= FH * FW) continue;
-            F[0][slot][i] = pre[0][j];
-            if (NF == 2) F[NF - 1][slot][i] = pre[0][j] - pre[NF - 1][j];
+    for (int j = 0; j < kAhead; ++j) fetch(min(z0 + j, z1), ring[j], hring[j]);
+    float nprev[NF][3];  // n of the previous plane at the own column (z, y, x)
+    double acc = 0.0;
+    // fully unrolled (planes z0 .. z0 + kZC, at most): ring slot j % kAhead is
+    // a fixed register set and the compiler sees every load it must wait for
+#pragma unroll
+    for (int jj = 0; jj <= kZC; ++jj) {
+        const int j = jj % kAhead;
+        const int p = z0 + jj;
+        if (p > z1) break;
+        const int b = p & 1;
+        const float fpp = fprev[0];  // f at plane p - 1 (the image itself, also for PICCS)
+        const bool pin = p >= 0 && p < nz;
+        float fc[NF];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            fc[k] = field(ring[j], k, pin && own_ofs >= 0);
+            F[b][k][own_f] = fc[k];
+            // unconditional: a store under a branch would pull the load into it
+            F[b][k][hf_idx] = field(hring[j], k, pin && hf_ofs >= 0);
         }
-    };
-    auto compute_n = [&](int p, int fs, int ps, int ns) {
+        // unconditional (a branch around the loads would make the compiler
+        // wait for all of them); past the chunk it re-reads plane z1 from cache
+        fetch(min(p + kAhead, z1), ring[j], hring[j]);
+        __syncthreads();
         const bool pex = p >= 0 && (kPadZ ? p <= nz : p < nz);
         const bool in = p < nz, has_below = p >= 1;
+        float ncur[NF][3];
 #pragma unroll
-        for (int j = 0; j < kPerN; ++j) {
-            if (nflag[j] < 0) continue;
-            const int i = tid + j * kTvThreads;
-            const int ly = i / NW, lx = i - ly * NW;
-            const int ci = (ly + 1) * FW + lx + 1;  // the voxel in the f tile
-#pragma unroll
-            for (int k = 0; k < NF; ++k) {
-                float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
-                if (pex && (nflag[j] & 1)) {
-                    const float c = in ? F[k][fs][ci] : 0.0f;
-                    const float below = has_below ? F[k][ps][ci] : 0.0f;
-                    const float d0 = kPadZ ? c - below : (has_below ? c - below : 0.0f);
-                    const float d1 = ((nflag[j] & 2) && in) ? c - F[k][fs][ci - FW] : 0.0f;
-                    const float d2 = ((nflag[j] & 4) && in) ? c - F[k][fs][ci - 1] : 0.0f;
-                    float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
-                    if (kKind == 1) {
-                        const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
-                        w0 = __expf(-a0 * a0);
-                        w1 = __expf(-a1 * a1);
-                        w2 = __expf(-a2 * a2);
-                    }
-                    // hardware square root and reciprocal (1 ulp each)
-                    const float inv = __builtin_amdgcn_rcpf(
-                        __builtin_amdgcn_sqrtf(w0 * d0 * d0 + w1 * d1 * d1 + w2 * d2 * d2) + eps);
-                    n0 = w0 * d0 * inv;
-                    n1 = w1 * d1 * inv;
-                    n2 = w2 * d2 * inv;
-                }
-                Nn[k][ns][0][i] = n0;
-                Nn[k][ns][1][i] = n1;
-                Nn[k][ns][2][i] = n2;
+        for (int k = 0; k < NF; ++k) {
+            n_field<kKind>(fc[k], fprev[k], F[b][k][own_f - kFW], F[b][k][own_f - 1], own_flags, pex, in, has_below,
+                           tp, ncur[k][0], ncur[k][1], ncur[k][2]);
+            Nn[b][k][0][own_n] = ncur[k][1];
+            Nn[b][k][1][own_n] = ncur[k][2];
+            fprev[k] = fc[k];
+            if (hn_n >= 0) {
+                const float hc = F[b][k][hn_f];
+                float m0, m1, m2;
+                n_field<kKind>(hc, hprev[k], F[b][k][hn_f - kFW], F[b][k][hn_f - 1], hn_flags, pex, in, has_below,
+                               tp, m0, m1, m2);
+                Nn[b][k][0][hn_n] = m1;
+                Nn[b][k][1][hn_n] = m2;
+                hprev[k] = hc;
             }
         }
-    };
-
-    // g: two rows of the tile per thread, column tid % 64
-    const int tx = tid & 63;
-    constexpr int kRows = kTY / (kTvThreads / 64);
-    int gofs[kRows];  // in-plane offset, or -1 outside the volume
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-        const int ty = (tid >> 6) + r * (kTvThreads / 64);
-        const int gx = x0 + tx, gy = y0 + ty;
-        gofs[r] = (gx < nx && gy < ny) ? gy * nx + gx : -1;
-    }
-
-    double acc = 0.0;
-    fetch_plane(z0 - 1);
-    store_plane(0);
-    fetch_plane(z0);
-    store_plane(1);
-    fetch_plane(z0 + 1);
-    __syncthreads();
-    compute_n(z0, 1, 0, 0);
-    __syncthreads();
-    for (int p = z0 + 1; p <= z1; ++p) {
-        const int fs = (p - z0 + 1) & 1, ns = (p - z0) & 1;  // slots of plane p
-        store_plane(fs);
-        if (p < z1) fetch_plane(p + 1);
-        __syncthreads();
-        compute_n(p, fs, fs ^ 1, ns);
-        __syncthreads();
-        // g at plane p - 1
-        const int no = ns ^ 1;
-#pragma unroll
-        for (int r = 0; r < kRows; ++r) {
-            if (gofs[r] < 0) continue;
-            const int ty = (tid >> 6) + r * (kTvThreads / 64);
-            const int ni = ty * NW + tx;
+        // g at plane p - 1 from the n planes of p - 1 (buffer b ^ 1, complete
+        // since this plane's barrier) and n_z of plane p
+        if (p > z0 && col) {
             float g[NF];
 #pragma unroll
             for (int k = 0; k < NF; ++k)
-                g[k] = Nn[k][no][0][ni] + Nn[k][no][1][ni] + Nn[k][no][2][ni] - Nn[k][no][2][ni + 1] -
-                       Nn[k][no][1][ni + NW] - Nn[k][ns][0][ni];
+                g[k] = nprev[k][0] + nprev[k][1] + nprev[k][2] - Nn[b ^ 1][k][1][own_n + 1] -
+                       Nn[b ^ 1][k][0][own_n + kNW] - ncur[k][0];
             const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
             if (kStep) {
-                out[(long long)(p - 1) * plane + gofs[r]] = F[0][fs ^ 1][(ty + 1) * FW + tx + 1] - gv * scale;
+                out[(long long)(p - 1) * plane + own_ofs] = fpp - gv * scale;
             } else {
                 acc += (double)gv * (double)gv;
             }
         }
-        __syncthreads();  // the next plane overwrites slot fs ^ 1 and n slot ns ^ 1
+#pragma unroll
+        for (int k = 0; k < NF; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nprev[k][c] = ncur[k][c];
     }
     if (!kStep) {
         acc = block_sum(acc);
