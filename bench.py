@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=64,
                     help="views in the CPU-baseline sample, indices floor(k * views / cpu_views) (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
+    ap.add_argument("--ossart-iters", type=int, default=20,
+                    help="pair: OS-SART iterations (20-view subsets, lambda 1) after the timed region (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size once (counter calibration)")
     ap.add_argument("--workload", choices=["pair", "small", "batched", "learned", "precision"], default="pair",
@@ -176,6 +178,25 @@ def main():
     log("Ax interpolated %.2f ms (kernel %.2f)  Atb matched %.2f ms (kernel %.2f)" % (
         interp_ms, k_interp, matched_ms, k_matched))
 
+    ossart = None
+    if args.ossart_iters > 0:
+        # the reconstruction loop over the same data: OS-SART, 20-view subsets, lambda 1, from zero;
+        # the wall-clock includes its set-up (subset plans, W and V weights)
+        from tigre_amd import algorithms as algs
+
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rec = algs.ossart(proj, geo, angles, args.ossart_iters, blocksize=20, lmbda=1.0, return_tensor=True)
+        torch.cuda.synchronize()
+        t_os = time.perf_counter() - t
+        rmse = float(torch.sqrt(torch.mean((rec.double() - img.double()) ** 2)))
+        ossart = {"iterations": args.ossart_iters, "blocksize": 20, "lambda": 1.0, "seconds": round(t_os, 3),
+                  "ms_per_iteration": round(t_os * 1e3 / args.ossart_iters, 2),
+                  "gupdates_per_s": round(2.0 * V * nviews * args.ossart_iters / t_os / 1e9, 1),
+                  "rmse_vs_phantom": rmse}
+        log("OS-SART %d iterations: %.3f s, image RMSE %.4g" % (args.ossart_iters, t_os, rmse))
+        del rec
+
     if args.calib_copy_mib > 0:
         # one streaming elementwise kernel of known size (reads and writes calib_copy_mib MiB each),
         # for scaling the FETCH_SIZE / WRITE_SIZE counters
@@ -221,6 +242,8 @@ def main():
                       "ax_interpolated": round(k_interp, 3), "atb_matched": round(k_matched, 3)},
         "pair_ms_model_60pct": round((b_ax + b_atb) / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 2),
     }
+    if ossart is not None:
+        result["ossart"] = ossart
 
     if args.cpu_views > 0:
         import oracle
