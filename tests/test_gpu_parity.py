@@ -194,10 +194,11 @@ def test_matlab_layout(cuda, kind):
         assert oracle.rel_l2(out, ref) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(40, 48, 56), (19, 13, 130)])
+@pytest.mark.parametrize("shape", [(40, 48, 56), (19, 13, 130), (2, 17, 67)])
 def test_minTV(cuda, shape):
     """minTV against the oracle; the second shape has several x tiles with a
-    ragged last one, a ragged y tile and a 3-plane z chunk."""
+    ragged last one, a ragged y tile and a 3-plane z chunk, the third fewer
+    planes than the kernel prefetches."""
     import torch
 
     img = ellipsoid_phantom(shape, tuple(float(v) for v in shape), 6, seed=9)
@@ -233,11 +234,12 @@ def test_aw_minTV(cuda):
     assert np.array_equal(tigre_amd.AwminTV(img, 0.02, 15, delta=-0.05), got)
 
 
-def test_minPICCS(cuda):
+@pytest.mark.parametrize("shape", [(36, 40, 44), (19, 13, 130)])
+def test_minPICCS(cuda, shape):
     import torch
 
-    img = ellipsoid_phantom((36, 40, 44), (36.0, 40.0, 44.0), 6, seed=13)
-    prior = ellipsoid_phantom((36, 40, 44), (36.0, 40.0, 44.0), 6, seed=14)
+    img = ellipsoid_phantom(shape, tuple(float(v) for v in shape), 6, seed=13)
+    prior = ellipsoid_phantom(shape, tuple(float(v) for v in shape), 6, seed=14)
     img += np.random.default_rng(7).normal(0, 0.05, img.shape).astype(np.float32)
     got = tigre_amd.minPICCS(torch.from_numpy(img).to(cuda), torch.from_numpy(prior).to(cuda), 0.02, 15,
                              ratio=0.6).cpu().numpy()
