@@ -131,9 +131,10 @@ __device__ __forceinline__ void n_field(float c, float below, float cy, float cx
 // the x + 1 / y + 1 ones.  f and n planes are double-buffered so one barrier
 // per plane orders everything: plane p's f is written before it, read after
 // it; its n is written after it and read after the next plane's barrier.
-// The first lanes also own the halo entries of the f plane (next plane's value
-// prefetched in a register), and the lanes named by n_halo_pos those of the n
-// plane (their f of the previous plane kept in a register).
+// The first lanes also own the halo entries of the f plane (prefetched in the
+// same register ring as the lane's own column), and the lanes named by
+// n_halo_pos those of the n plane (their f of the previous plane kept in a
+// register).
 template <int kKind, bool kStep>
 __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __restrict__ f, float* __restrict__ out,
                                                              TvParams tp, int nx, int ny, int nz,
