@@ -292,6 +292,8 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
 __global__ __launch_bounds__(kTvThreads) void tv_scale_kernel(const double* __restrict__ partial, int npartial,
                                                               float alpha, double norm_eps, float* __restrict__ scale) {
     double s = 0.0;
+    // unrolled so the loads are issued together; the additions keep their order
+#pragma unroll 8
     for (int i = threadIdx.x; i < npartial; i += kTvThreads) s += partial[i];
     s = block_sum(s);
     if (threadIdx.x == 0) scale[0] = (float)((double)alpha / sqrt(s + norm_eps));
