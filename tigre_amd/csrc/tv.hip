@@ -13,9 +13,9 @@
 //   scale   one workgroup sums the partials in a fixed order (the same result
 //           every run) into alpha / sqrt(sum + norm_eps);
 //   pass 2  f' = f - scale * g, recomputing g from f.
-// Each pass is a 64 x 8 column tile that marches along z through a chunk of
-// planes, every lane one column with its z neighbours in registers and its
-// x / y neighbours through LDS, so f is read from HBM about once per pass:
+// Each pass is a 128 x 8 column tile that marches along z through a chunk of
+// planes, every lane two columns with their z neighbours in registers and
+// their x / y neighbours through LDS, so f is read from HBM about once per pass:
 // 12 bytes of traffic per voxel and iteration, no host round trip for the
 // norm.
 #include <cmath>
@@ -25,12 +25,18 @@
 namespace tigre {
 
 namespace {
-constexpr int kTX = 64, kTY = 8, kZC = 16;  // tile columns, rows (one wavefront each), planes per chunk
+constexpr int kTY = 8, kZC = 16;            // tile rows (one wavefront each), planes per chunk
+constexpr int kTX = 128;                    // tile columns: two adjacent ones per lane
 constexpr int kAhead = 4;                   // planes of f in flight per lane
-constexpr int kTvThreads = kTX * kTY;
-constexpr int kFW = kTX + 2, kFH = kTY + 2;  // f plane with a one-voxel halo
-constexpr int kNW = kTX + 1, kNH = kTY + 1;  // n plane: the tile plus the column x0 + 64 and row y0 + 4
-constexpr int kFHalo = 2 * kFW + 2 * kTY;    // halo entries of an f plane
+constexpr int kTvThreads = 64 * kTY;
+// LDS planes.  f: rows -1 .. kTY, columns -1 .. kTX at index lx + 2 (an even
+// row length, so every lane's column pair is 8-byte aligned); n: rows
+// 0 .. kTY, columns 0 .. kTX (padded to an even length).
+constexpr int kFW = kTX + 4, kFH = kTY + 2;
+constexpr int kNW = kTX + 2, kNH = kTY + 1;
+constexpr int kFHalo = 2 * (kTX + 2) + 2 * kTY;  // halo entries of an f plane
+__device__ __forceinline__ int f_index(int lx, int ly) { return (ly + 1) * kFW + lx + 2; }
+__device__ __forceinline__ int n_index(int lx, int ly) { return ly * kNW + lx; }
 
 struct TvParams {
     const float* prior;  // PICCS prior image
@@ -52,36 +58,20 @@ __device__ inline double block_sum(double v) {
     return s;  // valid in thread 0
 }
 
-// LDS index in the f plane of halo entry e: row -1 (x0 - 1 .. x0 + 64), row
-// kTY, then the columns x0 - 1 and x0 + 64 of rows 0 .. kTY - 1.
+// Tile-relative position of f-plane halo entry e: row -1 (x0 - 1 .. x0 + kTX),
+// row kTY, then the columns x0 - 1 and x0 + kTX of rows 0 .. kTY - 1.
 __device__ __forceinline__ void f_halo_pos(int e, int& lx, int& ly) {
-    if (e < kFW) {
+    if (e < kTX + 2) {
         lx = e - 1;
         ly = -1;
-    } else if (e < 2 * kFW) {
-        lx = e - kFW - 1;
+    } else if (e < 2 * (kTX + 2)) {
+        lx = e - (kTX + 2) - 1;
         ly = kTY;
     } else {
-        const int k = e - 2 * kFW;
+        const int k = e - 2 * (kTX + 2);
         lx = k < kTY ? -1 : kTX;
         ly = k < kTY ? k : k - kTY;
     }
-}
-// Halo n entry of lane (tx, ty), or false: row kTY (x0 .. x0 + 63) belongs to
-// the last wavefront, column kTX (rows 0 .. kTY - 1) to the first kTY lanes
-// of the one before (two wavefronts do a second n evaluation per plane).
-__device__ __forceinline__ bool n_halo_pos(int tx, int ty, int& lx, int& ly) {
-    if (ty == kTY - 1) {
-        lx = tx;
-        ly = kTY;
-        return true;
-    }
-    if (ty == kTY - 2 && tx < kTY) {
-        lx = kTX;
-        ly = tx;
-        return true;
-    }
-    return false;
 }
 
 // The normalised difference field at one voxel from its value c, the value
@@ -124,17 +114,19 @@ __device__ __forceinline__ void n_field(float c, float below, float cy, float cx
 // low x / y edge the difference is 0; beyond the high edge n is 0
 // (GD_TV.cu:104-114).  kStep false: pass 1 (partials); true: pass 2.
 //
-// A workgroup is a 64 x 8 column tile, one wavefront per row, marching along
-// z through a chunk of planes; every lane owns one column.  Its f and n values
-// of the previous plane stay in registers (the z neighbours), the f plane goes
-// through LDS for the x - 1 / y - 1 neighbours and the n_x / n_y planes for
-// the x + 1 / y + 1 ones.  f and n planes are double-buffered so one barrier
-// per plane orders everything: plane p's f is written before it, read after
-// it; its n is written after it and read after the next plane's barrier.
-// The first lanes also own the halo entries of the f plane (prefetched in the
-// same register ring as the lane's own column), and the lanes named by
-// n_halo_pos those of the n plane (their f of the previous plane kept in a
-// register).
+// A workgroup is a 128 x 8 column tile, one wavefront per row, marching along
+// z through a chunk of planes; every lane owns two adjacent columns.  Its f
+// and n values of the previous plane stay in registers (the z neighbours), as
+// do the x neighbours between its two columns; the f plane goes through LDS
+// for the other x - 1 / y - 1 neighbours and the n_x / n_y planes for the
+// x + 1 / y + 1 ones, in 8-byte pairs where both columns need one.  f and n
+// planes are double-buffered so one barrier per plane orders everything:
+// plane p's f is written before it, read after it; its n is written after it
+// and read after the next plane's barrier.  The first lanes also own the halo
+// entries of the f plane (prefetched in the same register ring as the own
+// columns); the last wavefront computes the n-plane halo row y0 + 8 (its
+// y - 1 neighbours are its own columns) and the first lanes of the one before
+// the halo column x0 + 128 (their f of the previous plane kept in registers).
 template <int kKind, bool kStep>
 __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __restrict__ f, float* __restrict__ out,
                                                              TvParams tp, int nx, int ny, int nz,
@@ -142,83 +134,104 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                                                              const float* __restrict__ scale_in) {
     constexpr int NF = kKind == 2 ? 2 : 1;
     constexpr bool kPadZ = kKind != 2;
-    // f planes (ring of two) with a halo of one voxel, then one slot per lane
-    // that lanes without a halo entry store their halo value to
-    __shared__ float F[2][NF][kFH * kFW + kTvThreads];
-    __shared__ float Nn[2][NF][2][kNH * kNW];     // n_y (0) and n_x (1) planes (ring of two)
+    // 16-byte aligned: every lane reads and writes 8-byte pairs (even indices)
+    __shared__ __align__(16) float F[2][NF][kFH * kFW];      // f planes (ring of two), halo of one voxel
+    __shared__ __align__(16) float Nn[2][NF][2][kNH * kNW];  // n_y (0) and n_x (1) planes (ring of two)
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
     const int z1 = min(z0 + kZC, nz);
     const long long plane = (long long)nx * ny;
     const float scale = kStep ? scale_in[0] : 0.0f;
 
-    // own column
-    const int gx = x0 + tx, gy = y0 + ty;
-    const bool col = gx < nx && gy < ny;
-    const int own_ofs = col ? gy * nx + gx : -1;
-    const int own_f = (ty + 1) * kFW + tx + 1, own_n = ty * kNW + tx;
-    const int own_flags = (col ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0);
-    // halo entry of the f plane (lanes tid < kFHalo)
-    int lx, ly;
-    int hf_idx = kFH * kFW + tid, hf_ofs = -1;
+    // own columns x0 + 2 tx + c, c = 0, 1
+    const int gy = y0 + ty;
+    int own_ofs[2], own_flags[2];
+    bool col[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int gx = x0 + 2 * tx + c;
+        col[c] = gx < nx && gy < ny;
+        own_ofs[c] = col[c] ? gy * nx + gx : -1;
+        own_flags[c] = (col[c] ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0);
+    }
+    const int own_f = f_index(2 * tx, ty), own_n = n_index(2 * tx, ty);  // even: 8-byte aligned pairs
+    // halo entry of the f plane (lanes tid < kFHalo); the other lanes store
+    // their unused halo value to their own first column just before the own
+    // pair, so the store needs no branch (a store under a branch would pull
+    // its load into the branch)
+    int hf_idx = own_f, hf_ofs = -1;
     if (tid < kFHalo) {
+        int lx, ly;
         f_halo_pos(tid, lx, ly);
-        hf_idx = (ly + 1) * kFW + lx + 1;
+        hf_idx = f_index(lx, ly);
         const int hx = x0 + lx, hy = y0 + ly;
         hf_ofs = (hx >= 0 && hx < nx && hy >= 0 && hy < ny) ? hy * nx + hx : -1;
     }
-    // halo entry of the n plane
-    int hn_f = -1, hn_n = -1, hn_ofs = -1, hn_flags = 0;
-    if (n_halo_pos(tx, ty, lx, ly)) {
-        hn_f = (ly + 1) * kFW + lx + 1;
-        hn_n = ly * kNW + lx;
-        const int hx = x0 + lx, hy = y0 + ly;
-        const bool hcol = hx < nx && hy < ny;
-        hn_ofs = hcol ? hy * nx + hx : -1;
-        hn_flags = (hcol ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+    // halo of the n plane: row kTY (columns 2 tx, 2 tx + 1) in the last
+    // wavefront, column kTX (row tx) in the first kTY lanes of the one before
+    const bool hrow = ty == kTY - 1, hcol = ty == kTY - 2 && tx < kTY;
+    int hr_ofs[2], hr_flags[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int hx = x0 + 2 * tx + c, hy = y0 + kTY;
+        const bool e = hx < nx && hy < ny;
+        hr_ofs[c] = e ? hy * nx + hx : -1;
+        hr_flags[c] = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+    }
+    int hc_ofs, hc_flags;
+    {
+        const int hx = x0 + kTX, hy = y0 + tx;
+        const bool e = hx < nx && hy < ny;
+        hc_ofs = e ? hy * nx + hx : -1;
+        hc_flags = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
     }
 
     auto load = [&](int k, long long idx) -> float {
         return k == 0 ? f[idx] : f[idx] - tp.prior[idx];
     };
-    // The image (and prior) of plane p at this lane's column and f-halo
-    // entry, fetched kAhead planes before its use into a ring of registers
-    // (one plane in flight per lane would leave HBM latency-bound).  The
-    // loads are unconditional, from a clamped in-bounds address, and raw:
-    // the masks and the PICCS difference are applied when the slot is used,
-    // so nothing has to wait for a load before then.
+    // The image (and prior) of plane p at the lane's two columns and its
+    // f-halo entry, fetched kAhead planes before use into a ring of registers.
+    // The loads are unconditional, from clamped in-bounds addresses, and raw:
+    // the masks and the PICCS difference are applied when the slot is used, so
+    // nothing has to wait for a load before then.
     const float* src[2] = {f, tp.prior};
-    const int own_ld = max(own_ofs, 0), hf_ld = max(hf_ofs, 0);
-    float ring[kAhead][NF], hring[kAhead][NF];
-    auto fetch = [&](int p, float* v, float* hv) {
+    const int ld0 = max(own_ofs[0], 0), ld1 = max(own_ofs[1], 0), hf_ld = max(hf_ofs, 0);
+    float ring[kAhead][2][3];  // [slot][image, prior][column 0, column 1, halo]
+    auto fetch = [&](int p, float (*v)[3]) {
         const long long base = (long long)min(max(p, 0), nz - 1) * plane;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            v[k] = src[k][base + own_ld];
-            hv[k] = src[k][base + hf_ld];
+            v[k][0] = src[k][base + ld0];
+            v[k][1] = src[k][base + ld1];
+            v[k][2] = src[k][base + hf_ld];
         }
     };
-    // field k of a fetched slot: 0 outside the volume, f - prior for PICCS
-    auto field = [&](const float* raw, int k, bool ok) -> float {
-        return ok ? (k == 0 ? raw[0] : raw[0] - raw[1]) : 0.0f;
+    // field k of entry e of a fetched slot: 0 outside the volume, f - prior for PICCS
+    auto field = [&](float (*v)[3], int k, int e, bool ok) -> float {
+        return ok ? (k == 0 ? v[0][e] : v[0][e] - v[1][e]) : 0.0f;
     };
 
-    float fprev[NF], hprev[NF];  // f of the previous plane: own column, n-halo column
+    // f of the previous plane: own columns, n-halo entries
+    float fprev[NF][2], hrprev[NF][2], hcprev[NF];
     {
         const int p = z0 - 1;
         const bool pin = p >= 0;
         const long long base = (long long)p * plane;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            fprev[k] = (pin && own_ofs >= 0) ? load(k, base + own_ofs) : 0.0f;
-            hprev[k] = (pin && hn_ofs >= 0) ? load(k, base + hn_ofs) : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                fprev[k][c] = (pin && own_ofs[c] >= 0) ? load(k, base + own_ofs[c]) : 0.0f;
+                hrprev[k][c] = (pin && hrow && hr_ofs[c] >= 0) ? load(k, base + hr_ofs[c]) : 0.0f;
+            }
+            hcprev[k] = (pin && hcol && hc_ofs >= 0) ? load(k, base + hc_ofs) : 0.0f;
         }
     }
 #pragma unroll
-    for (int j = 0; j < kAhead; ++j) fetch(min(z0 + j, z1), ring[j], hring[j]);
-    float nprev[NF][3];  // n of the previous plane at the own column (z, y, x)
+    for (int j = 0; j < kAhead; ++j) fetch(min(z0 + j, z1), ring[j]);
+    float nprev[NF][2][3];  // n of the previous plane at the own columns (z, y, x)
     double acc = 0.0;
-    // fully unrolled (planes z0 .. z0 + kZC, at most): ring slot j % kAhead is
+    // fully unrolled (planes z0 .. z0 + kZC, at most): ring slot jj % kAhead is
     // a fixed register set and the compiler sees every load it must wait for
 #pragma unroll
     for (int jj = 0; jj <= kZC; ++jj) {
@@ -226,59 +239,98 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         const int p = z0 + jj;
         if (p > z1) break;
         const int b = p & 1;
-        const float fpp = fprev[0];  // f at plane p - 1 (the image itself, also for PICCS)
+        const float fpp[2] = {fprev[0][0], fprev[0][1]};  // f at plane p - 1 (the image, also for PICCS)
         const bool pin = p >= 0 && p < nz;
-        float fc[NF];
+        float fc[NF][2];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            fc[k] = field(ring[j], k, pin && own_ofs >= 0);
-            F[b][k][own_f] = fc[k];
-            // unconditional: a store under a branch would pull the load into it
-            F[b][k][hf_idx] = field(hring[j], k, pin && hf_ofs >= 0);
+            fc[k][0] = field(ring[j], k, 0, pin && own_ofs[0] >= 0);
+            fc[k][1] = field(ring[j], k, 1, pin && own_ofs[1] >= 0);
+            F[b][k][hf_idx] = field(ring[j], k, 2, pin && hf_ofs >= 0);
+            *reinterpret_cast<float2*>(&F[b][k][own_f]) = make_float2(fc[k][0], fc[k][1]);
         }
         // unconditional (a branch around the loads would make the compiler
         // wait for all of them); past the chunk it re-reads plane z1 from cache
-        fetch(min(p + kAhead, z1), ring[j], hring[j]);
+        fetch(min(p + kAhead, z1), ring[j]);
         __syncthreads();
         const bool pex = p >= 0 && (kPadZ ? p <= nz : p < nz);
         const bool in = p < nz, has_below = p >= 1;
-        float ncur[NF][3];
+        float ncur[NF][2][3];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            n_field<kKind>(fc[k], fprev[k], F[b][k][own_f - kFW], F[b][k][own_f - 1], own_flags, pex, in, has_below,
-                           tp, ncur[k][0], ncur[k][1], ncur[k][2]);
-            Nn[b][k][0][own_n] = ncur[k][1];
-            Nn[b][k][1][own_n] = ncur[k][2];
-            fprev[k] = fc[k];
-            if (hn_n >= 0) {
-                const float hc = F[b][k][hn_f];
-                float m0, m1, m2;
-                n_field<kKind>(hc, hprev[k], F[b][k][hn_f - kFW], F[b][k][hn_f - 1], hn_flags, pex, in, has_below,
-                               tp, m0, m1, m2);
-                Nn[b][k][0][hn_n] = m1;
-                Nn[b][k][1][hn_n] = m2;
-                hprev[k] = hc;
+            const float2 up = *reinterpret_cast<const float2*>(&F[b][k][own_f - kFW]);  // y - 1 of both columns
+            const float left = F[b][k][own_f - 1];                                      // x - 1 of column 0
+            n_field<kKind>(fc[k][0], fprev[k][0], up.x, left, own_flags[0], pex, in, has_below, tp, ncur[k][0][0],
+                           ncur[k][0][1], ncur[k][0][2]);
+            n_field<kKind>(fc[k][1], fprev[k][1], up.y, fc[k][0], own_flags[1], pex, in, has_below, tp,
+                           ncur[k][1][0], ncur[k][1][1], ncur[k][1][2]);
+            *reinterpret_cast<float2*>(&Nn[b][k][0][own_n]) = make_float2(ncur[k][0][1], ncur[k][1][1]);
+            *reinterpret_cast<float2*>(&Nn[b][k][1][own_n]) = make_float2(ncur[k][0][2], ncur[k][1][2]);
+            fprev[k][0] = fc[k][0];
+            fprev[k][1] = fc[k][1];
+            if (hrow) {  // row kTY: its y - 1 neighbours are this lane's own columns
+                const int hi = f_index(2 * tx, kTY);
+                const float2 hv = *reinterpret_cast<const float2*>(&F[b][k][hi]);
+                const float hl = F[b][k][hi - 1];
+                float m0[3], m1[3];
+                n_field<kKind>(hv.x, hrprev[k][0], fc[k][0], hl, hr_flags[0], pex, in, has_below, tp, m0[0], m0[1],
+                               m0[2]);
+                n_field<kKind>(hv.y, hrprev[k][1], fc[k][1], hv.x, hr_flags[1], pex, in, has_below, tp, m1[0], m1[1],
+                               m1[2]);
+                const int hn = n_index(2 * tx, kTY);
+                *reinterpret_cast<float2*>(&Nn[b][k][0][hn]) = make_float2(m0[1], m1[1]);
+                *reinterpret_cast<float2*>(&Nn[b][k][1][hn]) = make_float2(m0[2], m1[2]);
+                hrprev[k][0] = hv.x;
+                hrprev[k][1] = hv.y;
+            }
+            if (hcol) {  // column kTX, row tx
+                const int hi = f_index(kTX, tx);
+                const float hc = F[b][k][hi];
+                float m[3];
+                n_field<kKind>(hc, hcprev[k], F[b][k][hi - kFW], F[b][k][hi - 1], hc_flags, pex, in, has_below, tp,
+                               m[0], m[1], m[2]);
+                const int hn = n_index(kTX, tx);
+                Nn[b][k][0][hn] = m[1];
+                Nn[b][k][1][hn] = m[2];
+                hcprev[k] = hc;
             }
         }
         // g at plane p - 1 from the n planes of p - 1 (buffer b ^ 1, complete
         // since this plane's barrier) and n_z of plane p
-        if (p > z0 && col) {
-            float g[NF];
+        if (p > z0) {
+            float res[2];
 #pragma unroll
-            for (int k = 0; k < NF; ++k)
-                g[k] = nprev[k][0] + nprev[k][1] + nprev[k][2] - Nn[b ^ 1][k][1][own_n + 1] -
-                       Nn[b ^ 1][k][0][own_n + kNW] - ncur[k][0];
-            const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
+            for (int c = 0; c < 2; ++c) {
+                float g[NF];
+#pragma unroll
+                for (int k = 0; k < NF; ++k) {
+                    // n_x at x + 1: the lane's own second column, or the next lane's first
+                    const float nxr = c == 0 ? nprev[k][1][2] : Nn[b ^ 1][k][1][own_n + 2];
+                    const float nyu = Nn[b ^ 1][k][0][own_n + kNW + c];
+                    g[k] = nprev[k][c][0] + nprev[k][c][1] + nprev[k][c][2] - nxr - nyu - ncur[k][c][0];
+                }
+                const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
+                res[c] = fpp[c] - gv * scale;
+                if (!kStep && col[c]) acc += (double)gv * (double)gv;
+            }
             if (kStep) {
-                out[(long long)(p - 1) * plane + own_ofs] = fpp - gv * scale;
-            } else {
-                acc += (double)gv * (double)gv;
+                // the pair as one 8-byte store where both columns exist and the
+                // address allows it (col[1] implies col[0])
+                float* o = out + (long long)(p - 1) * plane;
+                if (col[1] && (reinterpret_cast<unsigned long long>(o + own_ofs[0]) & 7ull) == 0) {
+                    *reinterpret_cast<float2*>(o + own_ofs[0]) = make_float2(res[0], res[1]);
+                } else {
+                    if (col[0]) o[own_ofs[0]] = res[0];
+                    if (col[1]) o[own_ofs[1]] = res[1];
+                }
             }
         }
 #pragma unroll
         for (int k = 0; k < NF; ++k)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) nprev[k][c] = ncur[k][c];
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) nprev[k][c][q] = ncur[k][c][q];
     }
     if (!kStep) {
         acc = block_sum(acc);
