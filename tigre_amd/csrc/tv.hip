@@ -83,27 +83,27 @@ __device__ __forceinline__ void n_field(float c, float below, float cy, float cx
                                         bool has_below, const TvParams& tp, float& n0, float& n1, float& n2) {
     constexpr bool kPadZ = kKind != 2;
     constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
-    n0 = n1 = n2 = 0.0f;
-    if (pex && (flags & 1)) {
-        const float cc = in ? c : 0.0f;
-        const float bb = has_below ? below : 0.0f;
-        const float d0 = kPadZ ? cc - bb : (has_below ? cc - bb : 0.0f);
-        const float d1 = ((flags & 2) && in) ? cc - cy : 0.0f;
-        const float d2 = ((flags & 4) && in) ? cc - cx : 0.0f;
-        float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
-        if (kKind == 1) {
-            const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
-            w0 = __expf(-a0 * a0);
-            w1 = __expf(-a1 * a1);
-            w2 = __expf(-a2 * a2);
-        }
-        // hardware square root and reciprocal (1 ulp each)
-        const float inv =
-            __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(w0 * d0 * d0 + w1 * d1 * d1 + w2 * d2 * d2) + eps);
-        n0 = w0 * d0 * inv;
-        n1 = w1 * d1 * inv;
-        n2 = w2 * d2 * inv;
+    // branch-free: where n does not exist the reciprocal is replaced by 0, so
+    // n is +-0 (every operand is finite: f planes hold 0 outside the volume)
+    const bool ok = pex && (flags & 1);
+    const float cc = in ? c : 0.0f;
+    const float bb = has_below ? below : 0.0f;
+    const float d0 = kPadZ ? cc - bb : (has_below ? cc - bb : 0.0f);
+    const float d1 = ((flags & 2) && in) ? cc - cy : 0.0f;
+    const float d2 = ((flags & 4) && in) ? cc - cx : 0.0f;
+    float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
+    if (kKind == 1) {
+        const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
+        w0 = __expf(-a0 * a0);
+        w1 = __expf(-a1 * a1);
+        w2 = __expf(-a2 * a2);
     }
+    // hardware square root and reciprocal (1 ulp each)
+    const float inv = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(w0 * d0 * d0 + w1 * d1 * d1 + w2 * d2 * d2) + eps);
+    const float s = ok ? inv : 0.0f;
+    n0 = w0 * d0 * s;
+    n1 = w1 * d1 * s;
+    n2 = w2 * d2 * s;
 }
 }  // namespace
 
