@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     constexpr bool kPadZ = kKind != 2;
     // 16-byte aligned: every lane reads and writes 8-byte pairs (even indices)
     __shared__ __align__(16) float F[2][NF][kFH * kFW];      // f planes (ring of two), halo of one voxel
-    __shared__ __align__(16) float Nn[2][NF][2][kNH * kNW];  // n_y (0) and n_x (1) planes (ring of two)
+    __shared__ __align__(16) float Nn[2][2][kNH * kNW];  // (combined) n_y (0) and n_x (1) planes (ring of two)
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
     const int z1 = min(z0 + kZC, nz);
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     }
 #pragma unroll
     for (int j = 0; j < kAhead; ++j) fetch(min(z0 + j, z1), ring[j]);
-    float nprev[NF][2][3];  // n of the previous plane at the own columns (z, y, x)
+    float nprev[2][3];  // (combined) n of the previous plane at the own columns (z, y, x)
     double acc = 0.0;
     // fully unrolled (planes z0 .. z0 + kZC, at most): ring slot jj % kAhead is
     // a fixed register set and the compiler sees every load it must wait for
@@ -255,17 +255,26 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         __syncthreads();
         const bool pex = p >= 0 && (kPadZ ? p <= nz : p < nz);
         const bool in = p < nz, has_below = p >= 1;
-        float ncur[NF][2][3];
+        // n of the own columns and of the halo entries, combined over the
+        // fields (PICCS: (1 - ratio) n(f) + ratio n(f - prior); g is linear in
+        // n, so one n field goes through LDS; the weight is 1 for one field)
+        float ncur[2][3], hrn[2][3], hcn[3];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
+            const float wk = NF == 1 ? 1.0f : (k == 0 ? 1.0f - tp.ratio : tp.ratio);
+            auto mix = [&](float* acc3, const float* t) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) acc3[q] = k == 0 ? wk * t[q] : fmaf(wk, t[q], acc3[q]);
+            };
             const float2 up = *reinterpret_cast<const float2*>(&F[b][k][own_f - kFW]);  // y - 1 of both columns
             const float left = F[b][k][own_f - 1];                                      // x - 1 of column 0
-            n_field<kKind>(fc[k][0], fprev[k][0], up.x, left, own_flags[0], pex, in, has_below, tp, ncur[k][0][0],
-                           ncur[k][0][1], ncur[k][0][2]);
-            n_field<kKind>(fc[k][1], fprev[k][1], up.y, fc[k][0], own_flags[1], pex, in, has_below, tp,
-                           ncur[k][1][0], ncur[k][1][1], ncur[k][1][2]);
-            *reinterpret_cast<float2*>(&Nn[b][k][0][own_n]) = make_float2(ncur[k][0][1], ncur[k][1][1]);
-            *reinterpret_cast<float2*>(&Nn[b][k][1][own_n]) = make_float2(ncur[k][0][2], ncur[k][1][2]);
+            float t0[3], t1[3];
+            n_field<kKind>(fc[k][0], fprev[k][0], up.x, left, own_flags[0], pex, in, has_below, tp, t0[0], t0[1],
+                           t0[2]);
+            n_field<kKind>(fc[k][1], fprev[k][1], up.y, fc[k][0], own_flags[1], pex, in, has_below, tp, t1[0], t1[1],
+                           t1[2]);
+            mix(ncur[0], t0);
+            mix(ncur[1], t1);
             fprev[k][0] = fc[k][0];
             fprev[k][1] = fc[k][1];
             if (hrow) {  // row kTY: its y - 1 neighbours are this lane's own columns
@@ -277,9 +286,8 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                                m0[2]);
                 n_field<kKind>(hv.y, hrprev[k][1], fc[k][1], hv.x, hr_flags[1], pex, in, has_below, tp, m1[0], m1[1],
                                m1[2]);
-                const int hn = n_index(2 * tx, kTY);
-                *reinterpret_cast<float2*>(&Nn[b][k][0][hn]) = make_float2(m0[1], m1[1]);
-                *reinterpret_cast<float2*>(&Nn[b][k][1][hn]) = make_float2(m0[2], m1[2]);
+                mix(hrn[0], m0);
+                mix(hrn[1], m1);
                 hrprev[k][0] = hv.x;
                 hrprev[k][1] = hv.y;
             }
@@ -289,11 +297,21 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                 float m[3];
                 n_field<kKind>(hc, hcprev[k], F[b][k][hi - kFW], F[b][k][hi - 1], hc_flags, pex, in, has_below, tp,
                                m[0], m[1], m[2]);
-                const int hn = n_index(kTX, tx);
-                Nn[b][k][0][hn] = m[1];
-                Nn[b][k][1][hn] = m[2];
+                mix(hcn, m);
                 hcprev[k] = hc;
             }
+        }
+        *reinterpret_cast<float2*>(&Nn[b][0][own_n]) = make_float2(ncur[0][1], ncur[1][1]);
+        *reinterpret_cast<float2*>(&Nn[b][1][own_n]) = make_float2(ncur[0][2], ncur[1][2]);
+        if (hrow) {
+            const int hn = n_index(2 * tx, kTY);
+            *reinterpret_cast<float2*>(&Nn[b][0][hn]) = make_float2(hrn[0][1], hrn[1][1]);
+            *reinterpret_cast<float2*>(&Nn[b][1][hn]) = make_float2(hrn[0][2], hrn[1][2]);
+        }
+        if (hcol) {
+            const int hn = n_index(kTX, tx);
+            Nn[b][0][hn] = hcn[1];
+            Nn[b][1][hn] = hcn[2];
         }
         // g at plane p - 1 from the n planes of p - 1 (buffer b ^ 1, complete
         // since this plane's barrier) and n_z of plane p
@@ -301,15 +319,10 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             float res[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                float g[NF];
-#pragma unroll
-                for (int k = 0; k < NF; ++k) {
-                    // n_x at x + 1: the lane's own second column, or the next lane's first
-                    const float nxr = c == 0 ? nprev[k][1][2] : Nn[b ^ 1][k][1][own_n + 2];
-                    const float nyu = Nn[b ^ 1][k][0][own_n + kNW + c];
-                    g[k] = nprev[k][c][0] + nprev[k][c][1] + nprev[k][c][2] - nxr - nyu - ncur[k][c][0];
-                }
-                const float gv = kKind == 2 ? (1.0f - tp.ratio) * g[0] + tp.ratio * g[NF - 1] : g[0];
+                // n_x at x + 1: the lane's own second column, or the next lane's first
+                const float nxr = c == 0 ? nprev[1][2] : Nn[b ^ 1][1][own_n + 2];
+                const float nyu = Nn[b ^ 1][0][own_n + kNW + c];
+                const float gv = nprev[c][0] + nprev[c][1] + nprev[c][2] - nxr - nyu - ncur[c][0];
                 res[c] = fpp[c] - gv * scale;
                 if (!kStep && col[c]) acc += (double)gv * (double)gv;
             }
@@ -326,11 +339,9 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             }
         }
 #pragma unroll
-        for (int k = 0; k < NF; ++k)
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
-#pragma unroll
-                for (int q = 0; q < 3; ++q) nprev[k][c][q] = ncur[k][c][q];
+            for (int q = 0; q < 3; ++q) nprev[c][q] = ncur[c][q];
     }
     if (!kStep) {
         acc = block_sum(acc);
