@@ -194,11 +194,12 @@ def test_matlab_layout(cuda, kind):
         assert oracle.rel_l2(out, ref) <= TOL
 
 
-@pytest.mark.parametrize("shape", [(40, 48, 56), (19, 13, 130), (2, 17, 67)])
+@pytest.mark.parametrize("shape", [(40, 48, 56), (19, 13, 130), (2, 17, 67), (1, 9, 300)])
 def test_minTV(cuda, shape):
     """minTV against the oracle; the second shape has several x tiles with a
     ragged last one, a ragged y tile and a 3-plane z chunk, the third fewer
-    planes than the kernel prefetches."""
+    planes than the kernel prefetches, the fourth a single plane across three
+    x tiles."""
     import torch
 
     img = ellipsoid_phantom(shape, tuple(float(v) for v in shape), 6, seed=9)
