@@ -21,6 +21,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "ray.hpp"  // xcd_logical_block
 
 namespace tigre {
 
@@ -138,7 +139,11 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     __shared__ __align__(16) float F[2][NF][kFH * kFW];      // f planes (ring of two), halo of one voxel
     __shared__ __align__(16) float Nn[2][2][kNH * kNW];  // (combined) n_y (0) and n_x (1) planes (ring of two)
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = blockIdx.z * kZC;
+    // tiles in x-fastest order, consecutive ones on one XCD: the halos a tile
+    // reads from its neighbours are then mostly in that XCD's L2
+    const int ntx = (nx + kTX - 1) / kTX, nty = (ny + kTY - 1) / kTY;
+    const int L = xcd_logical_block((int)blockIdx.x, (int)gridDim.x);
+    const int x0 = (L % ntx) * kTX, y0 = ((L / ntx) % nty) * kTY, z0 = (L / (ntx * nty)) * kZC;
     const int z1 = min(z0 + kZC, nz);
     const long long plane = (long long)nx * ny;
     const float scale = kStep ? scale_in[0] : 0.0f;
@@ -346,7 +351,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     if (!kStep) {
         acc = block_sum(acc);
         if (tid == 0)
-            partial[((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = acc;
+            partial[L] = acc;
     }
 }
 
@@ -367,16 +372,17 @@ template <int kKind>
 int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp, double norm_eps, long nx, long ny,
                    long nz, int maxiter, hipStream_t stream) {
     TIGRE_REQUIRE(nx > 0 && ny > 0 && nz > 0, "empty image");
-    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && nx * ny < (1LL << 31) && ny < 65535L * kTY && nx < 65535L * kTX,
-                  "image too large");
+    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && nx * ny < (1LL << 31), "image too large");
     const size_t n = (size_t)nx * ny * nz;
     if (maxiter <= 0) {
         if (dst != img) TIGRE_HIP_TRY(hipMemcpyAsync(dst, img, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
         return TIGRE_OK;
     }
-    const dim3 grid((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY),
-                    (unsigned)((nz + kZC - 1) / kZC));
-    const int nblocks = (int)(grid.x * grid.y * grid.z);
+    // one tile per workgroup in a 1-D grid (the kernel orders the tiles per XCD)
+    const long long tiles = ((nx + kTX - 1) / kTX) * ((ny + kTY - 1) / kTY) * ((nz + kZC - 1) / kZC);
+    TIGRE_REQUIRE(tiles < (1LL << 31), "image too large");
+    const int nblocks = (int)tiles;
+    const dim3 grid((unsigned)nblocks);
     float* buf[2] = {nullptr, nullptr};
     double* partial = nullptr;
     float* scale = nullptr;
