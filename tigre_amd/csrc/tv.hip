@@ -125,9 +125,9 @@ __device__ __forceinline__ void n_field(float c, float below, float cy, float cx
 // plane p's f is written before it, read after it; its n is written after it
 // and read after the next plane's barrier.  The first lanes also own the halo
 // entries of the f plane (prefetched in the same register ring as the own
-// columns); the last wavefront computes the n-plane halo row y0 + 8 (its
-// y - 1 neighbours are its own columns) and the first lanes of the one before
-// the halo column x0 + 128 (their f of the previous plane kept in registers).
+// columns); the last three wavefronts compute the n-plane halo row y0 + 8 and
+// column x0 + 128, one entry per lane (its f of the previous plane kept in a
+// register).
 template <int kKind, bool kStep>
 __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __restrict__ f, float* __restrict__ out,
                                                              TvParams tp, int nx, int ny, int nz,
@@ -172,23 +172,30 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         const int hx = x0 + lx, hy = y0 + ly;
         hf_ofs = (hx >= 0 && hx < nx && hy >= 0 && hy < ny) ? hy * nx + hx : -1;
     }
-    // halo of the n plane: row kTY (columns 2 tx, 2 tx + 1) in the last
-    // wavefront, column kTX (row tx) in the first kTY lanes of the one before
-    const bool hrow = ty == kTY - 1, hcol = ty == kTY - 2 && tx < kTY;
-    int hr_ofs[2], hr_flags[2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int hx = x0 + 2 * tx + c, hy = y0 + kTY;
-        const bool e = hx < nx && hy < ny;
-        hr_ofs[c] = e ? hy * nx + hx : -1;
-        hr_flags[c] = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+    // halo of the n plane, one entry per lane: row kTY, column 2 tx (last
+    // wavefront) and 2 tx + 1 (the one before), and column kTX, row tx (first
+    // kTY lanes of the third-to-last), so no wavefront does more than one extra
+    // n evaluation per plane
+    int hn_lx = -1, hn_ly = -1;
+    if (ty == kTY - 1) {
+        hn_lx = 2 * tx;
+        hn_ly = kTY;
+    } else if (ty == kTY - 2) {
+        hn_lx = 2 * tx + 1;
+        hn_ly = kTY;
+    } else if (ty == kTY - 3 && tx < kTY) {
+        hn_lx = kTX;
+        hn_ly = tx;
     }
-    int hc_ofs, hc_flags;
-    {
-        const int hx = x0 + kTX, hy = y0 + tx;
+    const bool hn_on = hn_lx >= 0;
+    int hn_ofs = -1, hn_flags = 0, hn_f = 0, hn_n = 0;
+    if (hn_on) {
+        const int hx = x0 + hn_lx, hy = y0 + hn_ly;
         const bool e = hx < nx && hy < ny;
-        hc_ofs = e ? hy * nx + hx : -1;
-        hc_flags = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+        hn_ofs = e ? hy * nx + hx : -1;
+        hn_flags = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+        hn_f = f_index(hn_lx, hn_ly);
+        hn_n = n_index(hn_lx, hn_ly);
     }
 
     auto load = [&](int k, long long idx) -> float {
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     };
 
     // f of the previous plane: own columns, n-halo entries
-    float fprev[NF][2], hrprev[NF][2], hcprev[NF];
+    float fprev[NF][2], hprev[NF];
     {
         const int p = z0 - 1;
         const bool pin = p >= 0;
@@ -225,11 +232,8 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                fprev[k][c] = (pin && own_ofs[c] >= 0) ? load(k, base + own_ofs[c]) : 0.0f;
-                hrprev[k][c] = (pin && hrow && hr_ofs[c] >= 0) ? load(k, base + hr_ofs[c]) : 0.0f;
-            }
-            hcprev[k] = (pin && hcol && hc_ofs >= 0) ? load(k, base + hc_ofs) : 0.0f;
+            for (int c = 0; c < 2; ++c) fprev[k][c] = (pin && own_ofs[c] >= 0) ? load(k, base + own_ofs[c]) : 0.0f;
+            hprev[k] = (pin && hn_on && hn_ofs >= 0) ? load(k, base + hn_ofs) : 0.0f;
         }
     }
 #pragma unroll
@@ -263,7 +267,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         // n of the own columns and of the halo entries, combined over the
         // fields (PICCS: (1 - ratio) n(f) + ratio n(f - prior); g is linear in
         // n, so one n field goes through LDS; the weight is 1 for one field)
-        float ncur[2][3], hrn[2][3], hcn[3];
+        float ncur[2][3], hn3[3];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
             const float wk = NF == 1 ? 1.0f : (k == 0 ? 1.0f - tp.ratio : tp.ratio);
@@ -282,41 +286,20 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             mix(ncur[1], t1);
             fprev[k][0] = fc[k][0];
             fprev[k][1] = fc[k][1];
-            if (hrow) {  // row kTY: its y - 1 neighbours are this lane's own columns
-                const int hi = f_index(2 * tx, kTY);
-                const float2 hv = *reinterpret_cast<const float2*>(&F[b][k][hi]);
-                const float hl = F[b][k][hi - 1];
-                float m0[3], m1[3];
-                n_field<kKind>(hv.x, hrprev[k][0], fc[k][0], hl, hr_flags[0], pex, in, has_below, tp, m0[0], m0[1],
-                               m0[2]);
-                n_field<kKind>(hv.y, hrprev[k][1], fc[k][1], hv.x, hr_flags[1], pex, in, has_below, tp, m1[0], m1[1],
-                               m1[2]);
-                mix(hrn[0], m0);
-                mix(hrn[1], m1);
-                hrprev[k][0] = hv.x;
-                hrprev[k][1] = hv.y;
-            }
-            if (hcol) {  // column kTX, row tx
-                const int hi = f_index(kTX, tx);
-                const float hc = F[b][k][hi];
+            if (hn_on) {
+                const float hc = F[b][k][hn_f];
                 float m[3];
-                n_field<kKind>(hc, hcprev[k], F[b][k][hi - kFW], F[b][k][hi - 1], hc_flags, pex, in, has_below, tp,
+                n_field<kKind>(hc, hprev[k], F[b][k][hn_f - kFW], F[b][k][hn_f - 1], hn_flags, pex, in, has_below, tp,
                                m[0], m[1], m[2]);
-                mix(hcn, m);
-                hcprev[k] = hc;
+                mix(hn3, m);
+                hprev[k] = hc;
             }
         }
         *reinterpret_cast<float2*>(&Nn[b][0][own_n]) = make_float2(ncur[0][1], ncur[1][1]);
         *reinterpret_cast<float2*>(&Nn[b][1][own_n]) = make_float2(ncur[0][2], ncur[1][2]);
-        if (hrow) {
-            const int hn = n_index(2 * tx, kTY);
-            *reinterpret_cast<float2*>(&Nn[b][0][hn]) = make_float2(hrn[0][1], hrn[1][1]);
-            *reinterpret_cast<float2*>(&Nn[b][1][hn]) = make_float2(hrn[0][2], hrn[1][2]);
-        }
-        if (hcol) {
-            const int hn = n_index(kTX, tx);
-            Nn[b][0][hn] = hcn[1];
-            Nn[b][1][hn] = hcn[2];
+        if (hn_on) {
+            Nn[b][0][hn_n] = hn3[1];
+            Nn[b][1][hn_n] = hn3[2];
         }
         // g at plane p - 1 from the n planes of p - 1 (buffer b ^ 1, complete
         // since this plane's barrier) and n_z of plane p
