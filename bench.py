@@ -23,6 +23,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads stay on neighbouring cores; set before any
+# OpenMP runtime (torch's or the oracle's) reads its environment.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
@@ -47,6 +51,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may run on, capped by
+    OMP_NUM_THREADS when set (the share of a shared host allotted to the job)."""
+    n = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, cap) if cap > 0 else n
+
+
+def host_description():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,8 +81,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512, help="volume edge / detector edge")
     ap.add_argument("--views", type=int, default=360)
-    ap.add_argument("--cpu-views", type=int, default=64,
-                    help="views in the CPU-baseline sample, indices floor(k * views / cpu_views) (0 = skip)")
+    ap.add_argument("--cpu-views", type=int, default=24,
+                    help="views in the timed CPU-baseline sample, indices floor(k * views / cpu_views) (0 = skip)")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU-baseline runs (the median is reported)")
+    ap.add_argument("--parity-views", type=int, default=64,
+                    help="views of the full-size parity check against the oracle (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
     ap.add_argument("--ossart-iters", type=int, default=20,
                     help="pair: OS-SART iterations (20-view subsets, lambda 1) after the timed region (0 = skip)")
@@ -245,31 +275,45 @@ def main():
     if ossart is not None:
         result["ossart"] = ossart
 
-    if args.cpu_views > 0:
+    if args.cpu_views > 0 or args.parity_views > 0:
         import oracle
 
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         oracle.set_threads(threads)
-        idx = np.array([k * nviews // args.cpu_views for k in range(args.cpu_views)])
-        sub = angles[idx]
-        gp = prepared(geo, sub)
         img_h = img.cpu().numpy()
-        t = time.perf_counter()
+    if args.parity_views > 0:
+        # full-size parity of the 360-view calls on a fixed view subset
+        idx = np.array([k * nviews // args.parity_views for k in range(args.parity_views)])
+        gp = prepared(geo, angles[idx])
         ref_p = oracle.ax(img_h, gp, "Siddon")
-        t_ax = time.perf_counter() - t
         proj_sub = proj[torch.as_tensor(idx, device=dev)].contiguous()
-        t = time.perf_counter()
         ref_v = oracle.atb(proj_sub.cpu().numpy(), gp, "FDK")
-        t_atb = time.perf_counter() - t
-        got_v = tigre_amd.Atb(proj_sub, geo, sub, "FDK").cpu().numpy()
-        cpu_rate = 2.0 * V * len(idx) / (t_ax + t_atb) / 1e9
-        result["cpu_baseline"] = {
-            "value": round(cpu_rate, 4), "unit": "Gupdates/s", "cores": oracle.get_threads(), "kind": "port",
-            "sample": "FP64 OpenMP oracle: Ax Siddon + Atb FDK over %d of the %d views (%.1f s)" % (
-                len(idx), nviews, t_ax + t_atb)}
+        got_v = tigre_amd.Atb(proj_sub, geo, angles[idx], "FDK").cpu().numpy()
         result["parity_full_size"] = {
             "ax_siddon_rel_l2": oracle.rel_l2(proj_sub.cpu().numpy(), ref_p),
             "atb_fdk_rel_l2": oracle.rel_l2(got_v, ref_v), "views": [int(i) for i in idx]}
+        log("parity on %d views: Ax %.2e, Atb %.2e" % (len(idx), result["parity_full_size"]["ax_siddon_rel_l2"],
+                                                       result["parity_full_size"]["atb_fdk_rel_l2"]))
+        del ref_p, ref_v, got_v
+    if args.cpu_views > 0:
+        # the FP64 OpenMP oracle on the host: Ax Siddon + Atb FDK over a view sample, median of runs
+        idx = np.array([k * nviews // args.cpu_views for k in range(args.cpu_views)])
+        gp = prepared(geo, angles[idx])
+        proj_h = proj[torch.as_tensor(idx, device=dev)].contiguous().cpu().numpy()
+        runs = []
+        for _ in range(max(1, args.cpu_runs)):
+            t = time.perf_counter()
+            oracle.ax(img_h, gp, "Siddon")
+            oracle.atb(proj_h, gp, "FDK")
+            runs.append(time.perf_counter() - t)
+        t_med = float(np.median(runs))
+        cpu_rate = 2.0 * V * len(idx) / t_med / 1e9
+        result["cpu_baseline"] = {
+            "value": round(cpu_rate, 4), "unit": "Gupdates/s", "cores": oracle.get_threads(), "kind": "port",
+            "sample": "FP64 OpenMP oracle: Ax Siddon + Atb FDK over %d of the %d views (views floor(k*%d/%d)), "
+                      "median of %d runs (%s s)" % (len(idx), nviews, nviews, len(idx), len(runs),
+                                                   ", ".join("%.1f" % r for r in runs)),
+            "host": host_description()}
         log("cpu baseline %.4f Gupdates/s on %d threads" % (cpu_rate, oracle.get_threads()))
     print(json.dumps(result), flush=True)
 

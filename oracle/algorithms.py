@@ -144,7 +144,11 @@ def weights_w(geo, angles):
 
 
 def weights_v(geo, angles, blocks):
-    """V per block: mean over z of Atb(ones) over a shrunk volume (iterative_recon_alg.py:232-270)."""
+    """V per block: mean over z of Atb(ones) over a shrunk volume (iterative_recon_alg.py:232-270);
+    for a parallel beam, the number of views of the block (:265-267)."""
+    if getattr(geo, "mode", "cone") == "parallel":
+        ny, nx = int(geo.nVoxel[1]), int(geo.nVoxel[2])
+        return np.stack([np.full((ny, nx), float(len(idx))) for idx in blocks])
     out = []
     for idx in blocks:
         g = sub_geometry(geo, idx)
@@ -203,14 +207,23 @@ def cgls(proj, geo, angles, niter):
 
 # -- ASD-POCS ----------------------------------------------------------------------
 def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, tviter=20, lmbda=1.0,
-             lmbda_red=1.0, epsilon=None):
-    """pocs_algorithms.py:105-169 with blocksize 1 (one view per SART step)."""
+             lmbda_red=1.0, epsilon=None, blocksize=1, trace=None, snapshots=None):
+    """pocs_algorithms.py:105-169: a SART sweep over subsets of ``blocksize``
+    views (1 for ASD-POCS) with the non-negativity clip, then ``tviter`` TV
+    steps of size dtvg; dtvg = alpha * dp after the first sweep, reduced by
+    alpha_red when dg > rmax * dp and dd > epsilon; beta = lmbda reduced by
+    lmbda_red per iteration (it stops the loop below 0.005 but never scales the
+    SART update, as in the reference).
+
+    ``trace`` (a list) receives (n_iter, dd, dp, dg, c, dtvg) per iteration,
+    dtvg being the step the TV call used; ``snapshots`` (a list) receives
+    (pre-TV image, post-TV image) per iteration."""
     geo = copy.deepcopy(geo)
     geo.check_geo(angles)
     ang = geo.angles
     gfull = prep(geo, ang)
     proj = np.asarray(proj, dtype=np.float64)
-    blocks = ordered_blocks(len(ang), 1)
+    blocks = ordered_blocks(len(ang), blocksize)
     W = weights_w(geo, ang)
     V = weights_v(geo, ang, blocks)
     if epsilon is None:
@@ -231,12 +244,17 @@ def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, t
         if n_iter == 1:
             dtvg = alpha * dp
         res_prev = res.copy()
+        used = dtvg
         res = pocs_tv(res, dtvg, tviter)
+        if snapshots is not None:
+            snapshots.append((res_prev, res.copy()))
         dg_vec = res - res_prev
         dg = np.linalg.norm(dg_vec)
         if dg > rmax * dp and dd > epsilon:
             dtvg *= alpha_red
         beta *= lmbda_red
         c = float(np.dot(dg_vec.ravel(), dp_vec.ravel())) / max(dg * dp, 1e-6)
+        if trace is not None:
+            trace.append((n_iter, float(dd), float(dp), float(dg), c, float(used)))
         if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter >= niter:
             return res

@@ -362,7 +362,11 @@ void og_ax_interp(const float* img, const og_geo* g, int layout_matlab, double* 
         og_forward_frame(g, view, 1, f);
         d3 O = {f[0], f[1], f[2]}, DU = {f[3], f[4], f[5]}, DV = {f[6], f[7], f[8]}, S = {f[9], f[10], f[11]};
         double crop = floor(og_maxdistance_cuboid(g, view));
-        int spherical = (g->angles[3 * view + 1] != 0.0) || (g->angles[3 * view + 2] != 0.0);
+        /* the flag sums |theta| + |psi| over the view's batch of PROJ_PER_BLOCK = 9
+         * consecutive views of the call (ray_interpolated_projection.cu:75, :352-365) */
+        int spherical = 0;
+        for (int j = (view / 9) * 9; j < (view / 9) * 9 + 9 && j < g->nangles; ++j)
+            if (g->angles[3 * j + 1] != 0.0 || g->angles[3 * j + 2] != 0.0) spherical = 1;
         double dmin = spherical ? fmin(fmin(g->dVoxelX, g->dVoxelY), g->dVoxelZ) : fmin(g->dVoxelX, g->dVoxelY);
         double cap = (2.0 * g->DSO[view] / dmin + crop) / acc;
 #pragma omp parallel for schedule(dynamic, 4)

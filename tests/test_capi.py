@@ -89,3 +89,101 @@ def test_front_end_checks():
         tigre_amd.Atb(np.zeros((3, 8, 8), dtype=np.float32), geo, np.zeros(2))
     with pytest.raises(TypeError):
         tigre_amd.Atb(np.zeros((2, 8, 8), dtype=np.float64), geo, np.zeros(2))
+
+
+# ---- the reference's C++ entry points ------------------------------------------
+REF_CUDA = "/root/reference/Common/CUDA"
+# One gateway per header, as the reference's own gateways include them (GD_TV.hpp
+# and PICCS.hpp share an include guard, so no unit can include both), with the
+# C++ expressions that name every function and GpuIds member it can reach.
+REF_GATEWAYS = {
+    "Siddon_projection.hpp": ["&siddon_ray_projection"],
+    "ray_interpolated_projection.hpp": ["&interpolation_projection"],
+    "Siddon_projection_parallel.hpp": ["&siddon_ray_projection_parallel"],
+    "ray_interpolated_projection_parallel.hpp": ["&interpolation_projection_parallel"],
+    "voxel_backprojection.hpp": ["&voxel_backprojection"],
+    "voxel_backprojection2.hpp": ["&voxel_backprojection2"],
+    "voxel_backprojection_parallel.hpp": ["&voxel_backprojection_parallel"],
+    "GD_TV.hpp": ["&pocs_tv"],
+    "GD_AwTV.hpp": ["&aw_pocs_tv"],
+    "PICCS.hpp": ["&piccs_tv"],
+    "tv_proximal.hpp": ["&tvdenoising"],
+    "gpuUtils.hpp": ["&GetGpuCount", "&GetGpuIdArray", "&GetGpuName"],
+    "GpuIds.hpp": ["&GpuIds::SetIds", "&GpuIds::GetLength", "&GpuIds::SetAllGpus", "&GpuIds::AreEqualDevices"],
+}
+
+
+@pytest.mark.parametrize("path", [_native.LIB_PATH, _native.LIB_PATH_MATLAB])
+def test_exports_reference_entry_points(path):
+    lib = ctypes.CDLL(path)
+    for name, mangled in _native.REFERENCE_ABI.items():
+        assert hasattr(lib, mangled), name
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_CUDA), reason="reference checkout not present")
+def test_gateway_built_on_reference_headers_links(tmp_path):
+    """A caller compiled against the reference's own headers (as the Cython and
+    MEX gateways are, Python build: IS_FOR_PYTIGRE) links against the library
+    with no undefined symbol, and the symbols it needs are the mangled names
+    the library's table lists."""
+    objs, undefined = [], set()
+    for k, (header, exprs) in enumerate(REF_GATEWAYS.items()):
+        uses = "\n".join("    { auto p = %s; keep((const void*)&p); }" % e for e in exprs)
+        extra = ""
+        if header == "GpuIds.hpp":  # constructor, destructor and both subscripts
+            extra = "    GpuIds ids;\n    (void)ids[0];\n    const GpuIds& c = ids;\n    (void)c[0];\n"
+        src = tmp_path / ("gateway%d.cpp" % k)
+        src.write_text('#include <cstddef>\n#include "%s"\nvoid keep(const void*);\nvoid use%d() {\n%s\n%s}\n'
+                       % (header, k, uses, extra))
+        obj = tmp_path / ("gateway%d.o" % k)
+        subprocess.run(["g++", "-std=c++17", "-DIS_FOR_PYTIGRE", "-I", REF_CUDA, "-c", str(src), "-o", str(obj)],
+                       check=True)
+        objs.append(str(obj))
+        undefined |= set(subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True,
+                                        check=True).stdout.split())
+    # what the gateways need beyond the C++ runtime and the helper
+    runtime = {"U", "_Z4keepPKv", "_GLOBAL_OFFSET_TABLE_", "__stack_chk_fail", "_Unwind_Resume",
+               "__gxx_personality_v0"}
+    assert undefined - runtime == set(_native.REFERENCE_ABI.values())
+    keep = tmp_path / "keep.cpp"
+    keep.write_text("void keep(const void*) {}\n")
+    libdir = os.path.dirname(_native.LIB_PATH)
+    subprocess.run(["g++", "-shared", "-fPIC", "-Wl,--no-undefined", *objs, str(keep), "-o",
+                    str(tmp_path / "libgateway.so"), "-L", libdir, "-ltigre_amd"], check=True)
+
+
+def test_gpu_ids_members():
+    """GpuIds members behave as GpuIds.cpp:6-60 describes (SetAllGpus lists
+    devices 0 .. n-1), with malloc'd ids released by the destructor."""
+    abi = _native.reference_abi()
+    g = _native.GpuIds()
+    abi["GpuIds::GpuIds"](ctypes.byref(g))
+    assert g.m_iCount == 0 and not g.m_piDeviceIds
+    ids = (ctypes.c_int * 3)(4, 1, 7)
+    abi["GpuIds::SetIds"](ctypes.byref(g), 3, ids)
+    assert abi["GpuIds::GetLength"](ctypes.byref(g)) == 3
+    assert [abi["GpuIds::operator[] const"](ctypes.byref(g), i) for i in range(3)] == [4, 1, 7]
+    abi["GpuIds::operator[]"](ctypes.byref(g), 1).contents.value = 9
+    assert g.m_piDeviceIds[1] == 9
+    abi["GpuIds::SetAllGpus"](ctypes.byref(g), 2)
+    assert g.m_iCount == 2 and [g.m_piDeviceIds[i] for i in range(2)] == [0, 1]
+    abi["GpuIds::~GpuIds"](ctypes.byref(g))
+    assert g.m_iCount == 0 and not g.m_piDeviceIds
+
+
+def test_reference_entry_points_reject_bad_arguments():
+    """Status codes instead of process exit: no views, and a zero voxel size."""
+    abi = _native.reference_abi()
+    geo = prepared(cone_geometry((8, 8, 8), (8.0, 8.0, 8.0), (8, 8), (1.0, 1.0), 100.0, 150.0), np.zeros(2))
+    cg = _native.CGeometry(geo, 2)
+    img = np.zeros((8, 8, 8), dtype=np.float32)
+    out = np.zeros((2, 8, 8), dtype=np.float32)
+    ang = np.zeros((2, 3), dtype=np.float32)
+    ptrs = (ctypes.c_void_p * 2)(out.ctypes.data, out.ctypes.data + 256)
+    ids = _native.GpuIds()
+    st = abi["siddon_ray_projection"](img.ctypes.data, cg.struct, ptrs, ang.ctypes.data, 0, ctypes.byref(ids))
+    assert st == 6 and "nangles" in _native.last_error()
+    cg.struct.dVoxelX = 0.0
+    st = abi["voxel_backprojection2"](out.ctypes.data, cg.struct, img.ctypes.data, ang.ctypes.data, 2,
+                                      ctypes.byref(ids))
+    assert st == 6 and "dVoxel" in _native.last_error()

@@ -303,3 +303,26 @@ def test_atb_fdk_windows(cuda, name):
     err = oracle.rel_l2(got, oracle.atb(proj, gp, "FDK"))
     print("Atb FDK windows %s rel-L2 %.3e" % (name, err))
     assert err <= TOL
+
+
+def test_interpolated_cap_flag_is_per_batch(cuda):
+    """The interpolated sample cap follows the reference's 9-view launch batch
+    (ray_interpolated_projection.cu:352-392): planar views batched with a tilted
+    one take the three-axis cap.  On voxels 400x finer in z than in x/y the two
+    caps give different ray sums (up to the whole ray); the HIP path matches the
+    oracle with and without the tilted view in the call."""
+    import torch
+
+    from tests.test_oracle_forward import _batch_geometry
+
+    geo, ang = _batch_geometry()
+    img = np.random.default_rng(9).uniform(0.5, 1.0, size=tuple(geo.nVoxel)).astype(np.float32)
+    d_img = torch.from_numpy(img).to(cuda)
+    for views in (ang, ang[1:]):
+        print("interpolated batch cap: %d views" % len(views), flush=True)
+        got = tigre_amd.Ax(d_img, geo, views, "interpolated")
+        torch.cuda.synchronize()
+        ref = oracle.ax(img, prepared(geo, views), "interpolated")
+        err = oracle.rel_l2(got.cpu().numpy(), ref)
+        print("Ax interpolated batch cap (%d views) rel-L2 %.3e" % (len(views), err))
+        assert err <= TOL

@@ -57,20 +57,30 @@ def test_atb_storage(cuda, dtype, btype):
     assert err <= TOL
 
 
+# Bounds of the final CGLS iterate (8 iterations) against the FP64 oracle's.
+# Float32 iterate with narrow working vectors: the north-star bound for BF16
+# (3e-3), and 1e-3 for FP16.  Narrow iterate: rounding x after each of the 8
+# updates adds up to 2^-8 (BF16) / 2^-11 (FP16) relative error each time;
+# measured 6.9e-3 / 8.6e-4, bounded at 1e-2 / 2e-3.
+CGLS_TOL = {"bfloat16": (3e-3, 1e-2), "float16": (1e-3, 2e-3)}
+
+
 @pytest.mark.parametrize("dtype", ["bfloat16", "float16"])
 def test_cgls_storage(cuda, dtype):
-    """CGLS with every vector stored narrow: the iterate against the FP64 oracle
-    CGLS, beside the float32-storage run (reported; bounded loosely, since the
-    rounding of each update accumulates over the iterations)."""
+    """CGLS with narrow storage against the FP64 oracle CGLS, beside the
+    float32-storage run: the default keeps the iterate in float32 (working
+    vectors narrow); narrow_iterate=True stores the iterate narrow too."""
     import torch
 
     geo, angles, img, gp = _case()
     proj = oracle.ax(img, gp).astype(np.float32)
     ref, _ = oalg.cgls(proj, geo, angles, 8)
     x32 = algs.cgls(proj, geo, angles, 8)
-    xn = algs.cgls(proj, geo, angles, 8, storage=getattr(torch, dtype))
-    e32 = oracle.rel_l2(x32, ref)
-    en = oracle.rel_l2(xn, ref)
-    print("CGLS 8 it: float32 storage rel-L2 %.3e, %s storage rel-L2 %.3e" % (e32, dtype, en))
+    xm = algs.cgls(proj, geo, angles, 8, storage=getattr(torch, dtype))
+    xn = algs.cgls(proj, geo, angles, 8, storage=getattr(torch, dtype), narrow_iterate=True)
+    e32, em, en = (oracle.rel_l2(x, ref) for x in (x32, xm, xn))
+    print("CGLS 8 it: float32 %.3e, %s working vectors + float32 iterate %.3e, all %s %.3e"
+          % (e32, dtype, em, dtype, en))
     assert e32 <= 1e-4
-    assert en <= 3e-2
+    assert em <= CGLS_TOL[dtype][0]
+    assert en <= CGLS_TOL[dtype][1]

@@ -86,3 +86,171 @@ def test_wang_weights_redundancy():
         assert pairs >= 8
         assert np.all(ww[t < -lim] == 0.0) and np.all(ww[t > lim] == 1.0)
         assert np.allclose(wang_weights(zgeo, theta), w, rtol=0, atol=1e-12)
+
+
+# -- OS-SART weights ------------------------------------------------------------------
+def _views_with_offsets(n, seed):
+    rng = np.random.default_rng(seed)
+    ang = np.stack([rng.uniform(0, 2 * np.pi, n), np.zeros(n), np.zeros(n)], 1)
+    return rng, ang
+
+
+def test_weights_w_is_inverse_chord():
+    """W = 1 / (Siddon projection of ones over the 2^3 enlarged volume)
+    (iterative_recon_alg.py:214-230): per ray, the inverse of its chord through
+    the box of size (max(sDetector_v, s_z), 1.1 s_y, 1.1 s_x) centred on the
+    view's offOrigin, and 0 (1 / inf) where the chord is at most min(dVoxel) / 2.
+    Chords from the independent world-frame ray/box model."""
+    from tests import refgeom
+
+    geo = cone_geometry((10, 12, 14), (10.0, 12.0, 14.0), (20, 24), (1.0, 1.0), 100.0, 160.0)
+    rng, ang = _views_with_offsets(6, 3)
+    geo.offOrigin = rng.uniform(-1, 1, size=(6, 3))
+    geo.offDetector = rng.uniform(-2, 2, size=(6, 2))
+    gp = prepared(geo, ang)
+    W = oalg.weights_w(gp, gp.angles)
+    s = np.asarray(gp.sVoxel, dtype=np.float64)
+    big = np.array([max(float(gp.sDetector[0]), s[0]), 1.1 * s[1], 1.1 * s[2]])[::-1]  # (x, y, z)
+    floor = min(np.asarray(gp.dVoxel, dtype=np.float64)) / 2
+    nV, nU = W.shape[1:]
+    checked = zeroed = 0
+    for i in range(len(ang)):
+        off = np.asarray(gp.offOrigin, dtype=np.float64).reshape(-1, 3)[i][::-1]
+        lo, hi = off - big / 2, off + big / 2
+        for v in range(0, nV, 3):
+            for u in range(0, nU, 3):
+                S, P = refgeom.source_and_pixel(gp, i, u, v)
+                chord = refgeom.segment_box_length(S, P, lo, hi)
+                expect = 1.0 / chord if chord > floor else 0.0
+                assert W[i, v, u] == pytest.approx(expect, rel=1e-6, abs=1e-12), (i, v, u)
+                checked += 1
+                zeroed += expect == 0.0
+    assert checked > 200 and zeroed < checked
+
+
+def test_weights_v_brute_force():
+    """V[block] = mean over z of the FDK backprojection of all-ones projections
+    over the volume shrunk by 0.9 max(s_y, s_x) / |(s_y, s_x)| on every axis
+    (iterative_recon_alg.py:232-270): per voxel, the sum over the block's views
+    of 1/U^2 times the bilinear sample of ones (zero border) at the voxel
+    centre's detector position, from the world-frame model."""
+    from tests import refgeom
+
+    geo = cone_geometry((5, 6, 7), (10.0, 12.0, 15.0), (14, 16), (1.0, 1.0), 60.0, 100.0)
+    _, ang = _views_with_offsets(6, 5)
+    blocks = [np.arange(0, 3), np.arange(3, 6)]
+    gp = prepared(geo, ang)
+    V = oalg.weights_v(gp, gp.angles, blocks)
+    s = np.asarray(gp.sVoxel, dtype=np.float64)
+    k = 0.9 * max(s[1], s[2]) / np.hypot(s[1], s[2])
+    gs = prepared(geo, ang)
+    gs.sVoxel = (s * k).astype(np.float32)
+    gs.dVoxel = (gs.sVoxel / np.asarray(gs.nVoxel)).astype(np.float32)
+    nz, ny, nx = [int(x) for x in gs.nVoxel]
+    nV, nU = int(gs.nDetector[0]), int(gs.nDetector[1])
+
+    def edge(t, n):
+        return min(max(min(t + 1.0, n - t), 0.0), 1.0)
+
+    for b, idx in enumerate(blocks):
+        for iy in range(ny):
+            for ix in range(nx):
+                acc = 0.0
+                for iz in range(nz):
+                    for i in idx:
+                        X = refgeom.voxel_centre(gs, i, iz, iy, ix)
+                        u, v = refgeom.project_point(gs, i, X)
+                        al = float(gs.angles[i][0])
+                        DSO = float(np.asarray(gs.DSO).reshape(-1)[i])
+                        U = (DSO - X[0] * np.cos(al) - X[1] * np.sin(al)) / DSO
+                        acc += edge(u, nU) * edge(v, nV) / U ** 2
+                assert V[b, iy, ix] == pytest.approx(acc / nz, rel=1e-6), (b, iy, ix)
+
+
+def test_weights_v_parallel_is_block_size():
+    """Parallel beam: V = the number of views of the block (iterative_recon_alg.py:265-267)."""
+    from tigre_amd.geometry import geometry
+
+    g = geometry(mode="parallel", nVoxel=np.array([4, 6, 8]))
+    ang = np.linspace(0, np.pi, 7, endpoint=False)
+    V = oalg.weights_v(g, ang, [np.arange(0, 5), np.arange(5, 7)])
+    assert V.shape == (2, 6, 8) and np.all(V[0] == 5.0) and np.all(V[1] == 2.0)
+
+
+# -- ASD-POCS ----------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def small_problem():
+    geo = cone_geometry((16, 16, 16), (16.0, 16.0, 16.0), (24, 24), (1.0, 1.0), 200.0, 300.0)
+    angles = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 4, seed=5)
+    proj = oracle.ax(img, prepared(geo, angles))
+    return geo, angles, img, proj
+
+
+def test_asd_pocs_without_tv_is_sart(small_problem):
+    """With no TV step (tviter = 0, or alpha = 0 so dtvg = 0), ASD-POCS is plain
+    SART: OS-SART with one view per subset, the same number of sweeps."""
+    geo, angles, _, proj = small_problem
+    sart = oalg.os_sart(proj, geo, angles, 3, blocksize=1)
+    for kw in ({"tviter": 0}, {"alpha": 0.0}):
+        trace = []
+        res = oalg.asd_pocs(proj, geo, angles, 3, trace=trace, **kw)
+        assert len(trace) == 3
+        assert np.array_equal(res, sart), kw
+
+
+def test_asd_pocs_tv_step_length(small_problem):
+    """Each TV step moves the image by dtvg |g| / sqrt(|g|^2 + 1e-7) = dtvg in L2
+    (GD_TV.cu:561-595), so with one TV step per iteration dg equals the step the
+    call used, and the first step is alpha * dp (pocs_algorithms.py:144-145);
+    with five steps dg is at most 5 dtvg."""
+    geo, angles, _, proj = small_problem
+    alpha = 0.05
+    trace = []
+    oalg.asd_pocs(proj, geo, angles, 4, alpha=alpha, tviter=1, trace=trace)
+    assert trace[0][5] == pytest.approx(alpha * trace[0][2], rel=1e-12)
+    for (_, _, _, dg, _, step) in trace:
+        assert dg == pytest.approx(step, rel=1e-6)
+    trace5 = []
+    oalg.asd_pocs(proj, geo, angles, 3, alpha=alpha, tviter=5, trace=trace5)
+    for (_, _, _, dg, _, step) in trace5:
+        assert step < dg <= 5 * step * (1 + 1e-9)
+
+
+@pytest.mark.parametrize("rmax,maxl2err,reduced", [(0.0, 0.0, True), (10.0, 0.0, False), (0.0, 1e30, False)])
+def test_asd_pocs_step_reduction_rule(small_problem, rmax, maxl2err, reduced):
+    """dtvg shrinks by alpha_red after an iteration exactly when dg > rmax dp and
+    dd > epsilon (pocs_algorithms.py:152-153).  The step of each iteration is
+    read back from dg (one TV step per iteration moves the image by the step);
+    dd and dp are recomputed from the iterates."""
+    geo, angles, _, proj = small_problem
+    trace, snaps = [], []
+    alpha_red = 0.7
+    oalg.asd_pocs(proj, geo, angles, 4, alpha=0.05, alpha_red=alpha_red, rmax=rmax, tviter=1,
+                  epsilon=maxl2err, trace=trace, snapshots=snaps)
+    gfull = prepared(geo, angles)
+    prev_post = np.zeros_like(snaps[0][0])
+    for k, ((_, dd, dp, dg, c, step), (pre, post)) in enumerate(zip(trace, snaps)):
+        assert dd == pytest.approx(np.linalg.norm(oracle.ax(pre.astype(np.float32), gfull) - proj), rel=1e-9)
+        assert dp == pytest.approx(np.linalg.norm(pre - prev_post), rel=1e-9)
+        assert dg == pytest.approx(np.linalg.norm(post - pre), rel=1e-9)
+        assert -1.0 - 1e-9 <= c <= 1.0 + 1e-9
+        prev_post = post
+        if k + 1 < len(trace):
+            ratio = trace[k + 1][3] / dg
+            assert ratio == pytest.approx(alpha_red if reduced else 1.0, rel=1e-5), (k, ratio)
+
+
+def test_asd_pocs_beta_stop(small_problem):
+    """beta = lmbda is multiplied by lmbda_red every iteration and the loop stops
+    once beta < 0.005 (pocs_algorithms.py:155-158): with lmbda_red = 0.5 that is
+    after 8 iterations (0.5^8 < 0.005 < 0.5^7), whatever niter allows; beta
+    never scales the SART update, so the first iterations equal those of a run
+    with lmbda_red = 1."""
+    geo, angles, _, proj = small_problem
+    t_half, t_one = [], []
+    oalg.asd_pocs(proj, geo, angles, 20, tviter=2, lmbda_red=0.5, trace=t_half)
+    oalg.asd_pocs(proj, geo, angles, 3, tviter=2, lmbda_red=1.0, trace=t_one)
+    assert len(t_half) == 8 and len(t_one) == 3
+    # (the oracle's OpenMP norm sum may round differently with another thread count)
+    assert np.allclose(np.array(t_half[:3], dtype=np.float64), np.array(t_one, dtype=np.float64), rtol=1e-12, atol=0)

@@ -159,3 +159,37 @@ def test_frames_closed_form():
     assert np.allclose(f[1], [0.0, 3.0 / 1.5, 0.0]) and np.allclose(f[2], [0.0, 0.0, -2.0])
     fc = oracle.forward_frame(gp, 0, 1)
     assert np.allclose(fc[3] - f[3], [-0.5, -0.5, -0.5])
+
+
+def _batch_geometry():
+    """Voxels 400x finer in z than in x/y: the planar sample cap
+    (2 DSO / min(dVx, dVy) + crop) / accuracy ends inside the volume for the
+    steep rows, the three-axis cap (min over dVz as well) beyond it."""
+    g = cone_geometry((400, 8, 8), (2.0, 16.0, 16.0), (16, 8), (0.2, 3.0), 100.0, 150.0)
+    ang = np.zeros((9, 3))
+    ang[:, 0] = np.linspace(0.1, 1.7, 9)
+    ang[0, 1] = 0.02   # only the first view of the batch leaves the circular orbit
+    return g, ang
+
+
+def test_interpolated_cap_flag_is_per_batch():
+    """The interpolated projector picks its sample cap with one flag per launch
+    batch of 9 views (ray_interpolated_projection.cu:352-392): planar views that
+    share a batch with a tilted view get the three-axis cap, which here lies
+    beyond the volume, so they equal the uncapped ray sums; the same views
+    projected without the tilted view get the planar cap, which cuts the steep
+    rows short."""
+    g, ang = _batch_geometry()
+    img = np.random.default_rng(9).uniform(0.5, 1.0, size=tuple(g.nVoxel)).astype(np.float32)
+    with_tilted = oracle.ax(img, prepared(g, ang), "interpolated")
+    alone = oracle.ax(img, prepared(g, ang[1:]), "interpolated")
+    gp = prepared(g, ang)
+    for i in (1, 4, 8):
+        for v in (0, 3, 8, 15):
+            for u in (0, 3, 7):
+                ref = _interp_reference(img, gp, i, u, v)
+                assert with_tilted[i, v, u] == pytest.approx(ref, rel=1e-9), (i, v, u)
+    assert np.all(alone <= with_tilted[1:] * (1 + 1e-12))
+    cut = (with_tilted[1:] - alone) / with_tilted[1:]
+    assert cut.max() > 0.05          # steep rows lose samples inside the volume
+    assert cut[:, 7:9, :].max() < 1e-12  # the central rows keep all of theirs

@@ -133,6 +133,82 @@ SIGNATURES = {
     "tigre_version": (ctypes.c_char_p, []),
 }
 
+# The reference's C++ entry points (include/tigre_reference_abi.hpp), by their
+# Itanium-mangled names: what a gateway compiled against the reference headers
+# links to.  Geometry travels by value (a TigreGeometry), GpuIds by reference.
+REFERENCE_ABI = {
+    "siddon_ray_projection": "_Z21siddon_ray_projectionPf8GeometryPS_PKfiRK6GpuIds",
+    "interpolation_projection": "_Z24interpolation_projectionPf8GeometryPS_PKfiRK6GpuIds",
+    "siddon_ray_projection_parallel": "_Z30siddon_ray_projection_parallelPf8GeometryPS_PKfiRK6GpuIds",
+    "interpolation_projection_parallel": "_Z33interpolation_projection_parallelPf8GeometryPS_PKfiRK6GpuIds",
+    "voxel_backprojection": "_Z20voxel_backprojectionPf8GeometryS_PKfiRK6GpuIds",
+    "voxel_backprojection2": "_Z21voxel_backprojection2Pf8GeometryS_PKfiRK6GpuIds",
+    "voxel_backprojection_parallel": "_Z29voxel_backprojection_parallelPf8GeometryS_PKfiRK6GpuIds",
+    "pocs_tv": "_Z7pocs_tvPfS_fPKliRK6GpuIds",
+    "aw_pocs_tv": "_Z10aw_pocs_tvPfS_fPKlifRK6GpuIds",
+    "piccs_tv": "_Z8piccs_tvPKfS0_PfffPKliRK6GpuIds",
+    "tvdenoising": "_Z11tvdenoisingPfS_fPKfPKliRK6GpuIds",
+    "GetGpuCount": "_Z11GetGpuCountv",
+    "GetGpuIdArray": "_Z13GetGpuIdArrayPKcPiiPc",
+    "GetGpuName": "_Z10GetGpuNameiPc",
+    "GpuIds::GpuIds": "_ZN6GpuIdsC1Ev",
+    "GpuIds::~GpuIds": "_ZN6GpuIdsD1Ev",
+    "GpuIds::SetIds": "_ZN6GpuIds6SetIdsEiPi",
+    "GpuIds::GetLength": "_ZNK6GpuIds9GetLengthEv",
+    "GpuIds::SetAllGpus": "_ZN6GpuIds10SetAllGpusEi",
+    "GpuIds::operator[]": "_ZN6GpuIdsixEi",
+    "GpuIds::operator[] const": "_ZNK6GpuIdsixEi",
+    "GpuIds::AreEqualDevices": "_ZNK6GpuIds15AreEqualDevicesEv",
+}
+# The same entry points with the MATLAB detector layout (v fastest), as the
+# reference's MEX build compiles them.
+LIB_PATH_MATLAB = os.path.join(os.path.dirname(LIB_PATH), "libtigre_amd_matlab.so")
+
+
+class GpuIds(ctypes.Structure):
+    """Mirror of the reference's struct GpuIds (GpuIds.hpp:3-14)."""
+
+    _fields_ = [("m_piDeviceIds", ctypes.POINTER(ctypes.c_int)), ("m_iCount", ctypes.c_int)]
+
+
+def reference_abi(path=None):
+    """The reference-signature entry points of the library (or of the MATLAB-layout
+    build), as ctypes functions keyed by their C++ names."""
+    L = ctypes.CDLL(path or LIB_PATH)
+    vp, ip = ctypes.c_void_p, ctypes.c_int
+    geo, ids = TigreGeometry, ctypes.POINTER(GpuIds)
+    sig = {
+        "siddon_ray_projection": (ip, [vp, geo, vp, vp, ip, ids]),
+        "interpolation_projection": (ip, [vp, geo, vp, vp, ip, ids]),
+        "siddon_ray_projection_parallel": (ip, [vp, geo, vp, vp, ip, ids]),
+        "interpolation_projection_parallel": (ip, [vp, geo, vp, vp, ip, ids]),
+        "voxel_backprojection": (ip, [vp, geo, vp, vp, ip, ids]),
+        "voxel_backprojection2": (ip, [vp, geo, vp, vp, ip, ids]),
+        "voxel_backprojection_parallel": (ip, [vp, geo, vp, vp, ip, ids]),
+        "pocs_tv": (None, [vp, vp, ctypes.c_float, vp, ip, ids]),
+        "aw_pocs_tv": (None, [vp, vp, ctypes.c_float, vp, ip, ctypes.c_float, ids]),
+        "piccs_tv": (None, [vp, vp, vp, ctypes.c_float, ctypes.c_float, vp, ip, ids]),
+        "tvdenoising": (None, [vp, vp, ctypes.c_float, vp, vp, ip, ids]),
+        "GetGpuCount": (ip, []),
+        "GetGpuIdArray": (ip, [ctypes.c_char_p, vp, ip, vp]),
+        "GetGpuName": (None, [ip, vp]),
+        "GpuIds::GpuIds": (None, [ids]),
+        "GpuIds::~GpuIds": (None, [ids]),
+        "GpuIds::SetIds": (None, [ids, ip, vp]),
+        "GpuIds::GetLength": (ip, [ids]),
+        "GpuIds::SetAllGpus": (None, [ids, ip]),
+        "GpuIds::operator[]": (ctypes.POINTER(ctypes.c_int), [ids, ip]),
+        "GpuIds::operator[] const": (ip, [ids, ip]),
+        "GpuIds::AreEqualDevices": (ctypes.c_bool, [ids]),
+    }
+    out = {}
+    for name, mangled in REFERENCE_ABI.items():
+        fn = getattr(L, mangled)
+        fn.restype, fn.argtypes = sig[name]
+        out[name] = fn
+    return out
+
+
 _lib = None
 
 

@@ -58,6 +58,9 @@ class ViewSets:
         self.geo = copy.deepcopy(geo)
         self.geo.check_geo(angles)
         self.angles = self.geo.angles
+        # the reference's Ax / Atb pick the parallel-beam operators from
+        # geo.mode (_Ax.pyx:67-87, _Atb.pyx:56-76)
+        self.parallel = getattr(self.geo, "mode", "cone") == "parallel"
         self.sets = []
         for idx in blocks:
             idx = np.asarray(idx, dtype=np.int64).reshape(-1)
@@ -81,9 +84,10 @@ class ViewSets:
         idx, g, cg, ang = self.sets[j]
         if out is None:
             out = torch.empty((len(idx),) + self.det_shape, dtype=img.dtype, device=img.device)
-        st = self.lib.tigre_ax_device_typed(ctypes.c_void_p(img.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
-                                            ang.ctypes.data_as(ctypes.c_void_p), len(idx), kind,
-                                            _native.LAYOUT_PYTHON, _storage_code(img), _stream_handle())
+        fn = self.lib.tigre_ax_parallel_device if self.parallel else self.lib.tigre_ax_device_typed
+        st = fn(ctypes.c_void_p(img.data_ptr()), cg.ptr(), ctypes.c_void_p(out.data_ptr()),
+                ang.ctypes.data_as(ctypes.c_void_p), len(idx), kind, _native.LAYOUT_PYTHON, _storage_code(img),
+                _stream_handle())
         _native.check(st, "TIGRE: Call to Ax failed")
         return out
 
@@ -93,10 +97,17 @@ class ViewSets:
         idx, g, cg, ang = self.sets[j]
         if out is None:
             out = torch.empty(self.vol_shape, dtype=proj.dtype, device=proj.device)
-        st = self.lib.tigre_atb_device_typed(ctypes.c_void_p(proj.data_ptr()), cg.ptr(),
-                                             ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p),
-                                             len(idx), kind, _native.LAYOUT_PYTHON, _storage_code(proj),
-                                             _stream_handle())
+        if self.parallel:
+            # one parallel backprojector serves 'FDK' and 'matched' (_Atb.pyx:56-76)
+            st = self.lib.tigre_atb_parallel_device(ctypes.c_void_p(proj.data_ptr()), cg.ptr(),
+                                                    ctypes.c_void_p(out.data_ptr()),
+                                                    ang.ctypes.data_as(ctypes.c_void_p), len(idx),
+                                                    _native.LAYOUT_PYTHON, _storage_code(proj), _stream_handle())
+        else:
+            st = self.lib.tigre_atb_device_typed(ctypes.c_void_p(proj.data_ptr()), cg.ptr(),
+                                                 ctypes.c_void_p(out.data_ptr()), ang.ctypes.data_as(ctypes.c_void_p),
+                                                 len(idx), kind, _native.LAYOUT_PYTHON, _storage_code(proj),
+                                                 _stream_handle())
         _native.check(st, "Atb:")
         return out
 
@@ -263,8 +274,13 @@ class _SartState:
         return 1.0 / W
 
     def _weights_v(self):
-        """V[block] = mean over z of Atb(ones) on a shrunk volume (iterative_recon_alg.py:232-270)."""
+        """V[block] = mean over z of Atb(ones) on a shrunk volume (iterative_recon_alg.py:232-270);
+        for a parallel beam V[block] = the number of views in the block (:265-267)."""
         torch = _torch()
+        if self.views.parallel:
+            nY, nX = self.views.vol_shape[1:]
+            return torch.stack([torch.full((nY, nX), float(len(idx)), dtype=torch.float32, device="cuda")
+                                for idx in self.blocks])
         g = copy.deepcopy(self.geo)
         s = np.asarray(g.sVoxel, dtype=np.float64)
         g.sVoxel = s * np.max(s[1:] / np.linalg.norm(s[1:])) * 0.9
@@ -308,10 +324,11 @@ def _sart_family(proj, geo, angles, niter, blocksize, **kwargs):
                     lmbda=kwargs.get("lmbda", 1.0), noneg=kwargs.get("noneg", True),
                     verbose=kwargs.get("verbose", False))
     l2 = []
-    lred = kwargs.get("lmbda_red", 1.0)
+    # lmbda_red is accepted and, as in the reference, not applied: its SART
+    # family stores it (iterative_recon_alg.py:143) but every update uses the
+    # unreduced lmbda (:389-399)
     for _ in range(niter):
         st.sweep()
-        st.lmbda *= lred
         if kwargs.get("computel2", False):
             l2.append(st.residual_norm())
     res = _result(st.res, kwargs.get("return_tensor", False))
@@ -341,21 +358,27 @@ def sirt(proj, geo, angles, niter, **kwargs):
 # CGLS (krylov_subspace_algorithms.py:22-97)
 # ---------------------------------------------------------------------------
 def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=False, computel2=False, init=None,
-         storage=None, **kwargs):
+         storage=None, narrow_iterate=False, **kwargs):
     """Conjugate gradient least squares with Siddon forward and matched
     backprojection (krylov_subspace_algorithms.py:22-97); FP64 inner products
     and norms; restarts once on residual growth.
 
     ``storage`` (a torch dtype: float32 by default, bfloat16 or float16) is the
-    type every volume and sinogram of the iteration is stored in; the operators
-    and the vector updates compute in float32 and round once on store."""
+    type the search direction, its backprojection and the sinograms (data,
+    residual, A p) are stored in; the operators and the vector updates compute
+    in float32 and round once on store.  The iterate x stays float32 unless
+    ``narrow_iterate`` is set: an iterate rounded to a 7-bit significand after
+    every update carries that rounding into the result (each of the niter
+    updates adds a relative error of up to 2^-8), which a float32 iterate
+    avoids at 2 extra bytes per voxel."""
     torch = _torch()
     dt = torch.float32 if storage is None else storage
+    xt = dt if narrow_iterate else torch.float32
     g = copy.deepcopy(geo)
     g.check_geo(angles)
     vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
     b = _to_device(proj).to(dt)
-    x = torch.zeros(vs.vol_shape, dtype=dt, device="cuda") if init is None else _to_device(init).to(dt)
+    x = torch.zeros(vs.vol_shape, dtype=xt, device="cuda") if init is None else _to_device(init).to(xt)
 
     # Narrow storage keeps the search direction normalised, p = n_p p_hat with
     # the scalar n_p in FP64: A p can exceed the float16 range (A^T A has a
@@ -363,8 +386,12 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
     # form in exact arithmetic.
     scaled = dt != torch.float32
 
+    def residual():
+        # b - A x, rounded once to the storage type
+        return torch.sub(b, vs.ax(x, 0)).to(dt)
+
     def start():
-        r = torch.sub(b, vs.ax(x, 0))
+        r = residual()
         p = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
         n_p = _norm(p)
         if scaled:
@@ -381,7 +408,7 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
         alpha = gamma / ((n_p * _norm(q)) ** 2)
         step = alpha * n_p
         torch.add(x, p, alpha=step, out=x)
-        l2[0, i] = _norm(torch.sub(b, vs.ax(x, 0)))
+        l2[0, i] = _norm(residual())
         if i > 0 and l2[0, i] > l2[0, i - 1]:
             torch.sub(x, p, alpha=step, out=x)
             if verbose:

@@ -39,12 +39,9 @@ class _Case:
                 if tuner is not None else (4, 8, 16)
         self.params = tuple(params)
         self.plans = [Plan(_sub_geometry(st.geo, idx), st.angles[idx], *self.params) for idx in self.blocks]
-        # lambda / V_j for every iteration, the expression the eager loop evaluates
-        self.inv_v = []
-        lam = lmbda
-        for _ in range(niter):
-            self.inv_v.append([(lam / st.V[j]).contiguous() for j in range(len(self.blocks))])
-            lam *= lmbda_red
+        # lambda / V_j, the expression the eager loop evaluates; lmbda_red is not
+        # applied, as in the reference's SART family (iterative_recon_alg.py:143, :389-399)
+        self.inv_v = [(lmbda / st.V[j]).contiguous() for j in range(len(self.blocks))]
         self.ws = torch.empty(max(max(p.workspace_bytes() for p in self.plans), 256), dtype=torch.uint8,
                               device="cuda")
         nmax = max(len(b) for b in self.blocks)
@@ -58,7 +55,7 @@ class _Case:
             ax = self.axbuf[: p.nviews]
             p.ax(self.res, ax, self.ws, _native.PROJ_SIDDON)
             sart_residual(self.proj_blocks[j], self.W_blocks[j], ax)
-            p.sart_update(ax, self.res, self.inv_v[it][j], self.ws, self.noneg, _native.BACKPROJ_FDK)
+            p.sart_update(ax, self.res, self.inv_v[j], self.ws, self.noneg, _native.BACKPROJ_FDK)
 
 
 class BatchedOSSART:
@@ -83,6 +80,11 @@ class BatchedOSSART:
 
     def results(self):
         return [c.res for c in self.cases]
+
+    def run(self):
+        """Every case's ``niter`` iterations from a zero image, on the current stream."""
+        self.reset()
+        return self.run_sequential()
 
     def run_sequential(self):
         """All iterations of case 0, then case 1, ... from the host, on the current stream."""

@@ -151,6 +151,36 @@ bool backward_is_axial(const HostGeometry& g, const std::vector<BwdView>& v);
 int select_device(const int* gpuids, int ngpus);
 int upload_async(void** d_ptr, const void* host, size_t bytes, hipStream_t stream);
 
+// Stream-ordered scratch of one call: every block allocated through it is
+// released with hipFreeAsync on the same stream when the guard goes out of
+// scope, on success and on every early error return alike.
+class StreamScratch {
+  public:
+    explicit StreamScratch(hipStream_t stream) : stream_(stream) {}
+    ~StreamScratch() {
+        for (void* p : blocks_) (void)hipFreeAsync(p, stream_);
+    }
+    StreamScratch(const StreamScratch&) = delete;
+    StreamScratch& operator=(const StreamScratch&) = delete;
+    template <typename T>
+    int alloc(T** p, size_t bytes) {
+        void* q = nullptr;
+        TIGRE_HIP_TRY(hipMallocAsync(&q, bytes, stream_));
+        blocks_.push_back(q);
+        *p = static_cast<T*>(q);
+        return TIGRE_OK;
+    }
+    int upload(void** p, const void* host, size_t bytes) {
+        TIGRE_CHECK(upload_async(p, host, bytes, stream_));
+        blocks_.push_back(*p);
+        return TIGRE_OK;
+    }
+
+  private:
+    hipStream_t stream_;
+    std::vector<void*> blocks_;
+};
+
 // Optional HIP-event timing of kernel launches (TIGRE_KERNEL_* ids).  The
 // scope records a start event on construction and a stop event on
 // destruction; both are no-ops while timing is disabled.

@@ -8,6 +8,7 @@
 //              with the angles negated (voxel_backprojection.cu:510-512).
 // Here every rotation is built once per view as a 3x3 matrix in FP64 and
 // applied to the detector corner/steps, instead of rotating points one by one.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -201,8 +202,17 @@ void forward_views(const HostGeometry& g, bool centre, std::vector<FwdView>* out
         }
         if (maxdist < 0.0) maxdist = 0.0;
         const double crop = std::floor(maxdist);
-        const double dmin = planar ? std::fmin(g.dVoxel[0], g.dVoxel[1])
-                                   : std::fmin(std::fmin(g.dVoxel[0], g.dVoxel[1]), g.dVoxel[2]);
+        // The cap's voxel size comes from a flag of the whole launch batch: the
+        // reference projects 9 views per kernel and takes the three-axis minimum
+        // for all of them when any view of the batch has theta or psi nonzero
+        // (ray_interpolated_projection.cu:352-392, PROJ_PER_BLOCK = 9); crop
+        // above is per view.
+        bool batch_spherical = false;
+        const int b0 = (i / 9) * 9, b1 = std::min(b0 + 9, g.nangles);
+        for (int j = b0; j < b1; ++j)
+            if (g.angles[3 * j + 1] != 0.0 || g.angles[3 * j + 2] != 0.0) batch_spherical = true;
+        const double dmin = batch_spherical ? std::fmin(std::fmin(g.dVoxel[0], g.dVoxel[1]), g.dVoxel[2])
+                                            : std::fmin(g.dVoxel[0], g.dVoxel[1]);
         r.first = std::floor(crop / g.accuracy);
         // parallel beam: the cap uses dVoxelX (ray_interpolated_projection_parallel.cu:175-176)
         r.cap = (2.0 * DSO / (g.parallel ? g.dVoxel[0] : dmin) + crop) / g.accuracy;

@@ -153,30 +153,23 @@ int ax_device(const void* d_img, const HostGeometry& g, void* d_proj_out, int pr
     TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
     TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
     const bool interp = projection_type == TIGRE_PROJ_INTERPOLATED;
+    StreamScratch scratch(stream);
     // narrow storage: the kernels write float projections, rounded at the end
     const long long nproj = (long long)g.nDetec[0] * g.nDetec[1] * g.nangles;
     float* d_proj = static_cast<float*>(d_proj_out);
-    if (dtype != TIGRE_DTYPE_F32)
-        TIGRE_HIP_TRY(hipMallocAsync((void**)&d_proj, (size_t)nproj * sizeof(float), stream));
+    if (dtype != TIGRE_DTYPE_F32) TIGRE_CHECK(scratch.alloc(&d_proj, (size_t)nproj * sizeof(float)));
     ForwardRecords rec;
     build_forward_records(g, interp, 4, &rec);
     void* d_views = nullptr;
     void* d_groups = nullptr;
-    TIGRE_CHECK(upload_async(&d_views, rec.ordered.data(), rec.ordered.size() * sizeof(FwdView), stream));
-    if (rec.ngroups() > 0)
-        TIGRE_CHECK(upload_async(&d_groups, rec.groups.data(), rec.groups.size() * sizeof(int), stream));
+    TIGRE_CHECK(scratch.upload(&d_views, rec.ordered.data(), rec.ordered.size() * sizeof(FwdView)));
+    if (rec.ngroups() > 0) TIGRE_CHECK(scratch.upload(&d_groups, rec.groups.data(), rec.groups.size() * sizeof(int)));
     char* ws = nullptr;
     const size_t wsb = ax_workspace_bytes(g, rec, interp);
-    if (wsb) TIGRE_HIP_TRY(hipMallocAsync((void**)&ws, wsb, stream));
+    if (wsb) TIGRE_CHECK(scratch.alloc(&ws, wsb));
     TIGRE_CHECK(ax_launch(d_img, dtype, g, rec, (const FwdView*)d_views, (const int*)d_groups, projection_type, layout,
                           d_proj, ws, 8, stream));
-    if (ws) TIGRE_HIP_TRY(hipFreeAsync(ws, stream));
-    if (d_groups) TIGRE_HIP_TRY(hipFreeAsync(d_groups, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
-    if (dtype != TIGRE_DTYPE_F32) {
-        TIGRE_CHECK(launch_store_converted(d_proj, d_proj_out, nproj, dtype, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(d_proj, stream));
-    }
+    if (dtype != TIGRE_DTYPE_F32) TIGRE_CHECK(launch_store_converted(d_proj, d_proj_out, nproj, dtype, stream));
     return TIGRE_OK;
 }
 
@@ -201,24 +194,20 @@ int atb_device(const void* d_proj, const HostGeometry& g, void* d_vol_out, int b
                   "unknown backprojection type");
     TIGRE_REQUIRE(layout == TIGRE_LAYOUT_PYTHON || layout == TIGRE_LAYOUT_MATLAB, "unknown detector layout");
     TIGRE_REQUIRE(valid_dtype(dtype), "unknown storage type");
+    StreamScratch scratch(stream);
     const long long nvox = (long long)g.nVoxel[0] * g.nVoxel[1] * g.nVoxel[2];
     float* d_vol = static_cast<float*>(d_vol_out);
-    if (dtype != TIGRE_DTYPE_F32) TIGRE_HIP_TRY(hipMallocAsync((void**)&d_vol, (size_t)nvox * sizeof(float), stream));
+    if (dtype != TIGRE_DTYPE_F32) TIGRE_CHECK(scratch.alloc(&d_vol, (size_t)nvox * sizeof(float)));
     std::vector<BwdView> views;
     backward_views(g, &views);
     const bool axial = backward_is_axial(g, views);
     void* d_views = nullptr;
-    TIGRE_CHECK(upload_async(&d_views, views.data(), views.size() * sizeof(BwdView), stream));
+    TIGRE_CHECK(scratch.upload(&d_views, views.data(), views.size() * sizeof(BwdView)));
     char* ws = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&ws, atb_workspace_bytes(g), stream));
+    TIGRE_CHECK(scratch.alloc(&ws, atb_workspace_bytes(g)));
     TIGRE_CHECK(atb_launch(d_proj, dtype, g, (const BwdView*)d_views, axial, backprojection_type, layout, d_vol, ws,
                            16, AtbUpdate{}, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(ws, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(d_views, stream));
-    if (dtype != TIGRE_DTYPE_F32) {
-        TIGRE_CHECK(launch_store_converted(d_vol, d_vol_out, nvox, dtype, stream));
-        TIGRE_HIP_TRY(hipFreeAsync(d_vol, stream));
-    }
+    if (dtype != TIGRE_DTYPE_F32) TIGRE_CHECK(launch_store_converted(d_vol, d_vol_out, nvox, dtype, stream));
     return TIGRE_OK;
 }
 

@@ -366,13 +366,14 @@ int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp
     TIGRE_REQUIRE(tiles < (1LL << 31), "image too large");
     const int nblocks = (int)tiles;
     const dim3 grid((unsigned)nblocks);
+    StreamScratch scratch(stream);
     float* buf[2] = {nullptr, nullptr};
     double* partial = nullptr;
     float* scale = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&buf[0], n * sizeof(float), stream));
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&buf[1], n * sizeof(float), stream));
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&partial, (size_t)nblocks * sizeof(double), stream));
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&scale, 256, stream));
+    TIGRE_CHECK(scratch.alloc(&buf[0], n * sizeof(float)));
+    TIGRE_CHECK(scratch.alloc(&buf[1], n * sizeof(float)));
+    TIGRE_CHECK(scratch.alloc(&partial, (size_t)nblocks * sizeof(double)));
+    TIGRE_CHECK(scratch.alloc(&scale, 256));
     const float* src = img;
     int cur = 0;
     for (int it = 0; it < maxiter; ++it) {
@@ -389,10 +390,6 @@ int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp
         cur ^= 1;
     }
     if (src != dst) TIGRE_HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(buf[0], stream));
-    TIGRE_HIP_TRY(hipFreeAsync(buf[1], stream));
-    TIGRE_HIP_TRY(hipFreeAsync(partial, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(scale, stream));
     return TIGRE_OK;
 }
 }  // namespace
@@ -476,8 +473,9 @@ int run_tvdenoise(const float* src, float* dst, float lambda, const float* spaci
                   int maxiter, hipStream_t stream) {
     const long D = nz + 2, sl = nx * ny;
     const size_t n = (size_t)D * sl;
+    StreamScratch scratch(stream);
     float* buf = nullptr;
-    TIGRE_HIP_TRY(hipMallocAsync((void**)&buf, 5 * n * sizeof(float), stream));
+    TIGRE_CHECK(scratch.alloc(&buf, 5 * n * sizeof(float)));
     float *f = buf, *u = buf + n, *px = buf + 2 * n, *py = buf + 3 * n, *pz = buf + 4 * n;
     long blocks = ((long)n + 255) / 256;
     if (blocks > 16384) blocks = 16384;
@@ -496,7 +494,6 @@ int run_tvdenoise(const float* src, float* dst, float lambda, const float* spaci
     }
     TIGRE_HIP_TRY(hipGetLastError());
     TIGRE_HIP_TRY(hipMemcpyAsync(dst, u + sl, (size_t)nz * sl * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    TIGRE_HIP_TRY(hipFreeAsync(buf, stream));
     return TIGRE_OK;
 }
 

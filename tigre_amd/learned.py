@@ -211,32 +211,59 @@ def tv_value(x, eps=1e-8):
 class LearnedProx:
     """ASD-POCS ``regularizer``: the learned step, with plain TV as fallback.
 
-    The learned result is kept when its TV objective is below the input's and,
-    if ``residual`` (a callable image -> data residual norm) is given, its data
-    residual does not exceed the input's by more than ``tol``; otherwise the
-    step is recomputed with minTV(x, dtvg, tviter).  ``taken`` / ``fallbacks``
-    count the two outcomes."""
+    The learned result y = net(x) (``step_scaled``: x + dtvg F(x), the form
+    trained to imitate one TV call) is kept when
 
-    def __init__(self, net, tviter=20, residual=None, tol=1e-3):
+    * its TV objective is below that of the step's input x, and
+    * if ``residual`` (a callable image -> data residual norm |A y - b|) is
+      given, its data residual does not exceed the data residual of the
+      previous step's output by more than the fraction ``tol`` (the first step
+      has no previous output and passes);
+
+    otherwise the step is minTV(x, dtvg, tviter), plain TV.  ``taken`` /
+    ``fallbacks`` count the two outcomes."""
+
+    def __init__(self, net, tviter=20, residual=None, tol=1e-3, step_scaled=False):
         self.net = net
         self.tviter = tviter
         self.residual = residual
         self.tol = tol
+        self.step_scaled = step_scaled
         self.taken = 0
         self.fallbacks = 0
+        self.last_residual = None
 
     def __call__(self, img, dtvg):
         from .tv import minTV
 
-        y = self.net.forward(img, scale=float(dtvg))
+        y = self.net.forward(img, scale=float(dtvg) if self.step_scaled else 1.0)
         ok = tv_value(y) < tv_value(img)
+        r = None
         if ok and self.residual is not None:
-            ok = self.residual(y) <= self.residual(img) * (1.0 + self.tol)
+            r = self.residual(y)
+            ok = self.last_residual is None or r <= self.last_residual * (1.0 + self.tol)
         if ok:
             self.taken += 1
+            self.last_residual = r
             return y
         self.fallbacks += 1
-        return minTV(img, dtvg, self.tviter)
+        out = minTV(img, dtvg, self.tviter)
+        if self.residual is not None:
+            self.last_residual = self.residual(out)
+        return out
+
+
+def data_residual(proj, geo, angles, projection_type="Siddon"):
+    """Callable image -> |Ax(image) - proj| (FP64 norm) over all views, for the
+    fallback test of :class:`LearnedProx`."""
+    import numpy as np
+
+    from .algorithms import ViewSets, _norm, _to_device
+
+    vs = ViewSets(geo, angles, [np.arange(len(angles))])
+    b = _to_device(proj)
+    kind = _native.PROJ_INTERPOLATED if projection_type == "interpolated" else _native.PROJ_SIDDON
+    return lambda img: _norm(vs.ax(img.contiguous(), 0, kind) - b)
 
 
 # ---- data for training and evaluation ----------------------------------------
@@ -265,8 +292,30 @@ def collect_pairs(proj, geo, angles, niter, tviter=20, alpha=0.002, **kwargs):
     return pairs
 
 
+def collect_far_pairs(proj, geo, angles, niter, first, tviter=20, alpha=0.002, **kwargs):
+    """Pairs a plain ASD-POCS run generates further ahead: (pre-TV image of
+    iteration k, the run's final post-TV image after ``niter`` iterations, 1.0)
+    for k = 1 .. ``first``.  Used with make_patches, the target of the crop of
+    x_k is the crop of the final image, so the network learns to take an early
+    iterate where plain ASD-POCS arrives many TV steps and data sweeps later."""
+    from .algorithms import asd_pocs
+    from .tv import minTV
+
+    early = []
+
+    def recording(img, step):
+        if len(early) < first:
+            early.append(img.clone())
+        return minTV(img, step, tviter)
+
+    final = asd_pocs(proj, geo, angles, niter, alpha=alpha, tviter=tviter, regularizer=recording, return_tensor=True,
+                     **kwargs)
+    return [(x, final, 1.0) for x in early]
+
+
 def make_patches(pairs, count, size, seed=0):
-    """``count`` random size^3 crops: inputs x and training targets x + (TV(x) - x) / dtvg."""
+    """``count`` random size^3 crops: inputs x and training targets x + (post - x) / step
+    (post = minimizeTV(x, dtvg) with step = dtvg, or a later iterate with step = 1)."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     xs, ts = [], []
