@@ -91,31 +91,35 @@ def main():
                     help="pair: OS-SART iterations (20-view subsets, lambda 1) after the timed region (0 = skip)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this size once (counter calibration)")
-    ap.add_argument("--workload", choices=["pair", "small", "batched", "learned", "precision"], default="pair",
+    ap.add_argument("--calib-valu", action="store_true",
+                    help="after the timed region, run the SART residual kernel once with a known VALU count "
+                         "(8192 workgroups x 4 wavefronts, 16 loop iterations per lane: 216 VALU per wavefront)")
+    ap.add_argument("--workload", choices=["pair", "small", "batched", "learned", "precision", "tv"], default="pair",
                     help="pair: Ax+Atb at 512^3 (default); batched: 64 uneven OS-SART cases in one graph; "
                          "learned: ASD-POCS with the learned proximal step vs plain TV at 256^3; "
                          "precision: BF16 / FP16 / FP32 storage at 1024^3 with CGLS; "
                          "small: every operator at 128^3 beside the full FP64 oracle")
     ap.add_argument("--train-steps", type=int, default=1000, help="learned: Adam steps")
+    ap.add_argument("--cgls-iters", type=int, default=30, help="precision: CGLS iterations per storage type")
+    ap.add_argument("--oracle-views", type=int, default=8, help="precision: views checked against the FP64 oracle")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="pair: skip the compact 1024^3 precision, 64-case batched and 256^3 learned summaries")
     ap.add_argument("--cases", type=int, default=64, help="batched: number of cases")
     ap.add_argument("--iters", type=int, default=10, help="batched: OS-SART iterations per case")
-    ap.add_argument("--streams", type=int, default=8, help="batched: graph branches (streams)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batched":
-        bench_batched(args)
-        return
-    if args.workload == "learned":
-        bench_learned(args)
-        return
-    if args.workload == "precision":
-        bench_precision(args)
+    if args.workload in ("batched", "learned", "precision"):
+        fn = {"batched": bench_batched, "learned": bench_learned, "precision": bench_precision}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
         return
     if args.workload == "small":
         bench_small(args)
+        return
+    if args.workload == "tv":
+        bench_tv(args)
         return
 
     import numpy as np
@@ -126,6 +130,7 @@ def main():
     import tigre_amd
     from tigre_amd import _native
     from tigre_amd.geometry import cone_geometry, prepared
+    from tigre_amd.operators import _stream_handle
     from tigre_amd.phantom import ellipsoid_phantom_torch
 
     dev = torch.device("cuda:0")
@@ -238,6 +243,19 @@ def main():
         log("calibration copy: %d MiB" % args.calib_copy_mib)
         del a, b
 
+    if args.calib_valu:
+        # r = W (p - r) over 8192 x 256 x 4 x 16 elements: every lane runs 16 iterations of the
+        # 13-instruction loop of sart_residual_kernel, plus 8 instructions outside it
+        n_el = 8192 * 256 * 4 * 16
+        a = torch.zeros(n_el, device=dev)
+        b = torch.ones(n_el, device=dev)
+        c = torch.ones(n_el, device=dev)
+        _native.check(lib.tigre_sart_residual(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                              ctypes.c_void_p(c.data_ptr()), n_el, _stream_handle()), "calibration")
+        torch.cuda.synchronize()
+        log("VALU calibration: 32768 wavefronts x 216 VALU instructions = 7077888 expected")
+        del a, b, c
+
     b_ax = nviews * (V + P) * 4.0
     b_atb = nviews * (2 * V + P) * 4.0
     if k_fdk >= k_siddon:
@@ -315,7 +333,48 @@ def main():
                                                    ", ".join("%.1f" % r for r in runs)),
             "host": host_description()}
         log("cpu baseline %.4f Gupdates/s on %d threads" % (cpu_rate, oracle.get_threads()))
+    if not args.no_workloads:
+        # the other benchmark workloads, compact: run after the timed region, each summarised in the line
+        del img, proj
+        torch.cuda.empty_cache()
+        result["workloads"] = run_workloads(args)
     print(json.dumps(result), flush=True)
+
+
+def _compact(d, drop):
+    return {k: v for k, v in d.items() if k not in drop}
+
+
+def run_workloads(args):
+    """Compact runs of the precision (1024^3, BF16/FP16/FP32 storage, CGLS with
+    10 iterations, 2 oracle views), batched (64 cases, 2 graph replays) and
+    learned (256^3 ASD-POCS) workloads; each entry is that workload's JSON
+    without its per-iteration curves, or the error it raised."""
+    import copy
+
+    import torch
+
+    out = {}
+    for name, fn, over, drop in (
+            ("precision_1024", bench_precision, {"cgls_iters": 10, "oracle_views": 2}, ()),
+            ("batched_64", bench_batched, {"steps": 2}, ("tuned",)),
+            ("learned_256", bench_learned, {}, ("runs",))):
+        a = copy.copy(args)
+        for k, v in over.items():
+            setattr(a, k, v)
+        t = time.perf_counter()
+        try:
+            r = fn(a)
+            if "config" in r and isinstance(r["config"], dict):
+                r["config"] = _compact(r["config"], ("cases", "oracle_views"))
+            out[name] = _compact(r, drop + ("n_gpus", "scaling", "vs_baseline"))
+        except Exception as e:  # a failed summary must not cost the main measurement
+            out[name] = {"error": repr(e)}
+        out[name]["wall_s"] = round(time.perf_counter() - t, 1)
+        log("%s: %.1f s" % (name, out[name]["wall_s"]))
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_batched(args):
@@ -357,7 +416,8 @@ def bench_batched(args):
 
     t = time.perf_counter()
     tuner = Autotuner(reps=3)
-    bo = BatchedOSSART(cases, niter=args.iters, blocksize=20, tuner=tuner, streams=args.streams)
+    bo = BatchedOSSART(cases, niter=args.iters, blocksize=20, tuner=tuner)
+    bo.run()  # builds the wave tables, loads the kernels
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t
     log("setup + autotuning %.2f s; tuned %s" % (t_setup, tuner.table))
@@ -369,55 +429,45 @@ def bench_batched(args):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    # 1) the public API, one call per case
+    # 1) the public API, one call per case (default plan parameters)
     t_api = clock(lambda: [algs.ossart(p, g, a, args.iters, blocksize=20, return_tensor=True) for p, g, a in cases])
     log("64 x ossart(): %.3f s" % t_api)
-    # 2) prepared plans, host-driven, case after case
+    # 2) the same plans as the batches, host-driven, case after case
     bo.reset()
     t_eager = clock(bo.run_sequential)
-    eager = [r.clone() for r in bo.results()]
-    log("plans, sequential launches: %.3f s" % t_eager)
-    # 3) one graph
+    ref = [r.clone() for r in bo.results()]
+    log("plans, case after case: %.3f s" % t_eager)
+    # 3) wave batches: one launch per stage over every case of a wave
+    t_waves = float(np.median([clock(bo.run) for _ in range(max(1, args.steps))]))
+    same_waves = all(torch.equal(a, b) for a, b in zip(bo.results(), ref))
+    log("wave batches: median %.3f s (%d waves), bit-identical: %s" % (t_waves, len(bo.waves), same_waves))
+    # 4) the waves captured in one graph
     t = time.perf_counter()
     bo.capture()
     t_capture = time.perf_counter() - t
-    bo.reset()
-    bo.replay()
-    torch.cuda.synchronize()
-    reps = []
-    for _ in range(max(1, args.steps)):
-        reps.append(clock(bo.replay))
+    reps = [clock(bo.replay) for _ in range(max(1, args.steps))]
     t_graph = float(np.median(reps))
-    same = all(torch.equal(a, b) for a, b in zip(bo.results(), eager))
-    log("graph replay: median %.3f s over %d (capture %.2f s), bit-identical to sequential: %s" % (
-        t_graph, len(reps), t_capture, same))
-    # 4) one graph per stream, replayed on concurrent streams
-    bo.graph = None
-    bo.capture_lanes()
-    bo.reset()
-    bo.replay_lanes()
-    torch.cuda.synchronize()
-    lane_reps = [clock(bo.replay_lanes) for _ in range(max(1, args.steps))]
-    t_lanes = float(np.median(lane_reps))
-    same_lanes = all(torch.equal(a, b) for a, b in zip(bo.results(), eager))
-    log("per-stream graphs: median %.3f s, bit-identical: %s" % (t_lanes, same_lanes))
+    same = all(torch.equal(a, b) for a, b in zip(bo.results(), ref))
+    log("graph replay: median %.3f s over %d (capture %.2f s), bit-identical: %s" % (t_graph, len(reps), t_capture,
+                                                                                 same))
+    del ref
     result = {
         "metric": "batched OS-SART wall-clock, %d cases x %d iterations" % (len(cases), args.iters),
         "value": round(t_graph, 4), "unit": "s", "n_gpus": 1, "steps": len(reps), "warmup": 1,
         "ms_per_step": round(t_graph * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (seeded 10-ellipsoid phantoms, projections from Ax)",
         "config": {"workload": "64 uneven OS-SART cases (edge 64-256, 60-360 views, DSO 800-1200 mm), "
-                               "10 iterations, 20-view subsets, one graph replay",
-                   "cases": [list(d) for d in desc], "streams": args.streams},
+                               "10 iterations, 20-view subsets, wave batches in one graph replay",
+                   "cases": [list(d) for d in desc], "waves": len(bo.waves)},
         "gupdates_per_s": round(updates / t_graph / 1e9, 2),
-        "per_stream_graphs_s": round(t_lanes, 4), "per_stream_graphs_bit_identical": same_lanes,
-        "sequential_api_s": round(t_api, 4), "sequential_plans_s": round(t_eager, 4),
-        "speedup_vs_api": round(t_api / t_graph, 2), "speedup_vs_sequential_plans": round(t_eager / t_graph, 2),
-        "bit_identical_to_sequential": same, "setup_s": round(t_setup, 2), "capture_s": round(t_capture, 2),
-        "tuned": {k: list(v) for k, v in tuner.table.items()},
+        "wave_batches_s": round(t_waves, 4), "sequential_api_s": round(t_api, 4),
+        "sequential_plans_s": round(t_eager, 4), "speedup_vs_api": round(t_api / t_graph, 2),
+        "speedup_vs_sequential_plans": round(t_eager / t_graph, 2),
+        "bit_identical_to_sequential_plans": bool(same and same_waves), "setup_s": round(t_setup, 2),
+        "capture_s": round(t_capture, 2), "tuned": {k: list(v) for k, v in tuner.table.items()},
     }
     bo.close()
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_learned(args):
@@ -526,7 +576,7 @@ def bench_learned(args):
                      "traffic": None, "model_tflop": round(flops_model / 1e12, 3),
                      "issued_tflop": round(flops_exec / 1e12, 3)},
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_precision(args):
@@ -535,9 +585,10 @@ def bench_precision(args):
     per-view detector offsets drawn from U[-2, 2] pixels on both axes (seeded);
     seeded 30-ellipsoid phantom, projections from Ax.  For FP32, BF16 and FP16
     storage (FP32 arithmetic): Ax Siddon and Atb matched timed and compared with
-    the FP32 result and, on 8 views, with the FP64 oracle; then 30 CGLS
-    iterations (FP64 inner products), the final iterate compared with the FP32
-    run's and with the phantom."""
+    the FP32 result and, on a view subset, with the FP64 oracle; then CGLS
+    (FP64 inner products; 30 iterations by default) with narrow working vectors
+    and a float32 iterate, and with the iterate stored narrow as well, the final
+    iterate compared with the FP32 run's and with the phantom."""
     import numpy as np
     import torch
 
@@ -547,7 +598,7 @@ def bench_precision(args):
     from tigre_amd.geometry import cone_geometry, prepared
     from tigre_amd.phantom import ellipsoid_phantom_torch
 
-    nx, nviews, niter = 1024, 720, 30
+    nx, nviews, niter = 1024, 720, args.cgls_iters
     geo = cone_geometry((nx, nx, nx), (256.0,) * 3, (768, 1024), (0.4, 0.4), 1000.0, 1536.0)
     rng = np.random.default_rng(20261202 + 3)
     geo.offDetector = rng.uniform(-2.0, 2.0, (nviews, 2)) * np.asarray(geo.dDetector)
@@ -569,14 +620,14 @@ def bench_precision(args):
         torch.cuda.synchronize()
         return out, time.perf_counter() - t0
 
-    sub = np.array([k * nviews // 8 for k in range(8)])
+    sub = np.array([k * nviews // args.oracle_views for k in range(args.oracle_views)])
     img_h = img.cpu().numpy()
     gp = prepared(copy_geo(geo, sub), angles[sub])
     t = time.perf_counter()
     ref_ax = oracle.ax(img_h, gp, "Siddon")
     ref_atb = oracle.atb(proj[torch.as_tensor(sub, device="cuda")].cpu().numpy(), gp, "matched")
     t_oracle = time.perf_counter() - t
-    log("oracle on 8 views: %.1f s" % t_oracle)
+    log("oracle on %d views: %.1f s" % (len(sub), t_oracle))
     res = {}
     x32 = None
     for name, dt in (("f32", torch.float32), ("bf16", torch.bfloat16), ("f16", torch.float16)):
@@ -587,9 +638,9 @@ def bench_precision(args):
         subgeo = copy_geo(geo, sub)
         atb_sub = tigre_amd.Atb(projd[idx].contiguous(), subgeo, angles[sub], "matched")
         entry = {"ax_ms": round(t_ax * 1e3, 2), "atb_matched_ms": round(t_atb * 1e3, 2),
-                 "ax_rel_l2_vs_f32": rel(ax.float(), proj), "ax_rel_l2_vs_fp64_8views":
+                 "ax_rel_l2_vs_f32": rel(ax.float(), proj), "ax_rel_l2_vs_fp64":
                      oracle.rel_l2(ax[idx].float().cpu().numpy(), ref_ax),
-                 "atb_rel_l2_vs_fp64_8views": oracle.rel_l2(atb_sub.float().cpu().numpy(), ref_atb)}
+                 "atb_rel_l2_vs_fp64": oracle.rel_l2(atb_sub.float().cpu().numpy(), ref_atb)}
         del ax, atb, atb_sub
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -601,23 +652,80 @@ def bench_precision(args):
             x32 = xf.clone()
         entry["cgls_rel_l2_vs_f32_run"] = rel(xf, x32)
         entry["cgls_rel_l2_vs_phantom"] = rel(xf, img)
+        if name != "f32":
+            # the iterate stored narrow as well (every update rounded)
+            del x, xf
+            t = time.perf_counter()
+            x = algs.cgls(proj, geo, angles, niter, storage=dt, narrow_iterate=True, return_tensor=True)
+            torch.cuda.synchronize()
+            xf = x.float()
+            entry["cgls_narrow_iterate"] = {"s": round(time.perf_counter() - t, 2),
+                                            "rel_l2_vs_f32_run": rel(xf, x32), "rel_l2_vs_phantom": rel(xf, img)}
         res[name] = entry
         log("%s: %s" % (name, entry))
         del x, xf, imgd, projd
         torch.cuda.empty_cache()
     V, n = float(nx) ** 3, nviews
     result = {
-        "metric": "storage precision at 1024^3 / 720 views: rel-L2 of Ax, Atb and CGLS(30) per storage type",
-        "value": res["bf16"]["cgls_rel_l2_vs_f32_run"], "unit": "rel-L2 (BF16 CGLS iterate vs FP32 run)",
+        "metric": "storage precision at 1024^3 / 720 views: rel-L2 of Ax, Atb and CGLS(%d) per storage type" % niter,
+        "value": res["bf16"]["cgls_rel_l2_vs_f32_run"],
+        "unit": "rel-L2 (CGLS iterate with BF16 working vectors vs the FP32 run)",
         "n_gpus": 1, "steps": niter, "warmup": 1, "ms_per_step": round(res["f32"]["cgls_s"] * 1e3 / niter, 1),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16/f16/f32 storage, f32 math",
         "data": "synthetic (seeded 30-ellipsoid phantom, projections from Ax)",
         "config": {"workload": "1024^3 (0.25 mm), 720 views, 1024x768 detector at 0.4 mm, jittered offsets, "
-                               "CGLS 30 iterations", "oracle_views": [int(i) for i in sub],
+                               "CGLS %d iterations" % niter, "oracle_views": [int(i) for i in sub],
                    "oracle_seconds": round(t_oracle, 1)},
         "by_storage": res,
         "gupdates_per_s_ax_f32": round(V * n / (res["f32"]["ax_ms"] * 1e-3) / 1e9, 1),
     }
+    return result
+
+
+def bench_tv(args):
+    """The TV steepest-descent step (minimizeTV, 20 iterations) on seeded
+    40-ellipsoid phantoms at 256^3 and 512^3: kernel time per iteration and the
+    rate against the 12 bytes per voxel and iteration of the two-pass model."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import tigre_amd
+    from tigre_amd import _native
+    from tigre_amd.phantom import ellipsoid_phantom_torch
+
+    lib = _native.lib()
+    out = {}
+    for n in (256, 512):
+        img = 0.02 * ellipsoid_phantom_torch((n, n, n), (float(n),) * 3, 40, seed=20261202 + 5, device="cuda")
+        tigre_amd.minTV(img, 1e-3, 20)
+        torch.cuda.synchronize()
+        total, count = ctypes.c_double(0.0), ctypes.c_int(0)
+        lib.tigre_kernel_time(_native.KERNEL_POCS_TV, ctypes.byref(total), ctypes.byref(count))
+        lib.tigre_kernel_timing(1)
+        s_, e_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s_.record()
+        for _ in range(max(1, args.steps)):
+            tigre_amd.minTV(img, 1e-3, 20)
+        e_.record()
+        e_.synchronize()
+        lib.tigre_kernel_timing(0)
+        _native.check(lib.tigre_kernel_time(_native.KERNEL_POCS_TV, ctypes.byref(total), ctypes.byref(count)),
+                      "kernel_time")
+        it_ms = total.value / count.value
+        call_ms = s_.elapsed_time(e_) / max(1, args.steps)
+        V = float(n) ** 3
+        out["%d^3" % n] = {"iteration_us": round(it_ms * 1e3, 1), "call_20_iterations_ms": round(call_ms, 3),
+                           "model_gb_s": round(12.0 * V / (it_ms * 1e-3) / 1e9, 1),
+                           "frac_of_hbm_peak": round(12.0 * V / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log("TV %d^3: %.1f us per iteration, 20 iterations %.3f ms" % (n, it_ms * 1e3, call_ms))
+        del img
+        torch.cuda.empty_cache()
+    result = {"metric": "minimizeTV iteration time (two passes + norm)", "value": out["512^3"]["iteration_us"],
+              "unit": "us per iteration at 512^3", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+              "higher_is_better": False, "dtype": "f32", "data": "synthetic (seeded 40-ellipsoid phantoms)",
+              "config": {"workload": "minTV 20 iterations at 256^3 and 512^3"}, "by_size": out}
     print(json.dumps(result), flush=True)
 
 

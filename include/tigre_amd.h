@@ -242,6 +242,33 @@ int tigre_plan_sart_update(TigrePlan plan, const float* d_residual, float* d_x, 
 int tigre_sart_residual(const float* d_proj, const float* d_w, float* d_ax, long long n, void* stream);
 int tigre_plan_destroy(TigrePlan plan);
 
+/* Batches: one SART subset update (tigre_plan_ax Siddon, tigre_sart_residual,
+ * tigre_plan_sart_update FDK) of each of many independent problems, run as one
+ * launch per stage (staging copies, beam projections, residuals, quad copies,
+ * backprojections) over all of them, so small problems fill the device next
+ * to large ones.  Replaces the reference's one-problem-at-a-time subset loop
+ * (iterative_recon_alg.py:326-345) when many reconstructions run together; no
+ * counterpart in the reference.  The job tables are uploaded at creation;
+ * tigre_batch_sart only launches kernels (no allocation, synchronisation or
+ * upload), so it can be captured in a HIP graph.  Results equal the three plan
+ * calls per step bit for bit.  Steps must not share an image, a scratch
+ * projection buffer or a workspace; FP32, cone beam, Python layout. */
+typedef struct TigreBatchStep {
+    TigrePlan plan;         /* the subset's plan */
+    float* d_x;             /* the problem's image, updated in place */
+    const float* d_proj;    /* measured projections of the subset (16-byte aligned) */
+    const float* d_w;       /* W weights of the subset (16-byte aligned) */
+    float* d_ax;            /* scratch projections of the subset's size (16-byte aligned) */
+    const float* d_inv_v;   /* lambda / V of the subset, nVoxelY x nVoxelX */
+    void* d_workspace;      /* tigre_plan_workspace_size(plan, TIGRE_DTYPE_F32) bytes, 256-byte aligned */
+    size_t workspace_bytes;
+    int noneg;              /* clip the updated image at 0 */
+} TigreBatchStep;
+typedef struct TigreBatch_* TigreBatch;
+int tigre_batch_create(const TigreBatchStep* steps, int nsteps, TigreBatch* out);
+int tigre_batch_sart(TigreBatch batch, void* stream);
+int tigre_batch_destroy(TigreBatch batch);
+
 /* ---------------------------------------------------------------------- */
 /* Learned proximal step: 3x3x3 convolutions (BF16 MFMA, FP32 accumulate)  */
 /* ---------------------------------------------------------------------- */

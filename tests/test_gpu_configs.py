@@ -69,6 +69,15 @@ def _geo512(n=512):
     return cone_geometry((n, n, n), (256.0,) * 3, (n, n), (0.8 * 512 / n,) * 2, 1000.0, 1536.0)
 
 
+def _views(geo, angles, idx):
+    """The geometry with its per-view arrays expanded over all views, restricted to views idx."""
+    import copy
+
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    return _sub_geometry(g, idx)
+
+
 SUB512 = np.array([k * 360 // 8 for k in range(8)])
 
 
@@ -92,7 +101,7 @@ def test_ax_512_full_call_on_view_subset(cuda, case512, ptype):
     geo, angles, img = case512
     _say("Ax %s 512^3 / 360 views" % ptype)
     got = tigre_amd.Ax(img, geo, angles, ptype)[torch.as_tensor(SUB512, device=cuda)].cpu().numpy()
-    ref = oracle.ax(img.cpu().numpy(), prepared(_sub_geometry(geo, SUB512), angles[SUB512]), ptype)
+    ref = oracle.ax(img.cpu().numpy(), prepared(_views(geo, angles, SUB512), angles[SUB512]), ptype)
     err = oracle.rel_l2(got, ref)
     _say("Ax %s 512^3 8-view subset rel-L2 %.3e" % (ptype, err))
     assert err <= TOL
@@ -103,7 +112,7 @@ def test_atb_512_view_subset(cuda, case512, btype):
     import torch
 
     geo, angles, img = case512
-    sub_geo = _sub_geometry(geo, SUB512)
+    sub_geo = _views(geo, angles, SUB512)
     gp = prepared(sub_geo, angles[SUB512])
     img_h = img.cpu().numpy()
     proj = oracle.ax(img_h, gp).astype(np.float32)
@@ -180,7 +189,7 @@ def test_storage_1024_two_views(cuda, case1024, dtype):
 
     geo, angles, img = case1024
     sub = np.array([0, 359])
-    sub_geo = _sub_geometry(geo, sub)
+    sub_geo = _views(geo, angles, sub)
     gp = prepared(sub_geo, angles[sub])
     t = img.to(getattr(torch, dtype))
     _say("Ax/Atb 1024^3 %s on 2 views" % dtype)

@@ -27,6 +27,13 @@ CNN_IN_F32X1 = 1
 CNN_EPI_BIAS_RELU = 0
 CNN_EPI_RELU_MASK = 1
 CNN_EPI_RESIDUAL = 2
+KERNEL_AX_SIDDON = 0
+KERNEL_AX_INTERPOLATED = 1
+KERNEL_ATB_FDK = 2
+KERNEL_ATB_MATCHED = 3
+KERNEL_RELAYOUT = 4
+KERNEL_POCS_TV = 5
+KERNEL_TVDENOISE = 6
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -116,6 +123,9 @@ SIGNATURES = {
     "tigre_sart_residual": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
                                            ctypes.c_void_p]),
     "tigre_plan_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tigre_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "tigre_batch_sart": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tigre_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tigre_cnn_conv": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_void_p]),
@@ -163,6 +173,14 @@ REFERENCE_ABI = {
 # The same entry points with the MATLAB detector layout (v fastest), as the
 # reference's MEX build compiles them.
 LIB_PATH_MATLAB = os.path.join(os.path.dirname(LIB_PATH), "libtigre_amd_matlab.so")
+
+
+class TigreBatchStep(ctypes.Structure):
+    """Mirror of ``TigreBatchStep`` (include/tigre_amd.h)."""
+
+    _fields_ = [("plan", ctypes.c_void_p), ("d_x", ctypes.c_void_p), ("d_proj", ctypes.c_void_p),
+                ("d_w", ctypes.c_void_p), ("d_ax", ctypes.c_void_p), ("d_inv_v", ctypes.c_void_p),
+                ("d_workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t), ("noneg", ctypes.c_int)]
 
 
 class GpuIds(ctypes.Structure):

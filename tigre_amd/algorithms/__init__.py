@@ -370,7 +370,10 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
     ``narrow_iterate`` is set: an iterate rounded to a 7-bit significand after
     every update carries that rounding into the result (each of the niter
     updates adds a relative error of up to 2^-8), which a float32 iterate
-    avoids at 2 extra bytes per voxel."""
+    avoids at 2 extra bytes per voxel.  With narrow storage the residual is
+    the true b - A x of each iteration (computed anyway for the l2 norm)
+    instead of the recursive r - alpha q of the reference: the same vector in
+    exact arithmetic, without the rounding a narrow recursion accumulates."""
     torch = _torch()
     dt = torch.float32 if storage is None else storage
     xt = dt if narrow_iterate else torch.float32
@@ -408,7 +411,8 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
         alpha = gamma / ((n_p * _norm(q)) ** 2)
         step = alpha * n_p
         torch.add(x, p, alpha=step, out=x)
-        l2[0, i] = _norm(residual())
+        rr = residual()
+        l2[0, i] = _norm(rr)
         if i > 0 and l2[0, i] > l2[0, i - 1]:
             torch.sub(x, p, alpha=step, out=x)
             if verbose:
@@ -419,7 +423,10 @@ def cgls(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
             reinit_at = i
             r, p, n_p, gamma = start()
             continue
-        torch.sub(r, q, alpha=step, out=r)
+        if scaled:
+            r = rr  # the true residual, rounded once: no drift of a narrow recursive update
+        else:
+            torch.sub(r, q, alpha=step, out=r)
         s = vs.atb(r, 0, _native.BACKPROJ_MATCHED)
         gamma1 = _norm(s) ** 2
         p = torch.add(s, p, alpha=gamma1 / gamma * n_p)

@@ -1,27 +1,31 @@
-"""Many independent OS-SART reconstructions on one device, captured in one graph.
+"""Many independent OS-SART reconstructions on one device, batched per subset step.
 
 Each case is an OS-SART problem (art_family_algorithms.py:47-62 with the
 subset update of iterative_recon_alg.py:326-345 / :389-399): for every subset
 j, r = W_j (p_j - Ax_j(x)), then x = max(x + (lambda / V_j) Atb_j(r), 0).
-Per subset the plan launches stage / beam-projection kernels, the fused
-residual and the backprojector whose epilogue applies the update, so one
-subset costs a handful of kernel launches and no host work.
 
-``run_sequential`` launches case after case from the host (the same kernels,
-one Python call per launch); ``capture`` records every case's iterations into
-one CUDA/HIP graph with the cases spread over a few streams (independent
-branches of the graph), and ``replay`` runs it with one host call;
-``capture_lanes`` / ``replay_lanes`` record one graph per stream instead and
-replay them on concurrent streams, which lets small cases overlap (the
-branches of one graph execute one after another on this runtime).  All give
-the same bits as :func:`tigre_amd.algorithms.ossart` when the plans use the
-default parameters, since the kernels and their rounding steps are the same.
+Cases advance in waves: wave w applies subset w mod n_c of every case c that
+still has iterations left (n_c subsets per iteration).  A wave is one
+``tigre_batch_sart`` call, five launches that each cover every case of the
+wave (staging copies, beam projections, residuals, quad copies, FDK
+backprojections with the update epilogue), so the small problems run beside
+the large ones instead of after them.  The job tables of every wave are built
+on the device once; ``run`` then only launches kernels, and ``capture``
+records all waves into one CUDA/HIP graph replayed with one host call.
+
+``run_sequential`` launches the same plan kernels case after case from the
+host.  Both give the bits of :func:`tigre_amd.algorithms.ossart` when the
+plans use the default parameters, since the kernels and their rounding steps
+are the same.
 """
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 
 from .. import _native
+from ..operators import _stream_handle
 from ..plan import Autotuner, Plan, sart_residual
 from . import _SartState, _sub_geometry, _torch
 
@@ -67,12 +71,13 @@ class BatchedOSSART:
     the defaults (4, 8, 16)."""
 
     def __init__(self, cases, niter, blocksize=20, lmbda=1.0, lmbda_red=1.0, noneg=True, OrderStrategy=None,
-                 tuner: Autotuner | None = None, params=None, streams=8):
+                 tuner: Autotuner | None = None, params=None):
         self.niter = int(niter)
         self.cases = [_Case(p, g, a, blocksize, lmbda, lmbda_red, self.niter, noneg, OrderStrategy, tuner, params)
                       for (p, g, a) in cases]
-        self.nstreams = max(1, int(streams))
         self.graph = None
+        self.waves = None
+        self._lib = _native.lib()
 
     def reset(self):
         for c in self.cases:
@@ -81,53 +86,51 @@ class BatchedOSSART:
     def results(self):
         return [c.res for c in self.cases]
 
+    def _build_waves(self):
+        """One batch per wave: subset w mod n_c of every case with iterations left."""
+        steps_per_case = [self.niter * len(c.plans) for c in self.cases]
+        self.waves = []
+        for w in range(max(steps_per_case)):
+            items = []
+            for c, n in zip(self.cases, steps_per_case):
+                if w >= n:
+                    continue
+                j = w % len(c.plans)
+                p = c.plans[j]
+                items.append(_native.TigreBatchStep(
+                    p._h.value, c.res.data_ptr(), c.proj_blocks[j].data_ptr(), c.W_blocks[j].data_ptr(),
+                    c.axbuf.data_ptr(), c.inv_v[j].data_ptr(), c.ws.data_ptr(), c.ws.numel(), int(c.noneg)))
+            arr = (_native.TigreBatchStep * len(items))(*items)
+            h = ctypes.c_void_p()
+            _native.check(self._lib.tigre_batch_create(arr, len(items), ctypes.byref(h)), "TIGRE: batch creation")
+            self.waves.append(h)
+
     def run(self):
-        """Every case's ``niter`` iterations from a zero image, on the current stream."""
+        """Every case's ``niter`` iterations from a zero image, one batch launch per wave, on the current stream."""
+        if self.waves is None:
+            self._build_waves()
         self.reset()
-        return self.run_sequential()
+        stream = _stream_handle()
+        for h in self.waves:
+            _native.check(self._lib.tigre_batch_sart(h, stream), "TIGRE: batched SART step")
+        return self.results()
 
     def run_sequential(self):
-        """All iterations of case 0, then case 1, ... from the host, on the current stream."""
+        """All iterations of case 0, then case 1, ... from the host, on the current stream (from the
+        current images)."""
         for c in self.cases:
             for it in range(self.niter):
                 c.iteration(it)
         return self.results()
 
-    def _assign(self):
-        # longest cases first, each to the least-loaded stream
-        load = [0.0] * self.nstreams
-        lanes = [[] for _ in range(self.nstreams)]
-        for i in sorted(range(len(self.cases)), key=lambda k: -self.cases[k].work):
-            s = int(np.argmin(load))
-            lanes[s].append(i)
-            load[s] += self.cases[i].work
-        return lanes
-
     def capture(self):
-        """Record every case's ``niter`` iterations into one graph (results start from zero at replay)."""
+        """Record ``run`` (zeroing included) into one graph."""
         torch = _torch()
-        # one eager pass loads every kernel before capture
-        self.run_sequential()
+        self.run()  # builds the tables and loads every kernel before capture
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
-        side = [torch.cuda.Stream() for _ in range(self.nstreams)]
-        cap = torch.cuda.Stream()
-        lanes = self._assign()
-        with torch.cuda.graph(self.graph, stream=cap):
-            main = torch.cuda.current_stream()
-            for c in self.cases:
-                c.res.zero_()
-            for s, idx in zip(side, lanes):
-                if not idx:
-                    continue
-                s.wait_stream(main)
-                with torch.cuda.stream(s):
-                    for i in idx:
-                        for it in range(self.niter):
-                            self.cases[i].iteration(it)
-            for s, idx in zip(side, lanes):
-                if idx:
-                    main.wait_stream(s)
+        with torch.cuda.graph(self.graph):
+            self.run()
         torch.cuda.synchronize()
         return self.graph
 
@@ -137,45 +140,11 @@ class BatchedOSSART:
         self.graph.replay()
         return self.results()
 
-    def capture_lanes(self):
-        """One graph per stream (its share of the cases), so that replays on
-        different streams can run at the same time (the branches of a single
-        graph run one after another on this runtime)."""
-        torch = _torch()
-        self.run_sequential()
-        torch.cuda.synchronize()
-        self.lanes = []
-        for idx in self._assign():
-            if not idx:
-                continue
-            s = torch.cuda.Stream()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=s):
-                for i in idx:
-                    self.cases[i].res.zero_()
-                    for it in range(self.niter):
-                        self.cases[i].iteration(it)
-            self.lanes.append((s, g))
-        torch.cuda.synchronize()
-        return self.lanes
-
-    def replay_lanes(self):
-        """Replay every lane's graph on its own stream, joined to the current stream."""
-        torch = _torch()
-        if not getattr(self, "lanes", None):
-            self.capture_lanes()
-        main = torch.cuda.current_stream()
-        for s, g in self.lanes:
-            s.wait_stream(main)
-            with torch.cuda.stream(s):
-                g.replay()
-        for s, _ in self.lanes:
-            main.wait_stream(s)
-        return self.results()
-
     def close(self):
         self.graph = None
-        self.lanes = []
+        for h in self.waves or []:
+            self._lib.tigre_batch_destroy(h)
+        self.waves = None
         for c in self.cases:
             for p in c.plans:
                 p.close()

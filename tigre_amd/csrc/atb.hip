@@ -282,17 +282,19 @@ __device__ __forceinline__ void samples_global(const char* pjb, unsigned rowb, u
     for (int k = 0; k < kZ; ++k) smp[k] = bilerp(q[k], wu, wv[k]);
 }
 
+// The workgroup of block coordinates (bx, by, bz): columns 16 bx .. + 15,
+// rows 16 by .. + 15, slices kZ bz .. + kZ - 1.
 template <int kW, int kZ>
-__global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
-                                                      const BwdView* __restrict__ views, float* vol, int nx,
-                                                      int ny, int nz, float scale, AtbUpdate ep) {
+__device__ __forceinline__ void atb_lds_block(const float* __restrict__ proj, int nU, int nV, int nviews,
+                                              const BwdView* __restrict__ views, float* vol, int nx, int ny, int nz,
+                                              float scale, const AtbUpdate& ep, int bx, int by, int bz) {
     static_assert(kW == 0 || kW == 2, "FDK or parallel-beam weight");
     __shared__ float4 win[4][2][kWinQuads];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: LDS bases stay scalar
-    const int ix = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int iy = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    const int z0 = blockIdx.z * kZ;
+    const int ix = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    const int iy = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int z0 = bz * kZ;
     // no early exit: every lane takes part in the copies, and the window is
     // read from the corner lanes 0, 7, 56, 63; lanes outside the volume only
     // skip the store
@@ -400,15 +402,38 @@ __global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict
     if (ix < nx && iy < ny) store_column<kZ>(acc, vol, ix, iy, z0, nx, ny, nz, scale, ep);
 }
 
+template <int kW, int kZ>
+__global__ __launch_bounds__(256, 3) void atb_lds_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
+                                                      const BwdView* __restrict__ views, float* vol, int nx,
+                                                      int ny, int nz, float scale, AtbUpdate ep) {
+    atb_lds_block<kW, kZ>(proj, nU, nV, nviews, views, vol, nx, ny, nz, scale, ep, blockIdx.x, blockIdx.y,
+                          blockIdx.z);
+}
+
+// Many independent FDK backprojections with the SART update epilogue in one
+// launch: job j owns blocks [begin, begin + gx gy gz) of the grid.
+template <int kZ>
+__global__ __launch_bounds__(256, 3) void atb_lds_multi_kernel(const AtbJob* __restrict__ jobs, int njobs) {
+    const int b = blockIdx.x;
+    const int j = find_job(jobs, njobs, b);
+    const AtbJob& J = jobs[j];
+    const int l = b - J.begin;
+    AtbUpdate ep;
+    ep.x = J.x;
+    ep.inv_v = J.inv_v;
+    ep.noneg = J.noneg;
+    atb_lds_block<0, kZ>(J.quads, J.nU, J.nV, J.nviews, J.views, J.vol, J.nx, J.ny, J.nz, J.scale, ep, l % J.gx,
+                         (l / J.gx) % J.gy, l / (J.gx * J.gy));
+}
+
 // Quad copy of the projections (see above), view-major, rows of quad_det(nU)
-// elements.  Reads either detector layout.
-__global__ __launch_bounds__(256) void quad_projections_kernel(const void* __restrict__ proj, int dtype, int nU,
-                                                               int nV, int nviews, int layout,
-                                                               float4* __restrict__ quads) {
+// elements.  Reads either detector layout.  Elements first, first + stride, ...
+__device__ __forceinline__ void quad_projections_range(const void* __restrict__ proj, int dtype, int nU, int nV,
+                                                       int nviews, int layout, float4* __restrict__ quads,
+                                                       long long first, long long stride) {
     const long long QU = quad_det(nU), QV = quad_det(nV);
     const long long total = QU * QV * nviews;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (long long)gridDim.x * blockDim.x) {
+    for (long long q = first; q < total; q += stride) {
         const long long u = q % QU - 1, v = (q / QU) % QV - 1, p = q / (QU * QV);
         const long long base = p * (long long)nU * nV;
         float t[4];
@@ -424,6 +449,52 @@ __global__ __launch_bounds__(256) void quad_projections_kernel(const void* __res
         const float du0 = t[1] - t[0], du1 = t[3] - t[2];
         quads[q] = make_float4(t[0], du0, t[2] - t[0], du1 - du0);
     }
+}
+
+__global__ __launch_bounds__(256) void quad_projections_kernel(const void* __restrict__ proj, int dtype, int nU,
+                                                               int nV, int nviews, int layout,
+                                                               float4* __restrict__ quads) {
+    quad_projections_range(proj, dtype, nU, nV, nviews, layout, quads,
+                           (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
+}
+
+// Quad copies of many FP32 projection sets (Python layout) in one launch.
+__global__ __launch_bounds__(256) void quad_projections_multi_kernel(const QuadJob* __restrict__ jobs, int njobs) {
+    const int b = blockIdx.x;
+    const int j = find_job(jobs, njobs, b);
+    const QuadJob& J = jobs[j];
+    quad_projections_range(J.proj, TIGRE_DTYPE_F32, J.nU, J.nV, J.nviews, TIGRE_LAYOUT_PYTHON,
+                           reinterpret_cast<float4*>(J.quads), (long long)(b - J.begin) * 256 + threadIdx.x,
+                           (long long)J.blocks * 256);
+}
+
+int quad_job_blocks(int nU, int nV, int nviews) {
+    const long long total = quad_det(nU) * quad_det(nV) * (long long)nviews;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    return (int)(blocks < 1 ? 1 : blocks);
+}
+
+int launch_quad_multi(const QuadJob* d_jobs, int njobs, long long blocks, hipStream_t stream) {
+    TIGRE_REQUIRE(njobs > 0 && blocks > 0 && blocks < (1LL << 31), "bad multi-job launch");
+    KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
+    hipLaunchKernelGGL(quad_projections_multi_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_atb_multi(const AtbJob* d_jobs, int njobs, long long blocks, int zper, hipStream_t stream) {
+    TIGRE_REQUIRE(njobs > 0 && blocks > 0 && blocks < (1LL << 31), "bad multi-job launch");
+    TIGRE_REQUIRE(zper == 4 || zper == 8 || zper == 16, "slices per lane must be 4, 8 or 16");
+    KernelTimerScope timer(TIGRE_KERNEL_ATB_FDK, stream);
+    if (zper == 16)
+        hipLaunchKernelGGL(atb_lds_multi_kernel<16>, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs);
+    else if (zper == 8)
+        hipLaunchKernelGGL(atb_lds_multi_kernel<8>, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs);
+    else
+        hipLaunchKernelGGL(atb_lds_multi_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
 }
 
 int launch_quad_projections(const void* proj, int dtype, int nU, int nV, int nviews, int layout, float* quads,

@@ -596,18 +596,16 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     }
 }
 
-// Two workgroups per CU (8 wavefronts per SIMD): the register budget is held
-// to that occupancy, which the slab barriers need to keep the SIMDs busy.
+// Workgroup L of one projection: tile L % tiles of group L / tiles.
 template <int kMode>
-__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_beam_kernel(
-    const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
-    const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
-    long long su, long long sv, long long sview, float* __restrict__ out, double acc, double dvx, double dvy,
-    double dvz, int max_planes) {
-    __shared__ SweepShared sh;
+__device__ __forceinline__ void beam_block(const float* __restrict__ stage_x, const float* __restrict__ stage_y,
+                                           int nx, int ny, int nz, int na_x, int na_y,
+                                           const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups,
+                                           int ngroups, int nU, int nV, long long su, long long sv, long long sview,
+                                           float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
+                                           int max_planes, int L, SweepShared& sh) {
     const int tilesU = (nU + kBeamTile - 1) / kBeamTile, tilesV = (nV + kBeamTile - 1) / kBeamTile;
     const int tiles = tilesU * tilesV;
-    const int L = xcd_logical_block(blockIdx.x, gridDim.x);
     const int gi = L / tiles, tile = L % tiles;
     if (gi >= ngroups) return;  // uniform per workgroup
     const BeamGroup grp = groups[gi];
@@ -624,17 +622,44 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     }
 }
 
+// Two workgroups per CU (8 wavefronts per SIMD): the register budget is held
+// to that occupancy, which the slab barriers need to keep the SIMDs busy.
+template <int kMode>
+__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_beam_kernel(
+    const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
+    const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
+    long long su, long long sv, long long sview, float* __restrict__ out, double acc, double dvx, double dvy,
+    double dvz, int max_planes) {
+    __shared__ SweepShared sh;
+    beam_block<kMode>(stage_x, stage_y, nx, ny, nz, na_x, na_y, views, groups, ngroups, nU, nV, su, sv, sview, out,
+                      acc, dvx, dvy, dvz, max_planes, xcd_logical_block(blockIdx.x, gridDim.x), sh);
+}
+
+// Siddon projections of many independent problems (Python detector layout) in
+// one launch; the XCD-ordered block index runs over the whole grid, so the
+// consecutive workgroups of one group still share an XCD.
+__global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void ax_beam_multi_kernel(
+    const BeamJob* __restrict__ jobs, int njobs) {
+    __shared__ SweepShared sh;
+    const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int j = find_job(jobs, njobs, L);
+    const BeamJob& J = jobs[j];
+    beam_block<0>(J.stage_x, J.stage_y, J.nx, J.ny, J.nz, J.na_x, J.na_y, J.views,
+                  reinterpret_cast<const BeamGroup*>(J.groups), J.ngroups, J.nU, J.nV, 1, J.nU,
+                  (long long)J.nU * J.nV, J.out, J.acc, J.dvx, J.dvy, J.dvz, J.max_planes, L - J.begin, sh);
+}
+
 // Staging copy for principal axis P: out[p][z][a] over the bordered ranges
 // p -1 .. nP + 1, z -1 .. nz, a -4 .. na_pad + 3 (zero outside the volume), from the
 // x-fastest volume.  64 x 64 tiles of (p, a) per z row go through LDS, so both
 // the reads (x fastest) and the writes (a fastest) are coalesced whichever of
 // p, a is x.
-__global__ __launch_bounds__(256) void stage_volume_kernel(const void* __restrict__ vol, int dtype, int nx, int ny,
-                                                           int nz, int principal, int na_pad,
-                                                           float* __restrict__ out) {
+__device__ __forceinline__ void stage_volume_block(const void* __restrict__ vol, int dtype, int nx, int ny, int nz,
+                                                   int principal, int na_pad, float* __restrict__ out, int bx, int by,
+                                                   int bz) {
     __shared__ float tile[64][65];
     const int NP = (principal == 0 ? nx : ny) + 3, NZ = nz + 2, NA = na_pad + 8;
-    const int a_base = blockIdx.x * 64 - 4, p_base = blockIdx.y * 64 - 1, z = (int)blockIdx.z - 1;
+    const int a_base = bx * 64 - 4, p_base = by * 64 - 1, z = bz - 1;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     // read: x fastest across threads
     for (int r = ty; r < 64; r += 4) {
@@ -663,6 +688,22 @@ __global__ __launch_bounds__(256) void stage_volume_kernel(const void* __restric
     }
 }
 
+__global__ __launch_bounds__(256) void stage_volume_kernel(const void* __restrict__ vol, int dtype, int nx, int ny,
+                                                           int nz, int principal, int na_pad,
+                                                           float* __restrict__ out) {
+    stage_volume_block(vol, dtype, nx, ny, nz, principal, na_pad, out, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Staging copies of many FP32 volumes in one launch.
+__global__ __launch_bounds__(256) void stage_volume_multi_kernel(const StageJob* __restrict__ jobs, int njobs) {
+    const int b = blockIdx.x;
+    const int j = find_job(jobs, njobs, b);
+    const StageJob& J = jobs[j];
+    const int l = b - J.begin;
+    stage_volume_block(J.vol, TIGRE_DTYPE_F32, J.nx, J.ny, J.nz, J.principal, J.na_pad, J.out, l % J.gx,
+                       (l / J.gx) % J.gy, l / (J.gx * J.gy));
+}
+
 }  // namespace
 
 int stage_pad(int n) { return (n + 3) & ~3; }
@@ -673,6 +714,35 @@ long long stage_elements(const HostGeometry& g, int principal) {
     // planes -1 .. nP + 1 written, plus one unwritten plane of slack: the
     // last rows a slab box copies may run past the copy by less than a plane
     return (long long)(nP + 4) * (g.nVoxel[2] + 2) * (stage_pad(nA) + 8);
+}
+
+int stage_job_blocks(const HostGeometry& g, int principal, int* gx, int* gy) {
+    const int nP = principal == 0 ? g.nVoxel[0] : g.nVoxel[1];
+    const int nA = principal == 0 ? g.nVoxel[1] : g.nVoxel[0];
+    *gx = (stage_pad(nA) + 8 + 63) / 64;
+    *gy = (nP + 3 + 63) / 64;
+    return *gx * *gy * (g.nVoxel[2] + 2);
+}
+
+long long beam_job_blocks(const HostGeometry& g, int ngroups) {
+    const long long tiles = ((g.nDetec[0] + kBeamTile - 1) / kBeamTile) * ((g.nDetec[1] + kBeamTile - 1) / kBeamTile);
+    return tiles * ngroups;
+}
+
+int launch_stage_multi(const StageJob* d_jobs, int njobs, long long blocks, hipStream_t stream) {
+    TIGRE_REQUIRE(njobs > 0 && blocks > 0 && blocks < (1LL << 31), "bad multi-job launch");
+    KernelTimerScope timer(TIGRE_KERNEL_RELAYOUT, stream);
+    hipLaunchKernelGGL(stage_volume_multi_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
+}
+
+int launch_ax_beam_multi(const BeamJob* d_jobs, int njobs, long long blocks, hipStream_t stream) {
+    TIGRE_REQUIRE(njobs > 0 && blocks > 0 && blocks < (1LL << 31), "bad multi-job launch");
+    KernelTimerScope timer(TIGRE_KERNEL_AX_SIDDON, stream);
+    hipLaunchKernelGGL(ax_beam_multi_kernel, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, d_jobs, njobs);
+    TIGRE_HIP_TRY(hipGetLastError());
+    return TIGRE_OK;
 }
 
 int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int principal, float* stage,

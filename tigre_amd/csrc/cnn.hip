@@ -123,16 +123,19 @@ __global__ __launch_bounds__(256) void conv_kernel(const void* __restrict__ x, c
 // Convolution of 32-channel bf16 activations with the input staged in LDS.
 // conv_kernel above reads every input fragment from global memory once per
 // tap (27 times per voxel), which bounds it by the texture-address path.
-// Here a workgroup computes 4 output rows (one per wavefront) of 128 voxels
-// (8 MFMA tiles per wavefront, so each tap's weight fragments serve 16
-// MFMAs) and, for each of the three input planes z - 1, z, z + 1 in turn,
-// copies the 6 x 130 voxel block its taps read into LDS with direct
-// global-to-LDS loads (voxels outside the volume are copied from a zero
-// chunk).  The 16-byte channel groups of a voxel are stored in the order
-// q ^ ((x >> 2) & 3): the 16 lanes of one group read 16 consecutive voxels
-// and then fall on distinct LDS banks.  A fragment address is a per-lane
-// constant for each tile and dx, plus an immediate offset for dy.
-constexpr int kLdsTiles = 8;                    // 16-voxel tiles per wavefront
+// Here a workgroup computes 4 output rows of 128 voxels and, for each of the
+// three input planes z - 1, z, z + 1 in turn, copies the 6 x 130 voxel block
+// its taps read into LDS with direct global-to-LDS loads (voxels outside the
+// volume are copied from a zero chunk).  Each wavefront owns 32 voxels (two
+// MFMA tiles) of all four rows, so an A fragment read from staged row r feeds
+// every output row that reads r (rows r - 2 .. r, one dy tap each): per plane
+// and dx, 12 fragment reads serve 48 MFMAs.  The 16-byte channel groups of a
+// voxel are stored in the order q ^ ((x >> 2) & 3): the 16 lanes of one group
+// read 16 consecutive voxels and then fall on distinct LDS banks.  A fragment
+// address is a per-lane constant for each dx, plus an immediate offset for
+// the tile and the row.
+constexpr int kLdsTiles = 8;                    // 16-voxel tiles per workgroup row
+constexpr int kWaveTiles = 2;                   // tiles per wavefront (of each of the 4 rows)
 constexpr int kLdsBX = 16 * kLdsTiles + 2;      // staged voxels along x (with the halo)
 constexpr int kLdsRows = 4 + 2;                 // staged rows: the workgroup's 4 and the halo
 constexpr int kLdsChunks = kLdsRows * kLdsBX * 4;  // 16-byte chunks of one staged plane
@@ -150,21 +153,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
     const int zz0 = (int)(blockIdx.z % (unsigned)D);
     const long long n = blockIdx.z / (unsigned)D;
-    const int yo = y0 + wave;  // this wavefront's output row (computed but not stored when >= H)
+    const int xw = x0 + 16 * kWaveTiles * wave;  // this wavefront's first output voxel (all four rows)
 
-    // A-fragment byte offsets in the plane for tile 0, dx = d - 1 and dy = -1
-    // (row `wave` of the block).  Tile t adds 16 voxels (1024 bytes; the
-    // swizzle depends on (x >> 2) & 3, which 16 voxels leave unchanged) and
-    // dy + 1 rows kLdsBX * 64 bytes each: both are immediate offsets.
+    // A-fragment byte offsets in the plane for staged row 0, this wavefront's
+    // first tile and dx = d - 1.  Its second tile adds 16 voxels (1024 bytes;
+    // the swizzle depends on (x >> 2) & 3, which 16 voxels leave unchanged) and
+    // staged row r adds r kLdsBX 64 bytes: both are immediate offsets.
     unsigned aoff[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const int xi = m + d;
-        aoff[d] = (unsigned)(((wave * kLdsBX + xi) * 4 + (q ^ ((xi >> 2) & 3))) * 16);
+        const int xi = 16 * kWaveTiles * wave + m + d;
+        aoff[d] = (unsigned)((xi * 4 + (q ^ ((xi >> 2) & 3))) * 16);
     }
-    f32x4 acc[kLdsTiles][2];
+    // acc[row][tile][output channel block]
+    f32x4 acc[4][kWaveTiles][2];
 #pragma unroll
-    for (int t = 0; t < kLdsTiles; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int t = 0; t < kWaveTiles; ++t) acc[r][t][0] = acc[r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
     const char* lds = reinterpret_cast<const char*>(plane);
 
     for (int dz = -1; dz <= 1; ++dz) {
@@ -187,48 +193,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy) {
+        for (int d = 0; d < 3; ++d) {
+            // weight fragments of the three dy taps of this (dz, dx)
+            bf16x8 b[3][2];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const int tap = (dz + 1) * 9 + dy * 3 + d;
-                const unsigned short* wt = w + (long long)tap * kC * kC;
-                const bf16x8 b0 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + m * kC + 8 * q));
-                const bf16x8 b1 =
-                    __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + (16 + m) * kC + 8 * q));
+            for (int dy = 0; dy < 3; ++dy) {
+                const unsigned short* wt = w + (long long)((dz + 1) * 9 + dy * 3 + d) * kC * kC;
+                b[dy][0] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + m * kC + 8 * q));
+                b[dy][1] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + (16 + m) * kC + 8 * q));
+            }
+            // staged row sr feeds output row orow = sr - dy for dy = 0, 1, 2 (orow in 0 .. 3)
 #pragma unroll
-                for (int t = 0; t < kLdsTiles; ++t) {
+            for (int sr = 0; sr < kLdsRows; ++sr) {
+#pragma unroll
+                for (int t = 0; t < kWaveTiles; ++t) {
                     const bf16x8 a = __builtin_bit_cast(
-                        bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + dy * (kLdsBX * 64))));
-                    acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc[t][0], 0, 0, 0);
-                    acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc[t][1], 0, 0, 0);
+                        bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + sr * (kLdsBX * 64))));
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy) {
+                        const int orow = sr - dy;
+                        if (orow < 0 || orow > 3) continue;  // resolved at compile time
+                        acc[orow][t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[dy][0], acc[orow][t][0], 0, 0, 0);
+                        acc[orow][t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[dy][1], acc[orow][t][1], 0, 0, 0);
+                    }
                 }
             }
         }
     }
 
-    if (yo >= H) return;
-    const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
 #pragma unroll
-    for (int t = 0; t < kLdsTiles; ++t) {
+    for (int orow = 0; orow < 4; ++orow) {
+        const int yo = y0 + orow;
+        if (yo >= H) break;
+        const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
 #pragma unroll
-        for (int hb = 0; hb < 2; ++hb) {
-            const int o = hb * 16 + m;
+        for (int t = 0; t < kWaveTiles; ++t) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int xo = x0 + 16 * t + 4 * q + r;
-                if (xo >= W) continue;
-                const long long vox = obase + xo;
-                float v = acc[t][hb][r];
-                if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
-                    v += bias[o];
-                    v = v > 0.0f ? v : 0.0f;
-                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
-                } else if (kEpi == TIGRE_CNN_EPI_RELU_MASK) {
-                    const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
-                    v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
-                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
-                } else {  // residual output: channel 0 only, FP32
-                    if (o == 0) static_cast<float*>(y)[vox] = static_cast<const float*>(aux)[vox] + (v + bias[0]);
+            for (int hb = 0; hb < 2; ++hb) {
+                const int o = hb * 16 + m;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int xo = xw + 16 * t + 4 * q + r;
+                    if (xo >= W) continue;
+                    const long long vox = obase + xo;
+                    float v = acc[orow][t][hb][r];
+                    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+                        v += bias[o];
+                        v = v > 0.0f ? v : 0.0f;
+                        static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                    } else if (kEpi == TIGRE_CNN_EPI_RELU_MASK) {
+                        const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
+                        v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+                        static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
+                    } else {  // residual output: channel 0 only, FP32
+                        if (o == 0) static_cast<float*>(y)[vox] = static_cast<const float*>(aux)[vox] + (v + bias[0]);
+                    }
                 }
             }
         }

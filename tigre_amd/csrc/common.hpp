@@ -274,6 +274,73 @@ int atb_launch(const void* d_proj, int dtype, const HostGeometry& g, const BwdVi
 // r = W (p - Ax) over n elements, written over ax (sart.hip).
 int launch_sart_residual(const float* proj, const float* w, float* ax, long long n, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// Multi-job launches: one launch runs the same stage of many independent
+// problems.  Job j owns the blocks [begin, begin + its block count) of the
+// 1-D grid; jobs are sorted by begin, the first begins at 0.
+// ---------------------------------------------------------------------------
+struct StageJob {    // staging copy of an FP32 volume for principal axis `principal`
+    const float* vol;
+    float* out;
+    int nx, ny, nz, principal, na_pad, gx, gy, begin;  // grid (gx, gy, nz + 2)
+};
+struct BeamJob {     // beam-staged Siddon projection of one view list
+    const float* stage_x;
+    const float* stage_y;
+    const FwdView* views;
+    const int* groups;
+    float* out;
+    double acc, dvx, dvy, dvz;
+    int nx, ny, nz, na_x, na_y, ngroups, nU, nV, max_planes, begin;
+};
+struct ResidualJob { // r = W (p - Ax), n elements
+    const float* proj;
+    const float* w;
+    float* ax;
+    long long n;
+    int blocks, begin;
+};
+struct QuadJob {     // quad copy of FP32 projections, Python layout
+    const float* proj;
+    float* quads;
+    int nU, nV, nviews, blocks, begin, pad;
+};
+struct AtbJob {      // FDK backprojection (axial views) with the SART update epilogue
+    const float* quads;
+    const BwdView* views;
+    float* vol;
+    const float* x;
+    const float* inv_v;
+    float scale;
+    int nU, nV, nviews, nx, ny, nz, noneg, gx, gy, begin;
+};
+
+// Index of the job that owns block b (binary search over the begins; every
+// lane of a workgroup computes the same index).
+template <class Job>
+__device__ __forceinline__ int find_job(const Job* __restrict__ jobs, int njobs, int b) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+int launch_stage_multi(const StageJob* d_jobs, int njobs, long long blocks, hipStream_t stream);
+int launch_ax_beam_multi(const BeamJob* d_jobs, int njobs, long long blocks, hipStream_t stream);
+int launch_residual_multi(const ResidualJob* d_jobs, int njobs, long long blocks, hipStream_t stream);
+int launch_quad_multi(const QuadJob* d_jobs, int njobs, long long blocks, hipStream_t stream);
+int launch_atb_multi(const AtbJob* d_jobs, int njobs, long long blocks, int zper, hipStream_t stream);
+// Block counts of one job of each kind (host side).
+int stage_job_blocks(const HostGeometry& g, int principal, int* gx, int* gy);
+long long beam_job_blocks(const HostGeometry& g, int ngroups);
+int residual_job_blocks(long long n);
+int quad_job_blocks(int nU, int nV, int nviews);
+
 // Convolutions of the learned proximal network (cnn.hip).
 size_t cnn_wgrad_workspace(int N, int D, int H, int W);
 int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const float* bias, const void* aux, void* y,

@@ -6,6 +6,7 @@
 // full).  The tunables are the views per beam workgroup (1, 2 or 4), the
 // LDS slab depth of the beam kernel (1..8 principal planes) and the slices
 // per lane of the backprojector (4, 8 or 16).
+#include <algorithm>
 #include <cstring>
 
 #include "common.hpp"
@@ -59,6 +60,50 @@ size_t plan_workspace(const TigrePlan_* p, int dtype) {
     size_t b = atb_workspace_bytes(g) + vtmp;
     for (int k = 0; k < 2; ++k) b = std::max(b, ax_workspace_bytes(g, p->fwd[k], k == 1) + ptmp);
     return b;
+}
+
+}  // namespace
+
+// A batch: one SART subset update of each of many independent problems, as
+// five multi-job launches (staging copies, beam projections, residuals, quad
+// copies, FDK backprojections with the update epilogue).  The job tables live
+// on the device from creation on, so a run only launches kernels.  Views a
+// beam workgroup does not take (gather kernels) and backprojections the staged
+// multi-job kernel does not cover run as per-problem launches in between.
+struct TigreBatch_ {
+    int device = 0;
+    void* d_tables = nullptr;
+    const StageJob* d_stage = nullptr;
+    const BeamJob* d_beam = nullptr;
+    const ResidualJob* d_res = nullptr;
+    const QuadJob* d_quad = nullptr;
+    const AtbJob* d_atb[3] = {nullptr, nullptr, nullptr};  // slices per lane 4, 8, 16
+    int n_stage = 0, n_beam = 0, n_res = 0, n_quad = 0, n_atb[3] = {0, 0, 0};
+    long long b_stage = 0, b_beam = 0, b_res = 0, b_quad = 0, b_atb[3] = {0, 0, 0};
+    std::vector<TigreBatchStep> steps;
+    std::vector<int> rest_steps;  // steps with views for the gather kernels
+    std::vector<int> atb_steps;   // steps whose backprojection runs on its own
+    ~TigreBatch_() {
+        if (d_tables) (void)hipFree(d_tables);
+    }
+};
+
+namespace {
+
+int zslot(int zper) { return zper == 4 ? 0 : (zper == 8 ? 1 : 2); }
+
+// Byte offset of the staging copy for principal axis a in a forward workspace
+// (the layout ax_launch uses), and of the bricked copy after them.
+size_t stage_offset(const HostGeometry& g, const ForwardRecords& rec, int a) {
+    size_t off = 0;
+    for (int k = 0; k < a; ++k)
+        if (rec.stage_used[k]) off += align256((size_t)stage_elements(g, k) * sizeof(float));
+    return off;
+}
+
+bool atb_multi_ok(const TigrePlan_* p) {
+    // the staged kernel: FDK weight, axial views, row pairs in 16 bits
+    return p->axial && !p->g.parallel && p->g.nDetec[1] + 1 < 65536;
 }
 
 }  // namespace
@@ -217,6 +262,243 @@ int tigre_cnn_wgrad(const void* d_x, int x_mode, const void* d_dy, int dy_mode, 
     TIGRE_REQUIRE(d_x != nullptr && d_dy != nullptr && d_dw != nullptr, "null device pointer");
     return launch_cnn_wgrad(d_x, x_mode, d_dy, dy_mode, d_dw, n, d, h, w, d_workspace, workspace_bytes,
                             (hipStream_t)stream);
+}
+
+int tigre_batch_create(const TigreBatchStep* steps, int nsteps, TigreBatch* out) {
+    set_error("");
+    TIGRE_REQUIRE(out != nullptr, "null batch handle");
+    *out = nullptr;
+    TIGRE_REQUIRE(steps != nullptr && nsteps > 0, "a batch needs at least one step");
+    int dev = 0;
+    TIGRE_HIP_TRY(hipGetDevice(&dev));
+    std::vector<StageJob> stage;
+    std::vector<BeamJob> beam;
+    std::vector<ResidualJob> res;
+    std::vector<QuadJob> quad;
+    std::vector<AtbJob> atb[3];
+    long long b_stage = 0, b_beam = 0, b_res = 0, b_quad = 0, b_atb[3] = {0, 0, 0};
+    std::vector<int> rest_steps, atb_steps;
+    std::vector<const void*> owned;  // buffers a step writes: no two steps may share one
+    for (int i = 0; i < nsteps; ++i) {
+        const TigreBatchStep& st = steps[i];
+        const TigrePlan_* p = st.plan;
+        TIGRE_CHECK(plan_on_device(p));
+        TIGRE_REQUIRE(!p->g.parallel, "batched SART steps are cone beam");
+        TIGRE_REQUIRE(st.d_x && st.d_proj && st.d_w && st.d_ax && st.d_inv_v && st.d_workspace,
+                      "null pointer in a batch step");
+        TIGRE_REQUIRE((reinterpret_cast<uintptr_t>(st.d_workspace) & 255) == 0, "workspace must be 256-byte aligned");
+        TIGRE_REQUIRE(st.workspace_bytes >= plan_workspace(p, TIGRE_DTYPE_F32),
+                      "workspace smaller than tigre_plan_workspace_size");
+        TIGRE_REQUIRE(((reinterpret_cast<uintptr_t>(st.d_proj) | reinterpret_cast<uintptr_t>(st.d_w) |
+                        reinterpret_cast<uintptr_t>(st.d_ax)) & 15) == 0,
+                      "projection buffers must be 16-byte aligned");
+        for (const void* q : {(const void*)st.d_x, (const void*)st.d_ax, (const void*)st.d_workspace}) {
+            for (const void* o : owned) TIGRE_REQUIRE(o != q, "two batch steps share an image, scratch or workspace");
+            owned.push_back(q);
+        }
+        const HostGeometry& g = p->g;
+        const ForwardRecords& rec = p->fwd[0];
+        TIGRE_REQUIRE(stage_elements(g, 0) < (1LL << 32) && stage_elements(g, 1) < (1LL << 32),
+                      "volume too large for 32-bit staging offsets");
+        char* ws = static_cast<char*>(st.d_workspace);
+        float* stg[2] = {nullptr, nullptr};
+        for (int a = 0; a < 2; ++a) {
+            if (!rec.stage_used[a]) continue;
+            stg[a] = reinterpret_cast<float*>(ws + stage_offset(g, rec, a));
+            StageJob j;
+            j.vol = st.d_x;
+            j.out = stg[a];
+            j.nx = g.nVoxel[0];
+            j.ny = g.nVoxel[1];
+            j.nz = g.nVoxel[2];
+            j.principal = a;
+            j.na_pad = (a == 0 ? (g.nVoxel[1] + 3) & ~3 : (g.nVoxel[0] + 3) & ~3);
+            j.begin = (int)b_stage;
+            b_stage += stage_job_blocks(g, a, &j.gx, &j.gy);
+            TIGRE_REQUIRE(j.gy <= 65535 && g.nVoxel[2] + 2 <= 65535, "volume too large for the staging copy");
+            stage.push_back(j);
+        }
+        if (rec.ngroups() > 0) {
+            BeamJob j;
+            j.stage_x = stg[0];
+            j.stage_y = stg[1];
+            j.views = static_cast<const FwdView*>(p->d_fwd[0]);
+            j.groups = static_cast<const int*>(p->d_groups[0]);
+            j.out = st.d_ax;
+            j.acc = g.accuracy;
+            j.dvx = g.dVoxel[0];
+            j.dvy = g.dVoxel[1];
+            j.dvz = g.dVoxel[2];
+            j.nx = g.nVoxel[0];
+            j.ny = g.nVoxel[1];
+            j.nz = g.nVoxel[2];
+            j.na_x = (g.nVoxel[1] + 3) & ~3;
+            j.na_y = (g.nVoxel[0] + 3) & ~3;
+            j.ngroups = rec.ngroups();
+            j.nU = g.nDetec[0];
+            j.nV = g.nDetec[1];
+            j.max_planes = p->max_planes;
+            j.begin = (int)b_beam;
+            b_beam += beam_job_blocks(g, rec.ngroups());
+            beam.push_back(j);
+        }
+        if (rec.nrest() > 0) rest_steps.push_back(i);
+        const long long nproj = (long long)g.nDetec[0] * g.nDetec[1] * g.nangles;
+        {
+            ResidualJob j;
+            j.proj = st.d_proj;
+            j.w = st.d_w;
+            j.ax = st.d_ax;
+            j.n = nproj;
+            j.blocks = residual_job_blocks(nproj);
+            j.begin = (int)b_res;
+            b_res += j.blocks;
+            res.push_back(j);
+        }
+        {
+            QuadJob j;
+            j.proj = st.d_ax;
+            j.quads = reinterpret_cast<float*>(ws);
+            j.nU = g.nDetec[0];
+            j.nV = g.nDetec[1];
+            j.nviews = g.nangles;
+            j.blocks = quad_job_blocks(j.nU, j.nV, j.nviews);
+            j.begin = (int)b_quad;
+            j.pad = 0;
+            b_quad += j.blocks;
+            quad.push_back(j);
+        }
+        TIGRE_REQUIRE(quad_det(g.nDetec[0]) * 16 < (1LL << 24) &&
+                          quad_det(g.nDetec[0]) * quad_det(g.nDetec[1]) * 16 < (1LL << 32),
+                      "detector too large for one backprojection launch");
+        if (atb_multi_ok(p)) {
+            const int k = zslot(p->zper);
+            AtbJob j;
+            j.quads = reinterpret_cast<const float*>(ws);
+            j.views = static_cast<const BwdView*>(p->d_bwd);
+            j.vol = st.d_x;
+            j.x = st.d_x;
+            j.inv_v = st.d_inv_v;
+            j.scale = 1.0f;
+            j.nU = g.nDetec[0];
+            j.nV = g.nDetec[1];
+            j.nviews = g.nangles;
+            j.nx = g.nVoxel[0];
+            j.ny = g.nVoxel[1];
+            j.nz = g.nVoxel[2];
+            j.noneg = st.noneg ? 1 : 0;
+            j.gx = (j.nx + 15) / 16;
+            j.gy = (j.ny + 15) / 16;
+            const int gz = (j.nz + p->zper - 1) / p->zper;
+            TIGRE_REQUIRE(j.gy <= 65535 && gz <= 65535, "volume too large for one launch");
+            j.begin = (int)b_atb[k];
+            b_atb[k] += (long long)j.gx * j.gy * gz;
+            atb[k].push_back(j);
+        } else {
+            atb_steps.push_back(i);
+        }
+    }
+    TIGRE_REQUIRE(b_stage < (1LL << 31) && b_beam < (1LL << 31) && b_res < (1LL << 31) && b_quad < (1LL << 31) &&
+                      b_atb[0] < (1LL << 31) && b_atb[1] < (1LL << 31) && b_atb[2] < (1LL << 31),
+                  "batch too large for one launch per stage");
+    TigreBatch_* b = new TigreBatch_();
+    int st = [&]() -> int {
+        b->device = dev;
+        b->steps.assign(steps, steps + nsteps);
+        b->rest_steps = rest_steps;
+        b->atb_steps = atb_steps;
+        // one table allocation: every array at a 256-byte boundary
+        const size_t sizes[7] = {stage.size() * sizeof(StageJob), beam.size() * sizeof(BeamJob),
+                                 res.size() * sizeof(ResidualJob), quad.size() * sizeof(QuadJob),
+                                 atb[0].size() * sizeof(AtbJob), atb[1].size() * sizeof(AtbJob),
+                                 atb[2].size() * sizeof(AtbJob)};
+        const void* src[7] = {stage.data(), beam.data(), res.data(), quad.data(), atb[0].data(), atb[1].data(),
+                              atb[2].data()};
+        size_t off[7], total = 0;
+        for (int k = 0; k < 7; ++k) {
+            off[k] = total;
+            total += align256(sizes[k]);
+        }
+        std::vector<char> host(total, 0);
+        for (int k = 0; k < 7; ++k)
+            if (sizes[k]) std::memcpy(host.data() + off[k], src[k], sizes[k]);
+        TIGRE_CHECK(upload_sync(&b->d_tables, host.data(), total));
+        const char* base = static_cast<const char*>(b->d_tables);
+        b->d_stage = reinterpret_cast<const StageJob*>(base + off[0]);
+        b->d_beam = reinterpret_cast<const BeamJob*>(base + off[1]);
+        b->d_res = reinterpret_cast<const ResidualJob*>(base + off[2]);
+        b->d_quad = reinterpret_cast<const QuadJob*>(base + off[3]);
+        for (int k = 0; k < 3; ++k) b->d_atb[k] = reinterpret_cast<const AtbJob*>(base + off[4 + k]);
+        b->n_stage = (int)stage.size();
+        b->n_beam = (int)beam.size();
+        b->n_res = (int)res.size();
+        b->n_quad = (int)quad.size();
+        for (int k = 0; k < 3; ++k) {
+            b->n_atb[k] = (int)atb[k].size();
+            b->b_atb[k] = b_atb[k];
+        }
+        b->b_stage = b_stage;
+        b->b_beam = b_beam;
+        b->b_res = b_res;
+        b->b_quad = b_quad;
+        return TIGRE_OK;
+    }();
+    if (st != TIGRE_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return TIGRE_OK;
+}
+
+int tigre_batch_sart(TigreBatch batch, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(batch != nullptr, "null batch");
+    int dev = -1;
+    TIGRE_HIP_TRY(hipGetDevice(&dev));
+    TIGRE_REQUIRE(dev == batch->device, "batch used on another device than the one it was created on");
+    hipStream_t s = (hipStream_t)stream;
+    TigreBatch_& b = *batch;
+    if (b.n_stage) TIGRE_CHECK(launch_stage_multi(b.d_stage, b.n_stage, b.b_stage, s));
+    if (b.n_beam) TIGRE_CHECK(launch_ax_beam_multi(b.d_beam, b.n_beam, b.b_beam, s));
+    for (int i : b.rest_steps) {  // views for the gather-driven Siddon kernel
+        const TigreBatchStep& st = b.steps[i];
+        const TigrePlan_* p = st.plan;
+        const HostGeometry& g = p->g;
+        const ForwardRecords& rec = p->fwd[0];
+        const BrickGrid bg = siddon_bricks(g);
+        float* bricked = reinterpret_cast<float*>(static_cast<char*>(st.d_workspace) + stage_offset(g, rec, 2));
+        TIGRE_CHECK(launch_brick_volume(st.d_x, TIGRE_DTYPE_F32, g, bg, bricked, s));
+        TIGRE_CHECK(launch_ax_siddon(bricked, g, bg, static_cast<const FwdView*>(p->d_fwd[0]) + rec.nbeam,
+                                     rec.nrest(), TIGRE_LAYOUT_PYTHON, st.d_ax, s));
+    }
+    TIGRE_CHECK(launch_residual_multi(b.d_res, b.n_res, b.b_res, s));
+    TIGRE_CHECK(launch_quad_multi(b.d_quad, b.n_quad, b.b_quad, s));
+    const int zpers[3] = {4, 8, 16};
+    for (int k = 0; k < 3; ++k)
+        if (b.n_atb[k]) TIGRE_CHECK(launch_atb_multi(b.d_atb[k], b.n_atb[k], b.b_atb[k], zpers[k], s));
+    for (int i : b.atb_steps) {  // backprojections outside the staged multi-job kernel
+        const TigreBatchStep& st = b.steps[i];
+        AtbUpdate ep;
+        ep.x = st.d_x;
+        ep.inv_v = st.d_inv_v;
+        ep.noneg = st.noneg ? 1 : 0;
+        TIGRE_CHECK(launch_atb(static_cast<const float*>(st.d_workspace), st.plan->g,
+                               static_cast<const BwdView*>(st.plan->d_bwd), false, st.plan->axial, st.d_x,
+                               st.plan->zper, ep, s));
+    }
+    return TIGRE_OK;
+}
+
+int tigre_batch_destroy(TigreBatch batch) {
+    set_error("");
+    if (batch == nullptr) return TIGRE_OK;
+    int prev = -1;
+    TIGRE_HIP_TRY(hipGetDevice(&prev));
+    TIGRE_HIP_TRY(hipSetDevice(batch->device));
+    delete batch;
+    TIGRE_HIP_TRY(hipSetDevice(prev));
+    return TIGRE_OK;
 }
 
 int tigre_plan_destroy(TigrePlan plan) {

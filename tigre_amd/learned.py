@@ -47,8 +47,11 @@ class ResidualCNN:
     9(dz+1) + 3(dy+1) + dx+1), Cin = 1 for the first layer, Cout = 1 for the
     last; biases b[l] have Cout entries."""
 
-    def __init__(self, seed=0, device="cuda"):
+    def __init__(self, seed=0, device="cuda", data_scale=1.0):
         torch = _torch()
+        # images are multiplied by data_scale on the way in and divided by it on
+        # the way out (``apply``), so the network sees values of order one
+        self.data_scale = float(data_scale)
         g = torch.Generator().manual_seed(int(seed))
         shapes = [(27, C, 1)] + [(27, C, C)] * 4 + [(27, 1, C)]
         self.W, self.b = [], []
@@ -84,10 +87,12 @@ class ResidualCNN:
         return self.W + self.b
 
     def state(self):
-        return {"W": [w.cpu().numpy() for w in self.W], "b": [v.cpu().numpy() for v in self.b]}
+        return {"W": [w.cpu().numpy() for w in self.W], "b": [v.cpu().numpy() for v in self.b],
+                "data_scale": self.data_scale}
 
     def load(self, state):
         torch = _torch()
+        self.data_scale = float(state.get("data_scale", 1.0))
         self.W = [torch.as_tensor(np.asarray(w, dtype=np.float32), device=self.device) for w in state["W"]]
         self.b = [torch.as_tensor(np.asarray(v, dtype=np.float32), device=self.device) for v in state["b"]]
         self.refresh()
@@ -142,6 +147,14 @@ class ResidualCNN:
         return out
 
     __call__ = forward
+
+    def apply(self, img):
+        """The trained map on an image in its own units: forward(img * s) / s, s = data_scale."""
+        s = self.data_scale
+        if s == 1.0:
+            return self.forward(img)
+        out = self.forward(img * s)
+        return out.div_(s)
 
     def backward(self, dout):
         """Gradients (dW, db) of sum(out * dout) after ``forward(..., keep=True)`` with scale 1."""
@@ -236,7 +249,7 @@ class LearnedProx:
     def __call__(self, img, dtvg):
         from .tv import minTV
 
-        y = self.net.forward(img, scale=float(dtvg) if self.step_scaled else 1.0)
+        y = self.net.forward(img, scale=float(dtvg)) if self.step_scaled else self.net.apply(img)
         ok = tv_value(y) < tv_value(img)
         r = None
         if ok and self.residual is not None:
@@ -330,15 +343,25 @@ def make_patches(pairs, count, size, seed=0):
     return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
 
 
-def train(net, x, target, steps, batch=8, lr=1e-3, seed=0, log=None):
-    """Adam on random minibatches of (x, target); returns the loss history."""
+def train(net, x, target, steps, batch=8, lr=1e-3, seed=0, log=None, lr_final=None):
+    """Adam on random minibatches of (x, target), both multiplied by the net's
+    data_scale; the learning rate falls from lr to lr_final (default: lr) along
+    a cosine.  Returns the loss history (in scaled units)."""
+    import math
+
     torch = _torch()
     g = torch.Generator(device="cpu").manual_seed(int(seed))
     losses = []
     n = x.shape[0]
+    ds = net.data_scale
+    lr_end = lr if lr_final is None else lr_final
     for s in range(steps):
         idx = torch.randint(0, n, (min(batch, n),), generator=g).to(x.device)
-        losses.append(net.train_step(x[idx].contiguous(), target[idx].contiguous(), lr=lr))
+        xb, tb = x[idx], target[idx]
+        if ds != 1.0:
+            xb, tb = xb * ds, tb * ds
+        rate = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * s / max(1, steps - 1)))
+        losses.append(net.train_step(xb.contiguous(), tb.contiguous(), lr=rate))
         if log is not None and (s % 50 == 0 or s == steps - 1):
             log("step %d loss %.4e" % (s, losses[-1]))
     return losses
