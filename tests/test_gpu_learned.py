@@ -67,7 +67,8 @@ def test_learned_asd_pocs(cuda, setup):
     geo, angles, case, net, _ = setup
     img, b = case(42)
     out = {}
-    for name, reg in (("plain", None), ("learned", L.LearnedProx(net, 20)), ("none", lambda x, s: x)):
+    learned = L.LearnedProx(net, 20, step_scaled=True)  # trained on one TV call: its step scales F
+    for name, reg in (("plain", None), ("learned", learned), ("none", lambda x, s: x)):
         res = algs.asd_pocs(b, geo, angles, ITERS, regularizer=reg, return_tensor=True)
         out[name] = _rmse(res, img)
         if name == "learned":
