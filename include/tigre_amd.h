@@ -296,6 +296,18 @@ int tigre_cnn_conv(const void* d_x, int x_mode, const void* d_w, const float* d_
 int tigre_cnn_wgrad_workspace_size(int n, int d, int h, int w, size_t* bytes);
 int tigre_cnn_wgrad(const void* d_x, int x_mode, const void* d_dy, int dy_mode, float* d_dw, int n, int d, int h,
                     int w, void* d_workspace, size_t workspace_bytes, void* stream);
+/* tigre_cnn_wgrad without a workspace argument and without any allocation
+ * (no counterpart in the reference: for callers that cannot allocate, such as
+ * code under HIP graph capture or a training loop that forbids hipMalloc).
+ * The per-block partial sums go to a static device array the library holds on
+ * each GPU it runs on until the process ends; its size, the largest
+ * tigre_cnn_wgrad_workspace_size of any volume (28 MiB), is returned by
+ * tigre_cnn_wgrad_noalloc_limit, and a volume needing more is rejected.  The
+ * result equals tigre_cnn_wgrad's bit for bit.  Calls that overlap on one GPU
+ * (different streams) share the array and race. */
+int tigre_cnn_wgrad_noalloc(const void* d_x, int x_mode, const void* d_dy, int dy_mode, float* d_dw, int n, int d,
+                            int h, int w, void* stream);
+int tigre_cnn_wgrad_noalloc_limit(size_t* bytes);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* tigre_last_error(void);

@@ -134,6 +134,10 @@ SIGNATURES = {
     "tigre_cnn_wgrad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_void_p]),
+    "tigre_cnn_wgrad_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                               ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_void_p]),
+    "tigre_cnn_wgrad_noalloc_limit": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "tigre_get_gpu_count": (ctypes.c_int, []),
     "tigre_get_gpu_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "tigre_kernel_timing": (ctypes.c_int, [ctypes.c_int]),

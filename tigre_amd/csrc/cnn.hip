@@ -23,6 +23,8 @@
 // Volumes are N x D x H x W (x fastest); taps index (dz, dy, dx) in
 // {-1, 0, 1}^3 as 9 (dz + 1) + 3 (dy + 1) + (dx + 1); outside the volume the
 // input is zero.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace tigre {
@@ -508,6 +510,51 @@ int launch_cnn_wgrad(const void* x, int x_mode, const void* dy, int dy_mode, flo
                        dw);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient without allocation.  The per-block partial sums go to a
+// library-held device array sized for the most any volume needs (wgrad_chunks
+// caps the chunks at 256: 256 x 28 x 32 x 32 floats, 28 MiB), one copy per GPU
+// for the life of the process.  Every partial the reduction reads is written
+// by the same call, so nothing needs clearing; calls that overlap on one GPU
+// (two streams) share the array and race.  The array's device address is
+// looked up once per device, with that device current; host code passes that
+// address, never the symbol itself.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) float scratch[256 * 28 * kC * kC];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_address(void** out) {
+    int dev = 0;
+    TIGRE_HIP_TRY(hipGetDevice(&dev));
+    TIGRE_REQUIRE(dev >= 0 && dev < kScratchDevices, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        TIGRE_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TIGRE_REQUIRE(p != nullptr, "no device address for the weight-gradient scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TIGRE_OK;
+}
+}  // namespace
+
+size_t cnn_wgrad_static_bytes() { return sizeof(scratch); }
+
+int launch_cnn_wgrad_noalloc(const void* x, int x_mode, const void* dy, int dy_mode, float* dw, int N, int D, int H,
+                             int W, hipStream_t stream) {
+    TIGRE_REQUIRE(N > 0 && D > 0 && H > 0 && W > 0, "empty volume");
+    const size_t need = cnn_wgrad_workspace(N, D, H, W);
+    TIGRE_REQUIRE(need <= sizeof(scratch), "weight-gradient partials exceed the library's static scratch");
+    void* ws = nullptr;
+    TIGRE_CHECK(scratch_address(&ws));
+    return launch_cnn_wgrad(x, x_mode, dy, dy_mode, dw, N, D, H, W, ws, sizeof(scratch), stream);
 }
 
 }  // namespace tigre

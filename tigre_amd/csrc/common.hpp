@@ -347,6 +347,11 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
                     int epi, int N, int D, int H, int W, hipStream_t stream);
 int launch_cnn_wgrad(const void* x, int x_mode, const void* dy, int dy_mode, float* dw, int N, int D, int H, int W,
                      void* ws, size_t ws_bytes, hipStream_t stream);
+// The same without a workspace: partials in the library's static device array
+// of cnn_wgrad_static_bytes() bytes (cnn.hip).
+size_t cnn_wgrad_static_bytes();
+int launch_cnn_wgrad_noalloc(const void* x, int x_mode, const void* dy, int dy_mode, float* dw, int N, int D, int H,
+                             int W, hipStream_t stream);
 
 // Operator drivers on device pointers (ops.cpp).
 int ax_device(const void* d_img, const HostGeometry& g, void* d_proj, int projection_type, int layout, int dtype,

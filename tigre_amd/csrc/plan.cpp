@@ -264,6 +264,20 @@ int tigre_cnn_wgrad(const void* d_x, int x_mode, const void* d_dy, int dy_mode, 
                             (hipStream_t)stream);
 }
 
+int tigre_cnn_wgrad_noalloc(const void* d_x, int x_mode, const void* d_dy, int dy_mode, float* d_dw, int n, int d,
+                            int h, int w, void* stream) {
+    set_error("");
+    TIGRE_REQUIRE(d_x != nullptr && d_dy != nullptr && d_dw != nullptr, "null device pointer");
+    return launch_cnn_wgrad_noalloc(d_x, x_mode, d_dy, dy_mode, d_dw, n, d, h, w, (hipStream_t)stream);
+}
+
+int tigre_cnn_wgrad_noalloc_limit(size_t* bytes) {
+    set_error("");
+    TIGRE_REQUIRE(bytes != nullptr, "null argument");
+    *bytes = cnn_wgrad_static_bytes();
+    return TIGRE_OK;
+}
+
 int tigre_batch_create(const TigreBatchStep* steps, int nsteps, TigreBatch* out) {
     set_error("");
     TIGRE_REQUIRE(out != nullptr, "null batch handle");
