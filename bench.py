@@ -32,6 +32,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 # (scaled by a copy-kernel calibration); parsed from this summary when present.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_512cube_360views.txt")
+# VALU / LDS instruction counts per launch from a rocprofv3 SQ_* counter pass; parsed when present.
+VALU_SUMMARY = os.path.join(ROOT, "profiles", "valu_lds_512cube_360views.txt")
+PROBE_BLOCKS, PROBE_ITERS = 8192, 4096
+# VALU instructions per wavefront of the probe: 8 FMAs per round plus 26 outside the loop
+PROBE_VALU_PER_WAVE = 8 * PROBE_ITERS + 26
 
 
 def measured_traffic(kernel):
@@ -45,6 +50,53 @@ def measured_traffic(kernel):
     except (OSError, ValueError):
         return None
     return None
+
+
+def measured_valu(kernel):
+    """(VALU, LDS, LDS bank-conflict cycles) per launch of `kernel` from the committed counter
+    summary (rows '| key | kernel name | SQ_INSTS_VALU | SQ_INSTS_SALU | SQ_INSTS_LDS |
+    SQ_LDS_BANK_CONFLICT | ...'), or None."""
+    try:
+        with open(VALU_SUMMARY) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 6 and cells[0] == kernel:
+                    return float(cells[2]), float(cells[4]), float(cells[5])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
+def valu_issue_rate(lib, dev):
+    """Measured VALU issue rate (wavefront instructions per second) of the
+    probe kernel: PROBE_BLOCKS x 4 wavefronts, PROBE_VALU_PER_WAVE each, timed
+    by the library's HIP events on the launch stream (median of 5)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from tigre_amd import _native
+    from tigre_amd.operators import _stream_handle
+
+    out = torch.empty(PROBE_BLOCKS * 256, device=dev)
+    total, count = ctypes.c_double(0.0), ctypes.c_int(0)
+    _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, 0,
+                                       _stream_handle()), "probe")
+    torch.cuda.synchronize()
+    lib.tigre_kernel_timing(1)
+    ms = []
+    for _ in range(5):
+        _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, 0,
+                                           _stream_handle()), "probe")
+        _native.check(lib.tigre_kernel_time(_native.KERNEL_PROBE, ctypes.byref(total), ctypes.byref(count)),
+                      "kernel_time")
+        ms.append(total.value / count.value)
+    lib.tigre_kernel_timing(0)
+    waves = PROBE_BLOCKS * 4
+    t = float(np.median(ms))
+    return {"wave_instr_per_s": waves * PROBE_VALU_PER_WAVE / (t * 1e-3), "probe_ms": round(t, 4),
+            "waves": waves, "valu_per_wave": PROBE_VALU_PER_WAVE}
 
 
 def log(*a):
@@ -84,7 +136,7 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=24,
                     help="views in the timed CPU-baseline sample, indices floor(k * views / cpu_views) (0 = skip)")
     ap.add_argument("--cpu-runs", type=int, default=3, help="CPU-baseline runs (the median is reported)")
-    ap.add_argument("--parity-views", type=int, default=64,
+    ap.add_argument("--parity-views", type=int, default=32,
                     help="views of the full-size parity check against the oracle (0 = skip)")
     ap.add_argument("--no-extra", action="store_true", help="skip the interpolated / matched timings")
     ap.add_argument("--ossart-iters", type=int, default=20,
@@ -256,6 +308,9 @@ def main():
         log("VALU calibration: 32768 wavefronts x 216 VALU instructions = 7077888 expected")
         del a, b, c
 
+    probe = valu_issue_rate(lib, dev)
+    log("VALU probe: %.4g wavefront instructions/s (%.3f ms)" % (probe["wave_instr_per_s"], probe["probe_ms"]))
+
     b_ax = nviews * (V + P) * 4.0
     b_atb = nviews * (2 * V + P) * 4.0
     if k_fdk >= k_siddon:
@@ -288,8 +343,21 @@ def main():
                    "atb_matched": round(matched_ms, 3)},
         "kernel_ms": {"ax_siddon": round(k_siddon, 3), "atb_fdk": round(k_fdk, 3),
                       "ax_interpolated": round(k_interp, 3), "atb_matched": round(k_matched, 3)},
-        "pair_ms_model_60pct": round((b_ax + b_atb) / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3, 2),
     }
+    # The same four kernels against the VALU issue rate the probe measured: instruction counts per
+    # launch from the committed counter summary, durations from this run.
+    valu = {}
+    for name, ms in (("ax_siddon", k_siddon), ("ax_interpolated", k_interp), ("atb_fdk", k_fdk),
+                     ("atb_matched", k_matched)):
+        counts = measured_valu(name) if n == 512 and nviews == 360 else None
+        if counts is None or not ms == ms:
+            continue
+        rate = counts[0] / (ms * 1e-3)
+        valu[name] = {"valu_instr_per_launch": counts[0], "kernel_ms": round(ms, 3),
+                      "achieved": round(rate, 4), "frac": round(rate / probe["wave_instr_per_s"], 4),
+                      "lds_instr_per_launch": counts[1], "lds_bank_conflict_cycles": counts[2]}
+    result["valu_roofline"] = {"bound": "valu", "unit": "wavefront instructions/s",
+                               "peak": round(probe["wave_instr_per_s"], 1), "probe": probe, "kernels": valu}
     if ossart is not None:
         result["ossart"] = ossart
 
@@ -300,6 +368,7 @@ def main():
         oracle.set_threads(threads)
         img_h = img.cpu().numpy()
     if args.parity_views > 0:
+        log("parity: FP64 oracle on %d views ..." % args.parity_views)
         # full-size parity of the 360-view calls on a fixed view subset
         idx = np.array([k * nviews // args.parity_views for k in range(args.parity_views)])
         gp = prepared(geo, angles[idx])
@@ -319,11 +388,12 @@ def main():
         gp = prepared(geo, angles[idx])
         proj_h = proj[torch.as_tensor(idx, device=dev)].contiguous().cpu().numpy()
         runs = []
-        for _ in range(max(1, args.cpu_runs)):
+        for r in range(max(1, args.cpu_runs)):
             t = time.perf_counter()
             oracle.ax(img_h, gp, "Siddon")
             oracle.atb(proj_h, gp, "FDK")
             runs.append(time.perf_counter() - t)
+            log("cpu baseline run %d: %.1f s on %d threads" % (r + 1, runs[-1], threads))
         t_med = float(np.median(runs))
         cpu_rate = 2.0 * V * len(idx) / t_med / 1e9
         result["cpu_baseline"] = {
@@ -522,7 +592,8 @@ def bench_learned(args):
 
     runs = {}
     for name in ("plain", "learned", "no_step"):
-        reg = None if name == "plain" else (L.LearnedProx(net, 20) if name == "learned" else (lambda z, s: z))
+        reg = None if name == "plain" else (L.LearnedProx(net, 20, step_scaled=True) if name == "learned"
+                                            else (lambda z, s: z))
         curve = []
         res, stats = algs.asd_pocs(b, geo, angles, niter, regularizer=reg, return_tensor=True, return_stats=True,
                                    callback=lambda k, r: curve.append(rmse(r)))

@@ -201,9 +201,20 @@ int tigre_get_gpu_name(int device, char* name, int len);
 #define TIGRE_KERNEL_RELAYOUT        4   /* volume bricking / projection quads */
 #define TIGRE_KERNEL_POCS_TV         5   /* both passes of one TV iteration */
 #define TIGRE_KERNEL_TVDENOISE       6
-#define TIGRE_KERNEL_COUNT           7
+#define TIGRE_KERNEL_PROBE           7   /* tigre_valu_probe */
+#define TIGRE_KERNEL_COUNT           8
 int tigre_kernel_timing(int enable);
 int tigre_kernel_time(int kernel, double* total_ms, int* launches);
+
+/* VALU issue-rate probe (no counterpart in the reference; a measurement aid):
+ * `blocks` workgroups of 256 lanes, each lane running `iters` rounds of eight
+ * independent VALU operations with no memory access, then one store to d_out
+ * (blocks x 256 floats).  Mode 0: FP32 FMAs on all 64 lanes of each
+ * wavefront; 1: the same with only lanes 0..15 active; 2: integer add and
+ * compare/select chains; 3: FMA chains interleaved with integer add chains.  Its instruction count is known, so its duration
+ * gives the VALU issue rate against which the operator kernels' instruction
+ * counts are priced. */
+int tigre_valu_probe(float* d_out, int blocks, int iters, int mode, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Projection plans                                                       */

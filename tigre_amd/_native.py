@@ -34,6 +34,7 @@ KERNEL_ATB_MATCHED = 3
 KERNEL_RELAYOUT = 4
 KERNEL_POCS_TV = 5
 KERNEL_TVDENOISE = 6
+KERNEL_PROBE = 7
 
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -138,6 +139,7 @@ SIGNATURES = {
                                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_void_p]),
     "tigre_cnn_wgrad_noalloc_limit": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
+    "tigre_valu_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "tigre_get_gpu_count": (ctypes.c_int, []),
     "tigre_get_gpu_name": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
     "tigre_kernel_timing": (ctypes.c_int, [ctypes.c_int]),
