@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 # OpenMP runtime (torch's or the oracle's) reads its environment.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# The CPUs this process may use, read before any OpenMP runtime starts: with a
+# bound OpenMP, the first runtime to start pins the main thread to one core, and
+# the affinity read afterwards is that one core.
+HOST_CPUS = len(os.sched_getaffinity(0))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
@@ -106,7 +110,7 @@ def log(*a):
 def cpu_threads():
     """Threads for the CPU baseline: every CPU this process may run on, capped by
     OMP_NUM_THREADS when set (the share of a shared host allotted to the job)."""
-    n = len(os.sched_getaffinity(0))
+    n = HOST_CPUS
     cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     return min(n, cap) if cap > 0 else n
 
@@ -121,7 +125,7 @@ def host_description():
                     break
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": HOST_CPUS,
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "omp_places": os.environ.get("OMP_PLACES")}
 
@@ -692,6 +696,7 @@ def bench_precision(args):
         return out, time.perf_counter() - t0
 
     sub = np.array([k * nviews // args.oracle_views for k in range(args.oracle_views)])
+    oracle.set_threads(cpu_threads())
     img_h = img.cpu().numpy()
     gp = prepared(copy_geo(geo, sub), angles[sub])
     t = time.perf_counter()
@@ -821,8 +826,7 @@ def bench_small(args):
     angles = np.linspace(0, 2 * np.pi, 128, endpoint=False)
     img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 10, seed=20261202 + 1)
     gp = prepared(geo, angles)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
+    oracle.set_threads(cpu_threads())
     dimg = torch.as_tensor(img, device="cuda")
     proj_ref = oracle.ax(img, gp, "Siddon").astype(np.float32)
     dproj = torch.as_tensor(proj_ref, device="cuda")
