@@ -256,6 +256,119 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     }
 }
 
+// The residual output layer (32 bf16 channels -> one FP32 channel added to
+// aux) with the taps as the GEMM columns.  P[u][tap] = sum_i x[u][i] w[tap][0][i]
+// for every input voxel u is one MFMA pair per 16 voxels (columns = taps
+// 0..31, 27 used), and y[v] = aux[v] + (sum_tap P[v + tap][tap] + bias[0]).
+// conv_lds_kernel spends 27 MFMAs per 16 outputs on this layer with 15 of its
+// 16 columns idle; here it is 2 per 16 input voxels.  A workgroup owns a
+// 64 x 4 tile of outputs and a chunk of kOutZ planes, and marches through the
+// input planes z0 - 1 .. z0 + kOutZ: for each it computes P over the 66 x 6
+// halo'd tile into LDS; then every lane (one output column) adds the 9-tap
+// in-plane sum of each dz group to the accumulator of the output plane that
+// group feeds (input plane z feeds z + 1 with the dz = -1 taps, z with dz = 0,
+// z - 1 with dz = +1), and writes the output plane that has received all three.
+// Every output sums its groups in the order dz = -1, 0, +1 and each group in
+// (dy, dx) order: the result does not depend on the launch shape.  The next
+// plane's A fragments are loaded while the current plane is summed.
+constexpr int kOutTX = 64, kOutTY = 4, kOutZ = 16;
+constexpr int kOutHX = kOutTX + 2, kOutHY = kOutTY + 2;
+constexpr int kOutU = kOutHX * kOutHY;           // halo'd voxels per plane (396)
+constexpr int kOutBlocks = (kOutU + 15) / 16;     // MFMA row blocks per plane (25)
+constexpr int kOutPerWave = (kOutBlocks + 3) / 4; // row blocks per wavefront, at most (7)
+constexpr int kOutStride = kOutBlocks * 16 + 1;   // LDS row stride of one tap (odd: conflict-free writes)
+
+__global__ __launch_bounds__(256) void conv_out_kernel(const unsigned short* __restrict__ x,
+                                                       const unsigned short* __restrict__ w,
+                                                       const float* __restrict__ bias, const float* __restrict__ aux,
+                                                       float* __restrict__ y, int N, int D, int H, int W) {
+    __shared__ float P[27 * kOutStride];  // [tap][halo'd voxel]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int x0 = blockIdx.x * kOutTX, y0 = blockIdx.y * kOutTY;
+    const int nchunk = (D + kOutZ - 1) / kOutZ;
+    const int zc = (int)(blockIdx.z % (unsigned)nchunk);
+    const long long n = blockIdx.z / (unsigned)nchunk;
+    const int z0 = zc * kOutZ, z1 = min(z0 + kOutZ, D);  // output planes z0 .. z1 - 1
+
+    // B fragments: column 16 hb + m is the tap, k = 8 q .. 8 q + 7 the input channel
+    bf16x8 b[2];
+#pragma unroll
+    for (int hb = 0; hb < 2; ++hb) {
+        const int tap = 16 * hb + m;
+        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
+        if (tap < 27) raw = *reinterpret_cast<const uint4*>(w + (long long)tap * kC * kC + 8 * q);
+        b[hb] = __builtin_bit_cast(bf16x8, raw);
+    }
+    // A fragments of plane z: row block wave + 4 k, voxel 16 (wave + 4 k) + m, channels 8 q ..
+    auto load_plane = [&](int z, bf16x8* a) {
+#pragma unroll
+        for (int k = 0; k < kOutPerWave; ++k) {
+            const int u = 16 * (wave + 4 * k) + m;
+            const int ux = u % kOutHX, uy = u / kOutHX;
+            const int gx = x0 - 1 + ux, gy = y0 - 1 + uy;
+            const bool ok = u < kOutU && gx >= 0 && gx < W && gy >= 0 && gy < H && z >= 0 && z < D;
+            const long long vox = ((n * D + z) * H + gy) * (long long)W + gx;
+            a[k] = load_frag<TIGRE_CNN_IN_BF16X32>(x, vox, q, ok);
+        }
+    };
+    bf16x8 a[kOutPerWave];
+    load_plane(z0 - 1, a);
+
+    const int ox = x0 + lane, oy = y0 + wave;
+    const bool out_ok = ox < W && oy < H;
+    // accumulators of output planes zi - 1, zi, zi + 1 while input plane zi is processed
+    float a_prev = 0.0f, a_cur = 0.0f, a_next = 0.0f;
+    for (int zi = z0 - 1; zi <= z1; ++zi) {
+        if (zi >= 0 && zi < D) {  // uniform per workgroup
+#pragma unroll
+            for (int k = 0; k < kOutPerWave; ++k) {
+                const int blk = wave + 4 * k;
+                if (blk >= kOutBlocks) break;  // uniform per wavefront
+                const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+                const f32x4 r0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k], b[0], zero, 0, 0, 0);
+                const f32x4 r1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k], b[1], zero, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int u = 16 * blk + 4 * q + r;
+                    P[m * kOutStride + u] = r0[r];
+                    if (16 + m < 27) P[(16 + m) * kOutStride + u] = r1[r];
+                }
+            }
+            __syncthreads();  // P complete
+            if (zi + 1 <= z1) load_plane(zi + 1, a);
+            float s[3];
+            const int c = (wave + 1) * kOutHX + lane + 1;  // this lane's voxel in the halo'd tile
+#pragma unroll
+            for (int g = 0; g < 3; ++g) {
+                float t = 0.0f;
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int tap = 9 * g + 3 * (dy + 1) + (dx + 1);
+                        t += P[tap * kOutStride + c + dy * kOutHX + dx];
+                    }
+                s[g] = t;
+            }
+            a_next += s[0];  // dz = -1: output plane zi + 1
+            a_cur += s[1];   // dz = 0:  output plane zi
+            a_prev += s[2];  // dz = +1: output plane zi - 1
+            __syncthreads();  // P is read before the next plane overwrites it
+        } else if (zi + 1 <= z1) {
+            load_plane(zi + 1, a);
+        }
+        const int zo = zi - 1;  // complete: it has its dz = -1, 0 and +1 groups
+        if (zo >= z0 && zo < z1 && out_ok) {
+            const long long vox = ((n * D + zo) * H + oy) * (long long)W + ox;
+            y[vox] = aux[vox] + (a_prev + bias[0]);
+        }
+        a_prev = a_cur;
+        a_cur = a_next;
+        a_next = 0.0f;
+    }
+}
+
 // Convolution of a single FP32 channel (the image, or the loss gradient) into
 // 32 channels.  With one input channel the GEMM k dimension of conv_kernel
 // holds 1 useful channel of 32; here k is the tap instead: A[v][k] =
@@ -463,9 +576,18 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
                          (unsigned)(N * D));
 #define TIGRE_CONV_LDS(E) \
     hipLaunchKernelGGL((conv_lds_kernel<E>), lgrid, dim3(256), 0, stream, xa, w, bias, aux, y, N, D, H, W)
-        if (epi == TIGRE_CNN_EPI_BIAS_RELU) TIGRE_CONV_LDS(TIGRE_CNN_EPI_BIAS_RELU);
-        else if (epi == TIGRE_CNN_EPI_RELU_MASK) TIGRE_CONV_LDS(TIGRE_CNN_EPI_RELU_MASK);
-        else TIGRE_CONV_LDS(TIGRE_CNN_EPI_RESIDUAL);
+        if (epi == TIGRE_CNN_EPI_BIAS_RELU) {
+            TIGRE_CONV_LDS(TIGRE_CNN_EPI_BIAS_RELU);
+        } else if (epi == TIGRE_CNN_EPI_RELU_MASK) {
+            TIGRE_CONV_LDS(TIGRE_CNN_EPI_RELU_MASK);
+        } else {
+            const long long zgrid = (long long)N * ((D + kOutZ - 1) / kOutZ);
+            TIGRE_REQUIRE(zgrid <= 65535 && (H + kOutTY - 1) / kOutTY <= 65535, "volume too large for one launch");
+            const dim3 ogrid((unsigned)((W + kOutTX - 1) / kOutTX), (unsigned)((H + kOutTY - 1) / kOutTY),
+                             (unsigned)zgrid);
+            hipLaunchKernelGGL(conv_out_kernel, ogrid, dim3(256), 0, stream, xa, w, bias,
+                               static_cast<const float*>(aux), static_cast<float*>(y), N, D, H, W);
+        }
 #undef TIGRE_CONV_LDS
     } else {
         TIGRE_REQUIRE((long long)N * D <= 65535, "too many planes for one launch");
