@@ -39,7 +39,8 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_512cube_360views.t
 # VALU / LDS instruction counts per launch from a rocprofv3 SQ_* counter pass; parsed when present.
 VALU_SUMMARY = os.path.join(ROOT, "profiles", "valu_lds_512cube_360views.txt")
 PROBE_BLOCKS, PROBE_ITERS = 8192, 4096
-# VALU instructions per wavefront of the probe: 8 FMAs per round plus 26 outside the loop
+# VALU instructions per wavefront of the probe (its ISA): mode 0, eight FP32 FMA chains, and mode 3,
+# four FMA chains interleaved with four integer adds, both 8 per loop round plus 26 outside the loop
 PROBE_VALU_PER_WAVE = 8 * PROBE_ITERS + 26
 
 
@@ -72,9 +73,11 @@ def measured_valu(kernel):
 
 
 def valu_issue_rate(lib, dev):
-    """Measured VALU issue rate (wavefront instructions per second) of the
+    """Measured VALU issue rates (wavefront instructions per second) of the
     probe kernel: PROBE_BLOCKS x 4 wavefronts, PROBE_VALU_PER_WAVE each, timed
-    by the library's HIP events on the launch stream (median of 5)."""
+    by the library's HIP events on the launch stream (median of 5), for a pure
+    FP32-FMA stream (mode 0) and for FMAs interleaved with integer adds (mode
+    3, the higher of the two: a SIMD issues the mix faster than FMAs alone)."""
     import ctypes
 
     import numpy as np
@@ -85,22 +88,26 @@ def valu_issue_rate(lib, dev):
 
     out = torch.empty(PROBE_BLOCKS * 256, device=dev)
     total, count = ctypes.c_double(0.0), ctypes.c_int(0)
-    _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, 0,
-                                       _stream_handle()), "probe")
-    torch.cuda.synchronize()
-    lib.tigre_kernel_timing(1)
-    ms = []
-    for _ in range(5):
-        _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, 0,
-                                           _stream_handle()), "probe")
-        _native.check(lib.tigre_kernel_time(_native.KERNEL_PROBE, ctypes.byref(total), ctypes.byref(count)),
-                      "kernel_time")
-        ms.append(total.value / count.value)
-    lib.tigre_kernel_timing(0)
     waves = PROBE_BLOCKS * 4
-    t = float(np.median(ms))
-    return {"wave_instr_per_s": waves * PROBE_VALU_PER_WAVE / (t * 1e-3), "probe_ms": round(t, 4),
-            "waves": waves, "valu_per_wave": PROBE_VALU_PER_WAVE}
+    res = {"waves": waves, "valu_per_wave": PROBE_VALU_PER_WAVE}
+    for mode, key in ((0, "fma"), (3, "mixed")):
+        _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, mode,
+                                           _stream_handle()), "probe")
+        torch.cuda.synchronize()
+        lib.tigre_kernel_timing(1)
+        ms = []
+        for _ in range(5):
+            _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, mode,
+                                               _stream_handle()), "probe")
+            _native.check(lib.tigre_kernel_time(_native.KERNEL_PROBE, ctypes.byref(total), ctypes.byref(count)),
+                          "kernel_time")
+            ms.append(total.value / count.value)
+        lib.tigre_kernel_timing(0)
+        t = float(np.median(ms))
+        res[key + "_ms"] = round(t, 4)
+        res[key + "_wave_instr_per_s"] = waves * PROBE_VALU_PER_WAVE / (t * 1e-3)
+    res["wave_instr_per_s"] = max(res["fma_wave_instr_per_s"], res["mixed_wave_instr_per_s"])
+    return res
 
 
 def log(*a):
@@ -155,7 +162,12 @@ def main():
                          "learned: ASD-POCS with the learned proximal step vs plain TV at 256^3; "
                          "precision: BF16 / FP16 / FP32 storage at 1024^3 with CGLS; "
                          "small: every operator at 128^3 beside the full FP64 oracle")
-    ap.add_argument("--train-steps", type=int, default=1000, help="learned: Adam steps")
+    ap.add_argument("--train-steps", type=int, default=12000, help="learned: Adam steps")
+    ap.add_argument("--learned-cases", type=int, default=8, help="learned: training phantoms")
+    ap.add_argument("--learned-first", type=int, default=3,
+                    help="learned: pre-TV images of the first iterations paired with the final image")
+    ap.add_argument("--learned-crops", type=int, default=6000, help="learned: 32^3 training crops")
+    ap.add_argument("--learned-attempts", type=int, default=3, help="learned: calls that try the network")
     ap.add_argument("--cgls-iters", type=int, default=30, help="precision: CGLS iterations per storage type")
     ap.add_argument("--oracle-views", type=int, default=8, help="precision: views checked against the FP64 oracle")
     ap.add_argument("--no-workloads", action="store_true",
@@ -313,7 +325,8 @@ def main():
         del a, b, c
 
     probe = valu_issue_rate(lib, dev)
-    log("VALU probe: %.4g wavefront instructions/s (%.3f ms)" % (probe["wave_instr_per_s"], probe["probe_ms"]))
+    log("VALU probe: %.4g (FMA) / %.4g (FMA + integer) wavefront instructions/s" % (
+        probe["fma_wave_instr_per_s"], probe["mixed_wave_instr_per_s"]))
 
     b_ax = nviews * (V + P) * 4.0
     b_atb = nviews * (2 * V + P) * 4.0
@@ -549,12 +562,17 @@ def bench_learned(args):
 
     256^3 volume (1 mm voxels), 180 views over 2*pi, 256 x 256 detector at
     1.6 mm, DSO 1000 / DSD 1536 mm; seeded 40-ellipsoid phantoms scaled to
-    attenuation 0.02/mm per unit density; Poisson noise at I0 = 1e5 photons.
-    Training: (pre-TV, minimizeTV) pairs from plain ASD-POCS (tviter 20, alpha
-    0.002) on four phantoms, 2000 random 32^3 crops, Adam.  Evaluation on a
+    attenuation 0.02/mm per unit density; Poisson noise at I0 = 1e5 photons;
+    ASD-POCS with tviter 20, alpha 0.002.  Training data: plain ASD-POCS runs
+    of up to 40 iterations on ``learned_cases`` phantoms; pairs (pre-TV image
+    of iteration k, the run's final post-TV image) for the first
+    ``learned_first`` iterations k; random 32^3 crops (mostly inside the
+    object), Adam on the library's MFMA convolution kernels.  Evaluation on a
     held-out phantom: image RMSE after every iteration of plain ASD-POCS, of
-    ASD-POCS with the learned step (TV fallback on), and of the loop with no
-    regularization step; time to reach the plain run's final RMSE."""
+    ASD-POCS with the learned step (plain TV judging each learned step by the
+    data constraint and TV objective, at most ``learned_attempts`` attempts,
+    ended by the first fallback), and of the loop with no regularization
+    step; time to reach the plain run's final RMSE."""
     import numpy as np
     import torch
 
@@ -576,33 +594,46 @@ def bench_learned(args):
 
     t = time.perf_counter()
     pairs = []
-    for k in range(1, 5):
-        pairs += L.collect_pairs(case(k)[1], geo, angles, 10)
-    x, target = L.make_patches(pairs, 2000, 32, seed=seed0)
+    for k in range(1, args.learned_cases + 1):
+        pre, post = L.collect_trajectory(case(k)[1], geo, angles, 40)
+        pairs += [(x, post[-1], 1.0) for x in pre[:args.learned_first]]
+        del pre, post
+    x, target = L.make_patches(pairs, args.learned_crops, 32, seed=seed0, object_level=2e-3)
     del pairs
     torch.cuda.synchronize()
     t_pairs = time.perf_counter() - t
     log("pairs: %d crops in %.1f s" % (x.shape[0], t_pairs))
-    net = L.ResidualCNN(seed=seed0)
+    net = L.ResidualCNN(seed=seed0, data_scale=50.0)
     t = time.perf_counter()
-    losses = L.train(net, x, target, args.train_steps, batch=8, lr=1e-3, seed=seed0, log=log)
+    losses = L.train(net, x, target, args.train_steps, batch=8, lr=3e-4, seed=seed0, lr_final=3e-5,
+                     log=lambda *a: None)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t
+    log("training: %d steps in %.1f s, loss %.3g -> %.3g" % (args.train_steps, t_train, float(np.mean(losses[:200])),
+                                                            float(np.mean(losses[-200:]))))
+    del x, target
 
     img, b = case(0)
 
     def rmse(a):
         return float(torch.sqrt(torch.mean((a - img).double() ** 2)))
 
+    resid = L.data_residual(b, geo, angles)
+    eps = L.data_epsilon(b, geo, angles, resid)
     runs = {}
     for name in ("plain", "learned", "no_step"):
-        reg = None if name == "plain" else (L.LearnedProx(net, 20, step_scaled=True) if name == "learned"
-                                            else (lambda z, s: z))
+        if name == "plain":
+            reg = None
+        elif name == "learned":
+            reg = L.LearnedProx(net, 20, residual=resid, judge="tv", epsilon=eps, attempts=args.learned_attempts,
+                                latch=True)
+        else:
+            reg = lambda z, s: z  # noqa: E731
         curve = []
         res, stats = algs.asd_pocs(b, geo, angles, niter, regularizer=reg, return_tensor=True, return_stats=True,
                                    callback=lambda k, r: curve.append(rmse(r)))
         times = [float(v) for v in np.cumsum([s_[5] for s_ in stats])]
-        runs[name] = {"rmse": [round(c, 6) for c in curve], "seconds": [round(v, 4) for v in times]}
+        runs[name] = {"rmse": [round(c, 7) for c in curve], "seconds": [round(v, 4) for v in times]}
         if name == "learned":
             runs[name]["taken"], runs[name]["fallbacks"] = reg.taken, reg.fallbacks
         log("%s: final RMSE %.6f after %d iterations, %.2f s" % (name, curve[-1], len(curve), times[-1]))
@@ -629,8 +660,8 @@ def bench_learned(args):
     V = float(n) ** 3
     flops_model = 2.0 * 27 * (32 + 4 * 32 * 32 + 32) * V  # useful multiply-adds of the 6 layers
     # MFMA work issued: four 32 -> 32 layers in full, the first layer with its 27 taps packed into
-    # k = 32, the last (32 -> 1) on one 16-column MFMA block
-    flops_exec = (4 * 2.0 * 27 * 32 * 32 + 2.0 * 32 * 32 + 2.0 * 27 * 32 * 16) * V
+    # k = 32, the last (32 -> 1) with the taps as the 32 GEMM columns
+    flops_exec = (4 * 2.0 * 27 * 32 * 32 + 2.0 * 32 * 32 + 2.0 * 32 * 32) * V
     result = {
         "metric": "ASD-POCS wall-clock to the plain-TV RMSE, learned proximal step vs TV",
         "value": round(t_plain / t_learned, 3) if t_learned else 0.0, "unit": "x (speedup; 0 = RMSE not reached)",
@@ -638,14 +669,17 @@ def bench_learned(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (CNN), f32 (projectors)",
         "data": "synthetic (seeded 40-ellipsoid phantoms, Poisson noise I0 1e5)",
         "config": {"workload": "ASD-POCS 256^3, 180 views, 256x256 detector, tviter 20, alpha 0.002; 6-layer "
-                               "32-channel residual CNN", "train_crops": int(x.shape[0]), "crop": 32,
-                   "train_steps": args.train_steps},
+                               "32-channel residual CNN", "train_cases": args.learned_cases,
+                   "train_iterates": args.learned_first, "train_crops": args.learned_crops, "crop": 32,
+                   "train_steps": args.train_steps, "attempts": args.learned_attempts},
         "plain_final_rmse": goal, "plain_seconds": round(t_plain, 3),
         "learned_seconds_to_plain_rmse": round(t_learned, 3) if t_learned else None,
+        "learned_iterations_to_plain_rmse": reach[0] + 1 if reach else None,
         "runs": runs, "train": {"pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2),
-                                "loss_first": losses[0], "loss_last": losses[-1]},
+                                "loss_first": float(np.mean(losses[:200])), "loss_last": float(np.mean(losses[-200:]))},
         "step_ms": {"cnn_256cube": round(ms_cnn, 3), "minTV_20iter_256cube": round(ms_tv, 3)},
-        "roofline": {"bound": "mfma", "kernel": "conv_lds_kernel + conv_taps_kernel (6 layers, useful FLOPs)",
+        "roofline": {"bound": "mfma", "kernel": "conv_lds_kernel + conv_taps_kernel + conv_out_kernel (6 layers, "
+                                                "useful FLOPs)",
                      "achieved": round(flops_model / ms_cnn / 1e9, 1),
                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(flops_model / ms_cnn / 1e9 / 2500.0, 4),
                      "traffic": None, "model_tflop": round(flops_model / 1e12, 3),

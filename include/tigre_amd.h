@@ -211,7 +211,7 @@ int tigre_kernel_time(int kernel, double* total_ms, int* launches);
  * independent VALU operations with no memory access, then one store to d_out
  * (blocks x 256 floats).  Mode 0: FP32 FMAs on all 64 lanes of each
  * wavefront; 1: the same with only lanes 0..15 active; 2: integer add and
- * compare/select chains; 3: FMA chains interleaved with integer add chains.  Its instruction count is known, so its duration
+ * compare/select chains; 3: FMA chains interleaved with a ring of integer adds.  Its instruction count is known, so its duration
  * gives the VALU issue rate against which the operator kernels' instruction
  * counts are priced. */
 int tigre_valu_probe(float* d_out, int blocks, int iters, int mode, void* stream);

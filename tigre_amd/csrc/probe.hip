@@ -13,7 +13,8 @@ namespace {
 // kMode 0: all 64 lanes active; 1: lanes 0..15 only (one 16-lane group of the
 // wavefront, the rest masked off); 2: four independent integer chains of add,
 // compare and select (12 instructions per round); 3: four FMA chains interleaved
-// with four integer add chains (8 instructions per round).
+// with a ring of four integer adds (8 instructions per round: 4 v_fma_f32,
+// 3 v_add_u32, 1 v_xad_u32).
 template <int kMode>
 __global__ __launch_bounds__(256) void valu_probe_kernel(float* __restrict__ out, int iters, float b, float c) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;

@@ -224,8 +224,8 @@ def tv_value(x, eps=1e-8):
 class LearnedProx:
     """ASD-POCS ``regularizer``: the learned step, with plain TV as fallback.
 
-    The learned result y = net(x) (``step_scaled``: x + dtvg F(x), the form
-    trained to imitate one TV call) is kept when
+    ``judge="input"``: the learned result y = net(x) (``step_scaled``: x +
+    dtvg F(x), the form trained to imitate one TV call) is kept when
 
     * its TV objective is below that of the step's input x, and
     * if ``residual`` (a callable image -> data residual norm |A y - b|) is
@@ -233,23 +233,69 @@ class LearnedProx:
       previous step's output by more than the fraction ``tol`` (the first step
       has no previous output and passes);
 
-    otherwise the step is minTV(x, dtvg, tviter), plain TV.  ``taken`` /
-    ``fallbacks`` count the two outcomes."""
+    otherwise the step is minTV(x, dtvg, tviter), plain TV.
 
-    def __init__(self, net, tviter=20, residual=None, tol=1e-3, step_scaled=False):
+    ``judge="tv"`` (for a network trained to jump ahead of the iteration):
+    plain TV's own step t = minTV(x, dtvg, tviter) is computed every time and
+    judges y by the problem ASD-POCS solves, the smallest TV objective among
+    images whose data residual is at most ``epsilon``.  y is kept when it is
+    closer to that data constraint than t (max(|A y - b| - epsilon, 0) below
+    t's), or when both meet it and y's TV objective does not exceed t's;
+    otherwise t is returned.  It needs ``residual`` and ``epsilon``.
+
+    ``attempts`` bounds the number of calls that try the network (None: all);
+    with ``latch`` the first fallback ends the attempts.  The calls after the
+    attempts are plain minTV, with no network or judge work.
+
+    ``taken`` / ``fallbacks`` count the two outcomes of the attempts;
+    ``history`` holds one (residual of y, residual of t, TV of y, TV of t,
+    kept) tuple per attempt under ``judge="tv"``."""
+
+    def __init__(self, net, tviter=20, residual=None, tol=1e-3, step_scaled=False, judge="input", epsilon=None,
+                 attempts=None, latch=False):
+        if judge not in ("input", "tv"):
+            raise ValueError("judge must be 'input' or 'tv'")
+        if judge == "tv" and (residual is None or epsilon is None):
+            raise ValueError("judge='tv' needs residual and epsilon")
         self.net = net
         self.tviter = tviter
         self.residual = residual
         self.tol = tol
         self.step_scaled = step_scaled
+        self.judge = judge
+        self.epsilon = epsilon
         self.taken = 0
         self.fallbacks = 0
         self.last_residual = None
+        self.history = []
+        self.attempts = attempts
+        self.latch = latch
+        self._closed = False
 
     def __call__(self, img, dtvg):
         from .tv import minTV
 
+        if self._closed or (self.attempts is not None and self.taken + self.fallbacks >= self.attempts):
+            return minTV(img, dtvg, self.tviter)
+        if self.latch and self.fallbacks > 0:
+            self._closed = True
+            return minTV(img, dtvg, self.tviter)
         y = self.net.forward(img, scale=float(dtvg)) if self.step_scaled else self.net.apply(img)
+        if self.judge == "tv":
+            t = minTV(img, dtvg, self.tviter)
+            ry, rt = self.residual(y), self.residual(t)
+            iy, it = max(ry - self.epsilon, 0.0), max(rt - self.epsilon, 0.0)
+            if iy > 0.0 or it > 0.0:
+                ok, ty, tt = iy < it, None, None
+            else:
+                ty, tt = tv_value(y), tv_value(t)
+                ok = ty <= tt
+            self.history.append((ry, rt, ty, tt, bool(ok)))
+            if ok:
+                self.taken += 1
+                return y
+            self.fallbacks += 1
+            return t
         ok = tv_value(y) < tv_value(img)
         r = None
         if ok and self.residual is not None:
@@ -264,6 +310,16 @@ class LearnedProx:
         if self.residual is not None:
             self.last_residual = self.residual(out)
         return out
+
+
+def data_epsilon(proj, geo, angles, residual=None):
+    """The data-residual tolerance asd_pocs uses when maxl2err is not given:
+    0.2 |A FDK(proj) - proj| (pocs_algorithms.py:115-121 restated in
+    tigre_amd.algorithms.asd_pocs)."""
+    from .algorithms import fdk
+
+    r = residual if residual is not None else data_residual(proj, geo, angles)
+    return 0.2 * r(fdk(proj, geo, angles, return_tensor=True))
 
 
 def data_residual(proj, geo, angles, projection_type="Siddon"):
@@ -305,39 +361,62 @@ def collect_pairs(proj, geo, angles, niter, tviter=20, alpha=0.002, **kwargs):
     return pairs
 
 
-def collect_far_pairs(proj, geo, angles, niter, first, tviter=20, alpha=0.002, **kwargs):
-    """Pairs a plain ASD-POCS run generates further ahead: (pre-TV image of
-    iteration k, the run's final post-TV image after ``niter`` iterations, 1.0)
-    for k = 1 .. ``first``.  Used with make_patches, the target of the crop of
-    x_k is the crop of the final image, so the network learns to take an early
-    iterate where plain ASD-POCS arrives many TV steps and data sweeps later."""
+def collect_trajectory(proj, geo, angles, niter, tviter=20, alpha=0.002, **kwargs):
+    """The images of a plain ASD-POCS run: (pre-TV images, post-TV images), one
+    of each per iteration (the run may stop before ``niter`` on the reference's
+    criteria)."""
     from .algorithms import asd_pocs
     from .tv import minTV
 
-    early = []
+    pre, post = [], []
 
     def recording(img, step):
-        if len(early) < first:
-            early.append(img.clone())
-        return minTV(img, step, tviter)
+        out = minTV(img, step, tviter)
+        pre.append(img.clone())
+        post.append(out.clone())
+        return out
 
-    final = asd_pocs(proj, geo, angles, niter, alpha=alpha, tviter=tviter, regularizer=recording, return_tensor=True,
-                     **kwargs)
-    return [(x, final, 1.0) for x in early]
+    asd_pocs(proj, geo, angles, niter, alpha=alpha, tviter=tviter, regularizer=recording, return_tensor=True,
+             **kwargs)
+    return pre, post
 
 
-def make_patches(pairs, count, size, seed=0):
+def jump_pairs(trajectory, ahead, first=None):
+    """Pairs a plain ASD-POCS run generates further ahead: (pre-TV image of
+    iteration k, post-TV image of iteration k + ``ahead`` (the run's last if it
+    stopped earlier), 1.0) for the first ``first`` iterations k (all by
+    default).  With make_patches the target of a crop of x_k is the same crop
+    ``ahead`` TV steps and data sweeps later, so the network learns to take the
+    iterate where plain ASD-POCS would arrive only after them."""
+    pre, post = trajectory
+    last = len(post) - 1
+    ks = range(len(pre) if first is None else min(first, len(pre)))
+    return [(pre[k], post[min(k + ahead, last)], 1.0) for k in ks]
+
+
+def collect_far_pairs(proj, geo, angles, niter, first, tviter=20, alpha=0.002, **kwargs):
+    """(pre-TV image of iteration k, the run's final post-TV image, 1.0) for
+    k = 1 .. ``first`` of a plain ASD-POCS run of ``niter`` iterations."""
+    pre, post = collect_trajectory(proj, geo, angles, niter, tviter, alpha, **kwargs)
+    return jump_pairs((pre, post), niter, first)
+
+
+def make_patches(pairs, count, size, seed=0, object_level=None, air_keep=0.2):
     """``count`` random size^3 crops: inputs x and training targets x + (post - x) / step
-    (post = minimizeTV(x, dtvg) with step = dtvg, or a later iterate with step = 1)."""
+    (post = minimizeTV(x, dtvg) with step = dtvg, or a later iterate with step = 1).
+    With ``object_level``, a crop whose target mean is below it (mostly air
+    around the object) is kept only with probability ``air_keep``."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     xs, ts = [], []
-    for k in range(count):
+    while len(xs) < count:
         pre, post, step = pairs[int(rng.integers(len(pairs)))]
         D, H, W = pre.shape
         z, y, x = (int(rng.integers(0, max(1, n - size + 1))) for n in (D, H, W))
         a = pre[z:z + size, y:y + size, x:x + size]
         b = post[z:z + size, y:y + size, x:x + size]
+        if object_level is not None and float(b.mean()) < object_level and rng.random() >= air_keep:
+            continue
         xs.append(a)
         ts.append(a + (b - a) / step)
     return torch.stack(xs).contiguous(), torch.stack(ts).contiguous()
