@@ -39,9 +39,10 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_512cube_360views.t
 # VALU / LDS instruction counts per launch from a rocprofv3 SQ_* counter pass; parsed when present.
 VALU_SUMMARY = os.path.join(ROOT, "profiles", "valu_lds_512cube_360views.txt")
 PROBE_BLOCKS, PROBE_ITERS = 8192, 4096
-# VALU instructions per wavefront of the probe (its ISA): mode 0, eight FP32 FMA chains, and mode 3,
-# four FMA chains interleaved with four integer adds, both 8 per loop round plus 26 outside the loop
-PROBE_VALU_PER_WAVE = 8 * PROBE_ITERS + 26
+# VALU instructions per wavefront of the probe (its ISA, and SQ_INSTS_VALU exactly): mode 0, eight
+# FP32 FMA chains, 8 per loop round plus 26 outside the loop; mode 3, four FMA chains interleaved with
+# four integer adds, 8 per round plus 25 outside
+PROBE_VALU_PER_WAVE = {0: 8 * PROBE_ITERS + 26, 3: 8 * PROBE_ITERS + 25}
 
 
 def measured_traffic(kernel):
@@ -89,7 +90,7 @@ def valu_issue_rate(lib, dev):
     out = torch.empty(PROBE_BLOCKS * 256, device=dev)
     total, count = ctypes.c_double(0.0), ctypes.c_int(0)
     waves = PROBE_BLOCKS * 4
-    res = {"waves": waves, "valu_per_wave": PROBE_VALU_PER_WAVE}
+    res = {"waves": waves, "valu_per_wave": {"fma": PROBE_VALU_PER_WAVE[0], "mixed": PROBE_VALU_PER_WAVE[3]}}
     for mode, key in ((0, "fma"), (3, "mixed")):
         _native.check(lib.tigre_valu_probe(ctypes.c_void_p(out.data_ptr()), PROBE_BLOCKS, PROBE_ITERS, mode,
                                            _stream_handle()), "probe")
@@ -105,7 +106,7 @@ def valu_issue_rate(lib, dev):
         lib.tigre_kernel_timing(0)
         t = float(np.median(ms))
         res[key + "_ms"] = round(t, 4)
-        res[key + "_wave_instr_per_s"] = waves * PROBE_VALU_PER_WAVE / (t * 1e-3)
+        res[key + "_wave_instr_per_s"] = waves * PROBE_VALU_PER_WAVE[mode] / (t * 1e-3)
     res["wave_instr_per_s"] = max(res["fma_wave_instr_per_s"], res["mixed_wave_instr_per_s"])
     return res
 

@@ -37,8 +37,12 @@ __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Waves per SIMD the gather kernel is compiled for: the register budget that
 // keeps all kZ gathers of a column in flight (the compiler otherwise trades
-// them for occupancy and serialises the loads).
-constexpr int atb_waves(int kW, int kZ) { return kZ >= 16 ? (kW == 1 ? 3 : 4) : kZ == 8 ? (kW == 1 ? 5 : 6) : 8; }
+// them for occupancy and serialises the loads).  The matched weight with
+// tilted views and 16 slices needs more than the 168 registers of three
+// waves (it spilled), so it gets two.
+constexpr int atb_waves(int kW, bool kAxial, int kZ) {
+    return kZ >= 16 ? (kW == 1 ? (kAxial ? 3 : 2) : 4) : kZ == 8 ? (kW == 1 ? 5 : 6) : 8;
+}
 
 // Matched weight l2^(3/2) / s2 from the DSD-normalised squares l2 (source to
 // detector point, about 1) and s2 (source to voxel, about (DSO/DSD)^2): one
@@ -110,7 +114,7 @@ __device__ __forceinline__ void store_column(const float (&acc)[kZ], float* vol,
 // orthogonally, u = y + nU/2 - 0.5, v = z + nV/2 - 0.5, weight 1
 // (voxel_backprojection_parallel.cu:221-251).
 template <int kW, bool kAxial, int kZ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(kW, kZ), atb_waves(kW, kZ)))) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(atb_waves(kW, kAxial, kZ), atb_waves(kW, kAxial, kZ)))) void atb_kernel(const float* __restrict__ proj, int nU, int nV, int nviews,
                                                   const BwdView* __restrict__ views, float* vol, int nx,
                                                   int ny, int nz, float scale, AtbUpdate ep) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
