@@ -679,7 +679,7 @@ def bench_learned(args):
         "runs": runs, "train": {"pairs_s": round(t_pairs, 2), "train_s": round(t_train, 2),
                                 "loss_first": float(np.mean(losses[:200])), "loss_last": float(np.mean(losses[-200:]))},
         "step_ms": {"cnn_256cube": round(ms_cnn, 3), "minTV_20iter_256cube": round(ms_tv, 3)},
-        "roofline": {"bound": "mfma", "kernel": "conv_lds_kernel + conv_taps_kernel + conv_out_kernel (6 layers, "
+        "roofline": {"bound": "mfma", "kernel": "conv_zm_kernel + conv_taps_kernel + conv_out_kernel (6 layers, "
                                                 "useful FLOPs)",
                      "achieved": round(flops_model / ms_cnn / 1e9, 1),
                      "peak": 2500.0, "unit": "TFLOP/s", "frac": round(flops_model / ms_cnn / 1e9 / 2500.0, 4),

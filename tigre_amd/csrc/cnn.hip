@@ -2,19 +2,21 @@
 //
 // The network (a residual CNN that stands in for the TV step of ASD-POCS) is
 // six 3x3x3 'same' convolutions with 32 channels between them.  All of them,
-// forward and backward, run through two implicit-GEMM kernels built on
-// v_mfma_f32_16x16x32_bf16 (A: 16 rows x 32 k, B: 32 k x 16 columns, FP32
-// accumulate; lane l holds row / column l % 16 and k = 8 (l / 16) .. + 7, and
-// result register r of lane l is row 4 (l / 16) + r, column l % 16):
+// forward and backward, run as implicit GEMMs on v_mfma_f32_16x16x32_bf16
+// (A: 16 rows x 32 k, B: 32 k x 16 columns, FP32 accumulate; lane l holds
+// row / column l % 16 and k = 8 (l / 16) .. + 7, and result register r of lane
+// l is row 4 (l / 16) + r, column l % 16):
 //
-//  * conv_kernel: y[v][o] = sum_tap sum_i x[v + tap][i] w[tap][o][i].  GEMM
-//    rows are 16 consecutive voxels along x, columns the 32 output channels
-//    (two MFMA column blocks), k one tap's 32 input channels: one 16-byte
-//    load per lane per tap is an A fragment (8 channels of one voxel).  A
-//    wavefront computes four such tiles (64 voxels) and reuses each tap's
-//    weight fragments across them.  Inputs are activations (bf16, 32
-//    channels, channel fastest) or a single FP32 channel (the image, or the
-//    loss gradient), widened into channel 0 of the fragment.
+//  * conv_zm_kernel (32 -> 32 channels): rows 16 voxels along x, columns the
+//    32 output channels, k one tap's 32 input channels; input planes staged
+//    in LDS once each while the workgroup marches along z.
+//  * conv_taps_kernel (one FP32 channel -> 32): k is the tap instead, so one
+//    MFMA covers all 27 taps of 16 voxels.
+//  * conv_out_kernel (32 -> one FP32 channel added to a residual input): the
+//    taps are the columns instead, P[u][tap] for every input voxel, then
+//    in-plane tap sums in LDS.
+//  * conv_kernel: the plain gather form (one 16-byte load per lane per tap is
+//    an A fragment), for the single-channel input with the residual epilogue.
 //  * wgrad_kernel: dw[tap][o][i] = sum_v x[v + tap][i] dy[v][o], GEMM k =
 //    32 voxels of an x-row.  The x and dy rows are staged in LDS transposed
 //    (channel-major), so fragments are again 16-byte reads.  Tap 27 is the
@@ -24,6 +26,7 @@
 // {-1, 0, 1}^3 as 9 (dz + 1) + 3 (dy + 1) + (dx + 1); outside the volume the
 // input is zero.
 #include <mutex>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -122,114 +125,57 @@ __global__ __launch_bounds__(256) void conv_kernel(const void* __restrict__ x, c
     }
 }
 
-// Convolution of 32-channel bf16 activations with the input staged in LDS.
-// conv_kernel above reads every input fragment from global memory once per
-// tap (27 times per voxel), which bounds it by the texture-address path.
-// Here a workgroup computes 4 output rows of 128 voxels and, for each of the
-// three input planes z - 1, z, z + 1 in turn, copies the 6 x 130 voxel block
-// its taps read into LDS with direct global-to-LDS loads (voxels outside the
-// volume are copied from a zero chunk).  Each wavefront owns 32 voxels (two
-// MFMA tiles) of all four rows, so an A fragment read from staged row r feeds
-// every output row that reads r (rows r - 2 .. r, one dy tap each): per plane
-// and dx, 12 fragment reads serve 48 MFMAs.  The 16-byte channel groups of a
+// Staged input planes of the 32-channel convolutions.  A workgroup computes
+// 4 output rows of 128 voxels; the 6 x 130 voxel block of an input plane its
+// taps read is copied into LDS with direct global-to-LDS loads (voxels outside
+// the volume are copied from a zero chunk).  The 16-byte channel groups of a
 // voxel are stored in the order q ^ ((x >> 2) & 3): the 16 lanes of one group
 // read 16 consecutive voxels and then fall on distinct LDS banks.  A fragment
-// address is a per-lane constant for each dx, plus an immediate offset for
-// the tile and the row.
+// address is a per-lane constant for each dx, plus an immediate offset for the
+// tile and the row.
 constexpr int kLdsTiles = 8;                    // 16-voxel tiles per workgroup row
-constexpr int kWaveTiles = 2;                   // tiles per wavefront (of each of the 4 rows)
 constexpr int kLdsBX = 16 * kLdsTiles + 2;      // staged voxels along x (with the halo)
 constexpr int kLdsRows = 4 + 2;                 // staged rows: the workgroup's 4 and the halo
 constexpr int kLdsChunks = kLdsRows * kLdsBX * 4;  // 16-byte chunks of one staged plane
 
 __device__ uint4 g_zero_chunk[4];  // zero-initialised, never written: the source of the zero halo
 
+// The 32 -> 32 channel convolution (forward with bias and ReLU; data gradient
+// with the ReLU mask), implicit GEMM on v_mfma_f32_16x16x32_bf16: rows = 16
+// voxels along x, columns = 32 output channels (two MFMA blocks), k = one
+// tap's 32 input channels.  A workgroup owns a 4 x 128 voxel tile for a chunk
+// of kZmChunk output planes and marches through its input planes z0 - 1 ..
+// z0 + kZmChunk once: each input plane is staged once, into one of two LDS
+// buffers while the previous plane is computed, and feeds the accumulators of
+// the three output planes it contributes to (z + 1 through its dz = -1 taps,
+// z through dz = 0, z - 1 through dz = +1); the output plane that has
+// received all three is written.  The staged row sr feeds output row sr - dy
+// for each dy, so one A fragment read serves up to 3 dz x 3 dy x 2 column
+// blocks of MFMAs.  The 27 taps' weights are staged in LDS once per workgroup
+// (with the same channel-group swizzle as the activations: a B-fragment read
+// is conflict free).  LDS: two planes (97.5 KiB, separate arrays, so the
+// compiler does not wait for the copies into one before reading the other)
+// and the weights (54 KiB): one workgroup per CU of 8 wavefronts, each owning
+// one 16-voxel tile of the 4 rows and the accumulators of three output planes.
+// Every output voxel sums its taps in the order dz = -1, 0, +1, then dx, then
+// dy.  (Staging the three input planes of each output plane in turn, with a
+// full wait between copy and use, kept the MFMA pipes busy 31 % of the time:
+// 1.39 ms per 256^3 layer against 0.93 ms here.)
+constexpr int kZmChunk = 16;  // output planes per workgroup
+constexpr int kZmTiles = 1;   // 16-voxel tiles per wavefront (of each of the 4 rows)
+constexpr int kZmThreads = 64 * kLdsTiles / kZmTiles;
+
 template <int kEpi>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void conv_lds_kernel(const unsigned short* __restrict__ x,
-                                                       const unsigned short* __restrict__ w,
-                                                       const float* __restrict__ bias, const void* __restrict__ aux,
-                                                       void* __restrict__ y, int N, int D, int H, int W) {
-    __shared__ uint4 plane[kLdsChunks];  // [row][voxel][channel group, swizzled]
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m = lane & 15, q = lane >> 4;
-    const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
-    const int zz0 = (int)(blockIdx.z % (unsigned)D);
-    const long long n = blockIdx.z / (unsigned)D;
-    const int xw = x0 + 16 * kWaveTiles * wave;  // this wavefront's first output voxel (all four rows)
-
-    // A-fragment byte offsets in the plane for staged row 0, this wavefront's
-    // first tile and dx = d - 1.  Its second tile adds 16 voxels (1024 bytes;
-    // the swizzle depends on (x >> 2) & 3, which 16 voxels leave unchanged) and
-    // staged row r adds r kLdsBX 64 bytes: both are immediate offsets.
-    unsigned aoff[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const int xi = 16 * kWaveTiles * wave + m + d;
-        aoff[d] = (unsigned)((xi * 4 + (q ^ ((xi >> 2) & 3))) * 16);
-    }
-    // acc[row][tile][output channel block]
-    f32x4 acc[4][kWaveTiles][2];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int t = 0; t < kWaveTiles; ++t) acc[r][t][0] = acc[r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const char* lds = reinterpret_cast<const char*>(plane);
-
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int zz = zz0 + dz;
-        if (zz < 0 || zz >= D) continue;  // uniform: the plane contributes nothing
-        __syncthreads();                  // the previous plane's reads are done
-        // chunk i of the plane: row i / (4 kLdsBX), voxel (i / 4) % kLdsBX, stored group i % 4
-        const long long zrow = (n * D + zz) * (long long)H;
-        for (int i0 = 0; i0 < kLdsChunks; i0 += 256) {
-            const int i = i0 + tid;
-            if (i < kLdsChunks) {
-                const int yi = i / (4 * kLdsBX), r = i - yi * 4 * kLdsBX, xi = r >> 2, sg = r & 3;
-                const int gy = y0 - 1 + yi, gx = x0 - 1 + xi;
-                const void* src = g_zero_chunk;
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    src = x + ((zrow + gy) * W + gx) * kC + 8 * (sg ^ ((xi >> 2) & 3));
-                __builtin_amdgcn_global_load_lds(src, &plane[i0 + wave * 64], 16, 0, 0);
-            }
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            // weight fragments of the three dy taps of this (dz, dx)
-            bf16x8 b[3][2];
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                const unsigned short* wt = w + (long long)((dz + 1) * 9 + dy * 3 + d) * kC * kC;
-                b[dy][0] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + m * kC + 8 * q));
-                b[dy][1] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wt + (16 + m) * kC + 8 * q));
-            }
-            // staged row sr feeds output row orow = sr - dy for dy = 0, 1, 2 (orow in 0 .. 3)
-#pragma unroll
-            for (int sr = 0; sr < kLdsRows; ++sr) {
-#pragma unroll
-                for (int t = 0; t < kWaveTiles; ++t) {
-                    const bf16x8 a = __builtin_bit_cast(
-                        bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + sr * (kLdsBX * 64))));
-#pragma unroll
-                    for (int dy = 0; dy < 3; ++dy) {
-                        const int orow = sr - dy;
-                        if (orow < 0 || orow > 3) continue;  // resolved at compile time
-                        acc[orow][t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[dy][0], acc[orow][t][0], 0, 0, 0);
-                        acc[orow][t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[dy][1], acc[orow][t][1], 0, 0, 0);
-                    }
-                }
-            }
-        }
-    }
-
+__device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
+                                         const void* __restrict__ aux, void* __restrict__ y, long long n, int D,
+                                         int H, int W, int zo, int y0, int xw, int m, int q) {
 #pragma unroll
     for (int orow = 0; orow < 4; ++orow) {
         const int yo = y0 + orow;
         if (yo >= H) break;
-        const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
+        const long long obase = ((n * D + zo) * H + yo) * (long long)W;
 #pragma unroll
-        for (int t = 0; t < kWaveTiles; ++t) {
+        for (int t = 0; t < kZmTiles; ++t) {
 #pragma unroll
             for (int hb = 0; hb < 2; ++hb) {
                 const int o = hb * 16 + m;
@@ -240,19 +186,161 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                     const long long vox = obase + xo;
                     float v = acc[orow][t][hb][r];
                     if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
-                        v += bias[o];
+                        v += bias[hb];
                         v = v > 0.0f ? v : 0.0f;
-                        static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
-                    } else if (kEpi == TIGRE_CNN_EPI_RELU_MASK) {
+                    } else {
                         const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
                         v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
-                        static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
-                    } else {  // residual output: channel 0 only, FP32
-                        if (o == 0) static_cast<float*>(y)[vox] = static_cast<const float*>(aux)[vox] + (v + bias[0]);
                     }
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
                 }
             }
         }
+    }
+}
+
+template <int kEpi>
+__global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv_zm_kernel(
+    const unsigned short* __restrict__ x, const unsigned short* __restrict__ w, const float* __restrict__ bias,
+    const void* __restrict__ aux, void* __restrict__ y, int N, int D, int H, int W) {
+    // two staged-plane buffers as separate arrays, so the compiler can tell the copies into one from
+    // the reads of the other and does not wait for the copies before the reads
+    __shared__ uint4 plane_a[kLdsChunks], plane_b[kLdsChunks];  // [row][voxel][channel group, swizzled]
+    __shared__ uint4 wts[27 * kC * 4];       // [tap][output channel][input channel group, swizzled]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, q = lane >> 4;
+    const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
+    const int nchunk = (D + kZmChunk - 1) / kZmChunk;
+    const int zc = (int)(blockIdx.z % (unsigned)nchunk);
+    const long long n = blockIdx.z / (unsigned)nchunk;
+    const int z_lo = zc * kZmChunk, z_hi = min(z_lo + kZmChunk, D);  // output planes z_lo .. z_hi - 1
+    const int xw = x0 + 16 * kZmTiles * wave;
+
+    // weights: row r = 27 taps x 32 output channels, 4 groups of 8 input channels, group g of row r
+    // stored at g ^ ((r >> 2) & 3)
+    for (int i = tid; i < 27 * kC * 4; i += kZmThreads) {
+        const int r = i >> 2, g = i & 3;
+        wts[r * 4 + (g ^ ((r >> 2) & 3))] = *reinterpret_cast<const uint4*>(w + (long long)r * kC + 8 * g);
+    }
+    unsigned aoff[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int xi = 16 * kZmTiles * wave + m + d;
+        aoff[d] = (unsigned)((xi * 4 + (q ^ ((xi >> 2) & 3))) * 16);
+    }
+    // B fragment of (tap, hb): row tap 32 + 16 hb + m, group q; (r >> 2) & 3 = (m >> 2) & 3
+    const unsigned boff = (unsigned)(m * 64 + ((q ^ ((m >> 2) & 3)) * 16));
+    // this lane's output channels m and 16 + m
+    float bias_lane[2] = {0.0f, 0.0f};
+    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+        bias_lane[0] = bias[m];
+        bias_lane[1] = bias[16 + m];
+    }
+    const char* wl = reinterpret_cast<const char*>(wts);
+
+    auto stage = [&](int zi, auto buf) {  // issue the copies of input plane zi into buffer buf
+        const long long zrow = (n * D + zi) * (long long)H;
+        uint4* dst = decltype(buf)::value ? plane_b : plane_a;
+        for (int i0 = 0; i0 < kLdsChunks; i0 += kZmThreads) {
+            const int i = i0 + tid;
+            if (i < kLdsChunks) {
+                const int yi = i / (4 * kLdsBX), r = i - yi * 4 * kLdsBX, xi = r >> 2, sg = r & 3;
+                const int gy = y0 - 1 + yi, gx = x0 - 1 + xi;
+                const void* src = g_zero_chunk;
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    src = x + ((zrow + gy) * W + gx) * kC + 8 * (sg ^ ((xi >> 2) & 3));
+                __builtin_amdgcn_global_load_lds(src, &dst[i0 + wave * 64], 16, 0, 0);
+            }
+        }
+    };
+
+    f32x4 acc[3][4][kZmTiles][2];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int t = 0; t < kZmTiles; ++t) acc[s][r][t][0] = acc[s][r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    if (z_lo - 1 >= 0) stage(z_lo - 1, std::integral_constant<int, 0>{});
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    // one input plane, staged in buffer kB: the accumulator slots of output planes zi + 1, zi, zi - 1
+    // are kN, kC0, kP
+    auto plane_step = [&](int zi, auto sn, auto sc, auto sp, auto sb) {
+        constexpr int kN = decltype(sn)::value, kC0 = decltype(sc)::value, kP = decltype(sp)::value;
+        constexpr int kB = decltype(sb)::value;
+        if (zi + 1 <= z_hi && zi + 1 < D) stage(zi + 1, std::integral_constant<int, 1 - kB>{});
+        if (zi >= 0 && zi < D) {  // uniform per workgroup
+            const char* lds = reinterpret_cast<const char*>(kB ? plane_b : plane_a);
+            // output planes zi - dz, dz = -1, 0, 1, inside the chunk (uniform)
+            const bool on[3] = {zi + 1 < z_hi, zi >= z_lo && zi < z_hi, zi - 1 >= z_lo};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                // the A fragments of this dx for every staged row (and tile), then per dz the weight
+                // fragments of its three dy taps
+                bf16x8 a[kLdsRows][kZmTiles];
+#pragma unroll
+                for (int sr = 0; sr < kLdsRows; ++sr)
+#pragma unroll
+                    for (int t = 0; t < kZmTiles; ++t)
+                        a[sr][t] = __builtin_bit_cast(
+                            bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + sr * (kLdsBX * 64))));
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    if (!on[dz]) continue;
+                    bf16x8 b[3][2];  // [dy + 1][hb]
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                        for (int hb = 0; hb < 2; ++hb)
+                            b[dy][hb] = __builtin_bit_cast(
+                                bf16x8, *reinterpret_cast<const uint4*>(
+                                            wl + boff + (((dz * 9 + dy * 3 + d) * kC + 16 * hb) * 64)));
+                    f32x4(&A)[4][kZmTiles][2] = acc[dz == 0 ? kN : (dz == 1 ? kC0 : kP)];
+#pragma unroll
+                    for (int sr = 0; sr < kLdsRows; ++sr)
+#pragma unroll
+                        for (int t = 0; t < kZmTiles; ++t)
+#pragma unroll
+                            for (int dy = 0; dy < 3; ++dy) {
+                                const int orow = sr - dy;
+                                if (orow < 0 || orow > 3) continue;  // resolved at compile time
+                                A[orow][t][0] =
+                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sr][t], b[dy][0], A[orow][t][0], 0, 0, 0);
+                                A[orow][t][1] =
+                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sr][t], b[dy][1], A[orow][t][1], 0, 0, 0);
+                            }
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();  // the next plane is in LDS; this plane's buffer may be overwritten
+        const int zo = zi - 1;  // complete: it has received its dz = -1, 0 and +1 taps
+        if (zo >= z_lo && zo < z_hi) zm_store<kEpi>(acc[kP], bias_lane, aux, y, n, D, H, W, zo, y0, xw, m, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int t = 0; t < kZmTiles; ++t) acc[kP][r][t][0] = acc[kP][r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    };
+    // slots rotate (n, c, p) -> (p, n, c) (the freed slot takes output plane zi + 2) and the buffers
+    // alternate: both patterns repeat every six planes
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    for (int zi = z_lo - 1; zi <= z_hi; zi += 6) {
+        plane_step(zi, I0{}, I1{}, I2{}, I0{});
+        if (zi + 1 > z_hi) break;
+        plane_step(zi + 1, I2{}, I0{}, I1{}, I1{});
+        if (zi + 2 > z_hi) break;
+        plane_step(zi + 2, I1{}, I2{}, I0{}, I0{});
+        if (zi + 3 > z_hi) break;
+        plane_step(zi + 3, I0{}, I1{}, I2{}, I1{});
+        if (zi + 4 > z_hi) break;
+        plane_step(zi + 4, I2{}, I0{}, I1{}, I0{});
+        if (zi + 5 > z_hi) break;
+        plane_step(zi + 5, I1{}, I2{}, I0{}, I1{});
     }
 }
 
@@ -260,7 +348,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // aux) with the taps as the GEMM columns.  P[u][tap] = sum_i x[u][i] w[tap][0][i]
 // for every input voxel u is one MFMA pair per 16 voxels (columns = taps
 // 0..31, 27 used), and y[v] = aux[v] + (sum_tap P[v + tap][tap] + bias[0]).
-// conv_lds_kernel spends 27 MFMAs per 16 outputs on this layer with 15 of its
+// The 32 -> 32 kernel would spend 27 MFMAs per 16 outputs on this layer with 15 of its
 // 16 columns idle; here it is 2 per 16 input voxels.  A workgroup owns a
 // 64 x 4 tile of outputs and a chunk of kOutZ planes, and marches through the
 // input planes z0 - 1 .. z0 + kOutZ: for each it computes P over the 66 x 6
@@ -574,12 +662,15 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
         const unsigned short* xa = static_cast<const unsigned short*>(x);
         const dim3 lgrid((unsigned)((W + 16 * kLdsTiles - 1) / (16 * kLdsTiles)), (unsigned)((H + 3) / 4),
                          (unsigned)(N * D));
-#define TIGRE_CONV_LDS(E) \
-    hipLaunchKernelGGL((conv_lds_kernel<E>), lgrid, dim3(256), 0, stream, xa, w, bias, aux, y, N, D, H, W)
+        const long long zmz = (long long)N * ((D + kZmChunk - 1) / kZmChunk);
+        TIGRE_REQUIRE(zmz <= 65535, "too many planes for one launch");
+        const dim3 zgrid(lgrid.x, lgrid.y, (unsigned)zmz);
         if (epi == TIGRE_CNN_EPI_BIAS_RELU) {
-            TIGRE_CONV_LDS(TIGRE_CNN_EPI_BIAS_RELU);
+            hipLaunchKernelGGL((conv_zm_kernel<TIGRE_CNN_EPI_BIAS_RELU>), zgrid, dim3(kZmThreads), 0, stream, xa, w, bias,
+                               aux, y, N, D, H, W);
         } else if (epi == TIGRE_CNN_EPI_RELU_MASK) {
-            TIGRE_CONV_LDS(TIGRE_CNN_EPI_RELU_MASK);
+            hipLaunchKernelGGL((conv_zm_kernel<TIGRE_CNN_EPI_RELU_MASK>), zgrid, dim3(kZmThreads), 0, stream, xa, w, bias,
+                               aux, y, N, D, H, W);
         } else {
             const long long zgrid = (long long)N * ((D + kOutZ - 1) / kOutZ);
             TIGRE_REQUIRE(zgrid <= 65535 && (H + kOutTY - 1) / kOutTY <= 65535, "volume too large for one launch");
@@ -588,7 +679,6 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
             hipLaunchKernelGGL(conv_out_kernel, ogrid, dim3(256), 0, stream, xa, w, bias,
                                static_cast<const float*>(aux), static_cast<float*>(y), N, D, H, W);
         }
-#undef TIGRE_CONV_LDS
     } else {
         TIGRE_REQUIRE((long long)N * D <= 65535, "too many planes for one launch");
         const float* xf = static_cast<const float*>(x);
