@@ -40,10 +40,14 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kC = 32;            // channels between layers
 constexpr int kTilesPerWave = 4;  // 16-voxel tiles per wavefront (64 voxels along x)
 
+// Round to nearest even; infinities keep their bits and NaNs stay NaN (quiet
+// bit set when the dropped half of the significand was not zero).  Both
+// results are computed and one selected, so a store loop has no branches.
 __device__ __forceinline__ unsigned short bf16_bits(float f) {
     const unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    const unsigned special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+    const unsigned rounded = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    return (unsigned short)((u & 0x7f800000u) == 0x7f800000u ? special : rounded);
 }
 
 // A fragment of 8 channels (8 q .. 8 q + 7) of voxel `vox`, or zeros.
@@ -163,12 +167,13 @@ __device__ uint4 g_zero_chunk[4];  // zero-initialised, never written: the sourc
 // 1.39 ms per 256^3 layer against 0.93 ms here.)
 constexpr int kZmChunk = 16;  // output planes per workgroup
 constexpr int kZmTiles = 1;   // 16-voxel tiles per wavefront (of each of the 4 rows)
+constexpr int kZmWavesPerSimd = kZmTiles == 1 ? 2 : 1;
 constexpr int kZmThreads = 64 * kLdsTiles / kZmTiles;
 
-template <int kEpi>
-__device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
-                                         const void* __restrict__ aux, void* __restrict__ y, long long n, int D,
-                                         int H, int W, int zo, int y0, int xw, int m, int q) {
+template <int kEpi, bool full>
+__device__ __forceinline__ void zm_store_rows(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
+                                              const void* __restrict__ aux, void* __restrict__ y, long long n,
+                                              int D, int H, int W, int zo, int y0, int xw, int m, int q) {
 #pragma unroll
     for (int orow = 0; orow < 4; ++orow) {
         const int yo = y0 + orow;
@@ -182,7 +187,7 @@ __device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], con
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int xo = xw + 16 * t + 4 * q + r;
-                    if (xo >= W) continue;
+                    if (!full && xo >= W) continue;
                     const long long vox = obase + xo;
                     float v = acc[orow][t][hb][r];
                     if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
@@ -200,7 +205,19 @@ __device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], con
 }
 
 template <int kEpi>
-__global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv_zm_kernel(
+__device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
+                                         const void* __restrict__ aux, void* __restrict__ y, long long n, int D,
+                                         int H, int W, int zo, int y0, int xw, int m, int q) {
+    // a wavefront whose tile lies inside the volume along x stores without per-voxel tests
+    if (xw + 16 * kZmTiles <= W)
+        zm_store_rows<kEpi, true>(acc, bias, aux, y, n, D, H, W, zo, y0, xw, m, q);
+    else
+        zm_store_rows<kEpi, false>(acc, bias, aux, y, n, D, H, W, zo, y0, xw, m, q);
+}
+
+
+template <int kEpi>
+__global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(kZmWavesPerSimd, kZmWavesPerSimd))) void conv_zm_kernel(
     const unsigned short* __restrict__ x, const unsigned short* __restrict__ w, const float* __restrict__ bias,
     const void* __restrict__ aux, void* __restrict__ y, int N, int D, int H, int W) {
     // two staged-plane buffers as separate arrays, so the compiler can tell the copies into one from
@@ -470,7 +487,7 @@ __global__ __launch_bounds__(256) void conv_taps_kernel(const float* __restrict_
                                                         const float* __restrict__ bias, const void* __restrict__ aux,
                                                         void* __restrict__ y, int N, int D, int H, int W) {
     constexpr int kPlane = kLdsRows * kLdsBX;
-    __shared__ float blk[3 * kPlane];  // [dz][row][voxel]
+    __shared__ unsigned short blk[3 * kPlane];  // [dz][row][voxel], rounded to bf16 once when staged
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, q = lane >> 4;
     const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
@@ -484,7 +501,13 @@ __global__ __launch_bounds__(256) void conv_taps_kernel(const float* __restrict_
         float v = 0.0f;
         if (gz >= 0 && gz < D && gy >= 0 && gy < H && gx >= 0 && gx < W)
             v = x[((n * D + gz) * H + gy) * (long long)W + gx];
-        blk[i] = v;
+        blk[i] = bf16_bits(v);
+    }
+    // this lane's output channels m and 16 + m
+    float bias_lane[2] = {0.0f, 0.0f};
+    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+        bias_lane[0] = bias[m];
+        bias_lane[1] = bias[16 + m];
     }
     // B fragments: k = 8 q + j is the tap, column o = 16 hb + m
     bf16x8 b[2];
@@ -510,43 +533,49 @@ __global__ __launch_bounds__(256) void conv_taps_kernel(const float* __restrict_
     }
     __syncthreads();
 
-    f32x4 acc[kLdsTiles][2];
-#pragma unroll
-    for (int t = 0; t < kLdsTiles; ++t) {
-        unsigned short bits[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) bits[j] = toff[j] >= 0 ? bf16_bits(blk[toff[j] + 16 * t]) : (unsigned short)0;
-        const uint4 raw = make_uint4(bits[0] | ((unsigned)bits[1] << 16), bits[2] | ((unsigned)bits[3] << 16),
-                                     bits[4] | ((unsigned)bits[5] << 16), bits[6] | ((unsigned)bits[7] << 16));
-        const bf16x8 a = __builtin_bit_cast(bf16x8, raw);
-        acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-        acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[1], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-    }
-
     if (yo >= H) return;
     const long long obase = ((n * D + zz0) * H + yo) * (long long)W;
+    // one tile at a time (its MFMA pair, then its stores), so few registers are live and several
+    // workgroups share a CU; a workgroup whose tiles lie inside the volume along x stores without
+    // per-voxel tests
+    auto run = [&](auto full_tag) {
+        constexpr bool kFull = decltype(full_tag)::value;
 #pragma unroll
-    for (int t = 0; t < kLdsTiles; ++t) {
+        for (int t = 0; t < kLdsTiles; ++t) {
+            unsigned short bits[8];
 #pragma unroll
-        for (int hb = 0; hb < 2; ++hb) {
-            const int o = hb * 16 + m;
+            for (int j = 0; j < 8; ++j) bits[j] = toff[j] >= 0 ? blk[toff[j] + 16 * t] : (unsigned short)0;
+            const uint4 raw = make_uint4(bits[0] | ((unsigned)bits[1] << 16), bits[2] | ((unsigned)bits[3] << 16),
+                                         bits[4] | ((unsigned)bits[5] << 16), bits[6] | ((unsigned)bits[7] << 16));
+            const bf16x8 a = __builtin_bit_cast(bf16x8, raw);
+            f32x4 acc[2];
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[0], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[1], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int xo = x0 + 16 * t + 4 * q + r;
-                if (xo >= W) continue;
-                const long long vox = obase + xo;
-                float v = acc[t][hb][r];
-                if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
-                    v += bias[o];
-                    v = v > 0.0f ? v : 0.0f;
-                } else {
-                    const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
-                    v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+            for (int hb = 0; hb < 2; ++hb) {
+                const int o = hb * 16 + m;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int xo = x0 + 16 * t + 4 * q + r;
+                    if (!kFull && xo >= W) continue;
+                    const long long vox = obase + xo;
+                    float v = acc[hb][r];
+                    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+                        v += bias_lane[hb];
+                        v = v > 0.0f ? v : 0.0f;
+                    } else {
+                        const unsigned short am = static_cast<const unsigned short*>(aux)[vox * kC + o];
+                        v = (am != 0 && (am & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
+                    }
+                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
                 }
-                static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
             }
         }
-    }
+    };
+    if (x0 + 16 * kLdsTiles <= W)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 // Weight gradient.  Block: one tap (blockIdx.x, 27 = bias) and one chunk of
