@@ -7,9 +7,9 @@
 // row / column l % 16 and k = 8 (l / 16) .. + 7, and result register r of lane
 // l is row 4 (l / 16) + r, column l % 16):
 //
-//  * conv_zm_kernel (32 -> 32 channels): rows 16 voxels along x, columns the
-//    32 output channels, k one tap's 32 input channels; input planes staged
-//    in LDS once each while the workgroup marches along z.
+//  * conv_zm_kernel (32 -> 32 channels): rows 16 output channels, columns 16
+//    voxels along x, k one tap's 32 input channels; the weights in registers,
+//    input planes staged in LDS once each while the workgroup marches along z.
 //  * conv_taps_kernel (one FP32 channel -> 32): k is the tap instead, so one
 //    MFMA covers all 27 taps of 16 voxels.
 //  * conv_out_kernel (32 -> one FP32 channel added to a residual input): the
@@ -129,155 +129,144 @@ __global__ __launch_bounds__(256) void conv_kernel(const void* __restrict__ x, c
     }
 }
 
-// Staged input planes of the 32-channel convolutions.  A workgroup computes
-// 4 output rows of 128 voxels; the 6 x 130 voxel block of an input plane its
-// taps read is copied into LDS with direct global-to-LDS loads (voxels outside
-// the volume are copied from a zero chunk).  The 16-byte channel groups of a
-// voxel are stored in the order q ^ ((x >> 2) & 3): the 16 lanes of one group
-// read 16 consecutive voxels and then fall on distinct LDS banks.  A fragment
-// address is a per-lane constant for each dx, plus an immediate offset for the
-// tile and the row.
-constexpr int kLdsTiles = 8;                    // 16-voxel tiles per workgroup row
+// Staged input blocks.  A workgroup computes 4 output rows; the 6-row block
+// of an input plane its taps read (with one halo voxel on either side along x)
+// is copied into LDS, voxels outside the volume as zeros.
+constexpr int kLdsTiles = 8;                    // 16-voxel tiles per workgroup row (single channel)
 constexpr int kLdsBX = 16 * kLdsTiles + 2;      // staged voxels along x (with the halo)
 constexpr int kLdsRows = 4 + 2;                 // staged rows: the workgroup's 4 and the halo
-constexpr int kLdsChunks = kLdsRows * kLdsBX * 4;  // 16-byte chunks of one staged plane
 
 __device__ uint4 g_zero_chunk[4];  // zero-initialised, never written: the source of the zero halo
 
 // The 32 -> 32 channel convolution (forward with bias and ReLU; data gradient
 // with the ReLU mask), implicit GEMM on v_mfma_f32_16x16x32_bf16: rows = 16
-// voxels along x, columns = 32 output channels (two MFMA blocks), k = one
-// tap's 32 input channels.  A workgroup owns a 4 x 128 voxel tile for a chunk
-// of kZmChunk output planes and marches through its input planes z0 - 1 ..
-// z0 + kZmChunk once: each input plane is staged once, into one of two LDS
-// buffers while the previous plane is computed, and feeds the accumulators of
-// the three output planes it contributes to (z + 1 through its dz = -1 taps,
-// z through dz = 0, z - 1 through dz = +1); the output plane that has
-// received all three is written.  The staged row sr feeds output row sr - dy
-// for each dy, so one A fragment read serves up to 3 dz x 3 dy x 2 column
-// blocks of MFMAs.  The 27 taps' weights are staged in LDS once per workgroup
-// (with the same channel-group swizzle as the activations: a B-fragment read
-// is conflict free).  LDS: two planes (97.5 KiB, separate arrays, so the
-// compiler does not wait for the copies into one before reading the other)
-// and the weights (54 KiB): one workgroup per CU of 8 wavefronts, each owning
-// one 16-voxel tile of the 4 rows and the accumulators of three output planes.
-// Every output voxel sums its taps in the order dz = -1, 0, +1, then dx, then
-// dy.  (Staging the three input planes of each output plane in turn, with a
-// full wait between copy and use, kept the MFMA pipes busy 31 % of the time:
-// 1.39 ms per 256^3 layer against 0.93 ms here.)
-constexpr int kZmChunk = 16;  // output planes per workgroup
-constexpr int kZmTiles = 1;   // 16-voxel tiles per wavefront (of each of the 4 rows)
-constexpr int kZmWavesPerSimd = kZmTiles == 1 ? 2 : 1;
-constexpr int kZmThreads = 64 * kLdsTiles / kZmTiles;
+// output channels (one half of the 32, fixed per wavefront), columns = 16
+// voxels along x, k = one tap's 32 input channels.  The A fragments of all 27
+// taps for the wavefront's half (108 VGPRs) are loaded once, so LDS carries
+// only the staged activations: a B fragment is one 16-byte read of 8 channels
+// of one voxel, with the channel groups stored in the order q ^ ((x >> 2) & 3)
+// (conflict free).  Result register r of lane (m, q) is output channel
+// 16 hb + 4 q + r of voxel m: four consecutive channels, stored as one 8-byte
+// write.  A workgroup owns a 4 x (16 kZmTiles) voxel tile for a chunk of
+// kZmChunk output planes and marches through its input planes z0 - 1 ..
+// z0 + kZmChunk once: each input plane is staged once (direct global-to-LDS
+// loads, the zero halo from a zero chunk), into one of two LDS buffers while
+// the previous plane is computed, and feeds the accumulators of the three
+// output planes it contributes to (z + 1 through its dz = -1 taps, z through
+// dz = 0, z - 1 through dz = +1); the output plane that has received all three
+// is written.  The staged row sr feeds output row sr - dy for each dy, so one
+// fragment read serves up to 3 dz x 3 dy MFMAs.  The wavefronts are the
+// kZmTiles x-tiles times the two channel halves; the two halves of a tile
+// read the same staged fragments.  The two buffers are separate arrays, so
+// the compiler can tell the copies into one from the reads of the other and
+// does not wait for the copies before the reads.  Every output voxel sums its
+// taps in the order dz = -1, 0, +1, then dx, then dy.  (With the weights
+// staged in LDS as well and 32 output channels per wavefront, every plane
+// read 72 fragments per 216 MFMAs instead of 18 per 108: 0.88 ms per 256^3
+// layer against 0.77 ms here; staging the three input planes of each output
+// plane in turn, with a full wait between copy and use, 1.39 ms.)
+constexpr int kZmTiles = 2;                        // 16-voxel x-tiles per workgroup
+constexpr int kZmChunk = 32;                       // output planes per workgroup
+constexpr int kZmBX = 16 * kZmTiles + 2;           // staged voxels along x (with the halo)
+constexpr int kZmChunks = kLdsRows * kZmBX * 4;    // 16-byte chunks of one staged plane
+constexpr int kZmThreads = 128 * kZmTiles;
 
+// Output plane `plane` (pointer offsets of 32-bit lane parts from a uniform
+// plane base): row orow of the tile at lane offset loff + orow W kC.
 template <int kEpi, bool full>
-__device__ __forceinline__ void zm_store_rows(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
-                                              const void* __restrict__ aux, void* __restrict__ y, long long n,
-                                              int D, int H, int W, int zo, int y0, int xw, int m, int q) {
+__device__ __forceinline__ void zm_store(const f32x4 (&acc)[4], const float (&bias)[4], const void* __restrict__ aux,
+                                         void* __restrict__ y, long long plane, int H, int W, int y0, int xv,
+                                         int loff) {
+    const unsigned short* ap = static_cast<const unsigned short*>(aux) + plane;
+    unsigned short* yp = static_cast<unsigned short*>(y) + plane;
 #pragma unroll
     for (int orow = 0; orow < 4; ++orow) {
-        const int yo = y0 + orow;
-        if (yo >= H) break;
-        const long long obase = ((n * D + zo) * H + yo) * (long long)W;
+        if (y0 + orow >= H) break;
+        if (!full && xv >= W) continue;
+        const int off = loff + orow * W * kC;
+        float v[4] = {acc[orow][0], acc[orow][1], acc[orow][2], acc[orow][3]};
+        if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
 #pragma unroll
-        for (int t = 0; t < kZmTiles; ++t) {
-#pragma unroll
-            for (int hb = 0; hb < 2; ++hb) {
-                const int o = hb * 16 + m;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int xo = xw + 16 * t + 4 * q + r;
-                    if (!full && xo >= W) continue;
-                    const long long vox = obase + xo;
-                    float v = acc[orow][t][hb][r];
-                    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
-                        v += bias[hb];
-                        v = v > 0.0f ? v : 0.0f;
-                    } else {
-                        const unsigned short a = static_cast<const unsigned short*>(aux)[vox * kC + o];
-                        v = (a != 0 && (a & 0x8000u) == 0) ? v : 0.0f;  // activation > 0
-                    }
-                    static_cast<unsigned short*>(y)[vox * kC + o] = bf16_bits(v);
-                }
+            for (int r = 0; r < 4; ++r) {
+                v[r] += bias[r];
+                v[r] = v[r] > 0.0f ? v[r] : 0.0f;
             }
+        } else {
+            const uint2 a = *reinterpret_cast<const uint2*>(ap + off);
+            const unsigned ah[4] = {a.x & 0xffffu, a.x >> 16, a.y & 0xffffu, a.y >> 16};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[r] = (ah[r] != 0 && (ah[r] & 0x8000u) == 0) ? v[r] : 0.0f;  // activation > 0
         }
+        const uint2 out = make_uint2((unsigned)bf16_bits(v[0]) | ((unsigned)bf16_bits(v[1]) << 16),
+                                     (unsigned)bf16_bits(v[2]) | ((unsigned)bf16_bits(v[3]) << 16));
+        *reinterpret_cast<uint2*>(yp + off) = out;
     }
 }
 
 template <int kEpi>
-__device__ __forceinline__ void zm_store(const f32x4 (&acc)[4][kZmTiles][2], const float (&bias)[2],
-                                         const void* __restrict__ aux, void* __restrict__ y, long long n, int D,
-                                         int H, int W, int zo, int y0, int xw, int m, int q) {
-    // a wavefront whose tile lies inside the volume along x stores without per-voxel tests
-    if (xw + 16 * kZmTiles <= W)
-        zm_store_rows<kEpi, true>(acc, bias, aux, y, n, D, H, W, zo, y0, xw, m, q);
-    else
-        zm_store_rows<kEpi, false>(acc, bias, aux, y, n, D, H, W, zo, y0, xw, m, q);
-}
-
-
-template <int kEpi>
-__global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(kZmWavesPerSimd, kZmWavesPerSimd))) void conv_zm_kernel(
+__global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv_zm_kernel(
     const unsigned short* __restrict__ x, const unsigned short* __restrict__ w, const float* __restrict__ bias,
     const void* __restrict__ aux, void* __restrict__ y, int N, int D, int H, int W) {
-    // two staged-plane buffers as separate arrays, so the compiler can tell the copies into one from
-    // the reads of the other and does not wait for the copies before the reads
-    __shared__ uint4 plane_a[kLdsChunks], plane_b[kLdsChunks];  // [row][voxel][channel group, swizzled]
-    __shared__ uint4 wts[27 * kC * 4];       // [tap][output channel][input channel group, swizzled]
+    __shared__ uint4 plane_a[kZmChunks], plane_b[kZmChunks];  // [row][voxel][channel group, swizzled]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, q = lane >> 4;
-    const int x0 = blockIdx.x * (16 * kLdsTiles), y0 = blockIdx.y * 4;
+    const int tile = wave % kZmTiles, hb = wave / kZmTiles;
+    const int x0 = blockIdx.x * (16 * kZmTiles), y0 = blockIdx.y * 4;
     const int nchunk = (D + kZmChunk - 1) / kZmChunk;
     const int zc = (int)(blockIdx.z % (unsigned)nchunk);
     const long long n = blockIdx.z / (unsigned)nchunk;
     const int z_lo = zc * kZmChunk, z_hi = min(z_lo + kZmChunk, D);  // output planes z_lo .. z_hi - 1
-    const int xw = x0 + 16 * kZmTiles * wave;
+    const int xv = x0 + 16 * tile + m;                                // this lane's output voxel along x
+    const int och = 16 * hb + 4 * q;                                  // its first output channel
 
-    // weights: row r = 27 taps x 32 output channels, 4 groups of 8 input channels, group g of row r
-    // stored at g ^ ((r >> 2) & 3)
-    for (int i = tid; i < 27 * kC * 4; i += kZmThreads) {
-        const int r = i >> 2, g = i & 3;
-        wts[r * 4 + (g ^ ((r >> 2) & 3))] = *reinterpret_cast<const uint4*>(w + (long long)r * kC + 8 * g);
+    // A fragments: row = output channel 16 hb + m, k = input channels 8 q .. 8 q + 7, per tap
+    bf16x8 wf[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+        wf[t] = __builtin_bit_cast(bf16x8,
+                                   *reinterpret_cast<const uint4*>(w + ((long long)(t * kC + 16 * hb + m) * kC + 8 * q)));
+    float bias_lane[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bias_lane[r] = bias[och + r];
     }
-    unsigned aoff[3];
+    unsigned boff[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const int xi = 16 * kZmTiles * wave + m + d;
-        aoff[d] = (unsigned)((xi * 4 + (q ^ ((xi >> 2) & 3))) * 16);
+        const int xi = 16 * tile + m + d;
+        boff[d] = (unsigned)((xi * 4 + (q ^ ((xi >> 2) & 3))) * 16);
     }
-    // B fragment of (tap, hb): row tap 32 + 16 hb + m, group q; (r >> 2) & 3 = (m >> 2) & 3
-    const unsigned boff = (unsigned)(m * 64 + ((q ^ ((m >> 2) & 3)) * 16));
-    // this lane's output channels m and 16 + m
-    float bias_lane[2] = {0.0f, 0.0f};
-    if (kEpi == TIGRE_CNN_EPI_BIAS_RELU) {
-        bias_lane[0] = bias[m];
-        bias_lane[1] = bias[16 + m];
-    }
-    const char* wl = reinterpret_cast<const char*>(wts);
 
+    // the chunks this thread copies of every staged plane: element offsets inside the plane, or -1
+    // for the zero halo (a plane has fewer than 2^31 elements)
+    constexpr int kStageIters = (kZmChunks + kZmThreads - 1) / kZmThreads;
+    int soff[kStageIters];
+#pragma unroll
+    for (int it = 0; it < kStageIters; ++it) {
+        const int i = it * kZmThreads + tid;
+        const int yi = i / (4 * kZmBX), r = i - yi * 4 * kZmBX, xi = r >> 2, sg = r & 3;
+        const int gy = y0 - 1 + yi, gx = x0 - 1 + xi;
+        soff[it] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? (gy * W + gx) * kC + 8 * (sg ^ ((xi >> 2) & 3)) : -1;
+    }
+    const long long plane_elems = (long long)H * W * kC;
     auto stage = [&](int zi, auto buf) {  // issue the copies of input plane zi into buffer buf
-        const long long zrow = (n * D + zi) * (long long)H;
+        const unsigned short* xp = x + (n * D + zi) * plane_elems;
         uint4* dst = decltype(buf)::value ? plane_b : plane_a;
-        for (int i0 = 0; i0 < kLdsChunks; i0 += kZmThreads) {
-            const int i = i0 + tid;
-            if (i < kLdsChunks) {
-                const int yi = i / (4 * kLdsBX), r = i - yi * 4 * kLdsBX, xi = r >> 2, sg = r & 3;
-                const int gy = y0 - 1 + yi, gx = x0 - 1 + xi;
-                const void* src = g_zero_chunk;
-                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                    src = x + ((zrow + gy) * W + gx) * kC + 8 * (sg ^ ((xi >> 2) & 3));
-                __builtin_amdgcn_global_load_lds(src, &dst[i0 + wave * 64], 16, 0, 0);
+#pragma unroll
+        for (int it = 0; it < kStageIters; ++it) {
+            if (it * kZmThreads + tid < kZmChunks) {
+                const void* src = soff[it] >= 0 ? static_cast<const void*>(xp + soff[it]) : g_zero_chunk;
+                __builtin_amdgcn_global_load_lds(src, &dst[it * kZmThreads + wave * 64], 16, 0, 0);
             }
         }
     };
+    const int loff = (y0 * W + xv) * kC + och;  // this lane's output offset in row y0 of a plane
 
-    f32x4 acc[3][4][kZmTiles][2];
+    f32x4 acc[3][4];
 #pragma unroll
     for (int s = 0; s < 3; ++s)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int t = 0; t < kZmTiles; ++t) acc[s][r][t][0] = acc[s][r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < 4; ++r) acc[s][r] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     if (z_lo - 1 >= 0) stage(z_lo - 1, std::integral_constant<int, 0>{});
     __builtin_amdgcn_s_waitcnt(0);
@@ -288,61 +277,49 @@ __global__ __launch_bounds__(kZmThreads) __attribute__((amdgpu_waves_per_eu(kZmW
     auto plane_step = [&](int zi, auto sn, auto sc, auto sp, auto sb) {
         constexpr int kN = decltype(sn)::value, kC0 = decltype(sc)::value, kP = decltype(sp)::value;
         constexpr int kB = decltype(sb)::value;
+        // addresses are formed from the plane index afresh in every step (a scalar base plus the
+        // lane's 32-bit offset), instead of 64-bit pointers per unrolled step held in VGPRs
+        asm volatile("" : "+s"(zi));
         if (zi + 1 <= z_hi && zi + 1 < D) stage(zi + 1, std::integral_constant<int, 1 - kB>{});
         if (zi >= 0 && zi < D) {  // uniform per workgroup
             const char* lds = reinterpret_cast<const char*>(kB ? plane_b : plane_a);
-            // output planes zi - dz, dz = -1, 0, 1, inside the chunk (uniform)
             const bool on[3] = {zi + 1 < z_hi, zi >= z_lo && zi < z_hi, zi - 1 >= z_lo};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                // the A fragments of this dx for every staged row (and tile), then per dz the weight
-                // fragments of its three dy taps
-                bf16x8 a[kLdsRows][kZmTiles];
+                bf16x8 b[kLdsRows];
 #pragma unroll
                 for (int sr = 0; sr < kLdsRows; ++sr)
-#pragma unroll
-                    for (int t = 0; t < kZmTiles; ++t)
-                        a[sr][t] = __builtin_bit_cast(
-                            bf16x8, *reinterpret_cast<const uint4*>(lds + aoff[d] + (t * 1024 + sr * (kLdsBX * 64))));
+                    b[sr] = __builtin_bit_cast(bf16x8,
+                                               *reinterpret_cast<const uint4*>(lds + boff[d] + sr * (kZmBX * 64)));
 #pragma unroll
                 for (int dz = 0; dz < 3; ++dz) {
                     if (!on[dz]) continue;
-                    bf16x8 b[3][2];  // [dy + 1][hb]
-#pragma unroll
-                    for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                        for (int hb = 0; hb < 2; ++hb)
-                            b[dy][hb] = __builtin_bit_cast(
-                                bf16x8, *reinterpret_cast<const uint4*>(
-                                            wl + boff + (((dz * 9 + dy * 3 + d) * kC + 16 * hb) * 64)));
-                    f32x4(&A)[4][kZmTiles][2] = acc[dz == 0 ? kN : (dz == 1 ? kC0 : kP)];
+                    f32x4(&A)[4] = acc[dz == 0 ? kN : (dz == 1 ? kC0 : kP)];
 #pragma unroll
                     for (int sr = 0; sr < kLdsRows; ++sr)
 #pragma unroll
-                        for (int t = 0; t < kZmTiles; ++t)
-#pragma unroll
-                            for (int dy = 0; dy < 3; ++dy) {
-                                const int orow = sr - dy;
-                                if (orow < 0 || orow > 3) continue;  // resolved at compile time
-                                A[orow][t][0] =
-                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sr][t], b[dy][0], A[orow][t][0], 0, 0, 0);
-                                A[orow][t][1] =
-                                    __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[sr][t], b[dy][1], A[orow][t][1], 0, 0, 0);
-                            }
+                        for (int dy = 0; dy < 3; ++dy) {
+                            const int orow = sr - dy;
+                            if (orow < 0 || orow > 3) continue;  // resolved at compile time
+                            A[orow] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[dz * 9 + dy * 3 + d], b[sr], A[orow],
+                                                                             0, 0, 0);
+                        }
                 }
             }
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();  // the next plane is in LDS; this plane's buffer may be overwritten
         const int zo = zi - 1;  // complete: it has received its dz = -1, 0 and +1 taps
-        if (zo >= z_lo && zo < z_hi) zm_store<kEpi>(acc[kP], bias_lane, aux, y, n, D, H, W, zo, y0, xw, m, q);
+        if (zo >= z_lo && zo < z_hi) {
+            const long long plane = (n * D + zo) * plane_elems;
+            if (x0 + 16 * kZmTiles <= W)
+                zm_store<kEpi, true>(acc[kP], bias_lane, aux, y, plane, H, W, y0, xv, loff);
+            else
+                zm_store<kEpi, false>(acc[kP], bias_lane, aux, y, plane, H, W, y0, xv, loff);
+        }
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int t = 0; t < kZmTiles; ++t) acc[kP][r][t][0] = acc[kP][r][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < 4; ++r) acc[kP][r] = f32x4{0.f, 0.f, 0.f, 0.f};
     };
-    // slots rotate (n, c, p) -> (p, n, c) (the freed slot takes output plane zi + 2) and the buffers
-    // alternate: both patterns repeat every six planes
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
     using I2 = std::integral_constant<int, 2>;
@@ -693,7 +670,11 @@ int launch_cnn_conv(const void* x, int x_mode, const unsigned short* w, const fl
                          (unsigned)(N * D));
         const long long zmz = (long long)N * ((D + kZmChunk - 1) / kZmChunk);
         TIGRE_REQUIRE(zmz <= 65535, "too many planes for one launch");
-        const dim3 zgrid(lgrid.x, lgrid.y, (unsigned)zmz);
+        const dim3 zgrid((unsigned)((W + 16 * kZmTiles - 1) / (16 * kZmTiles)), lgrid.y, (unsigned)zmz);
+        TIGRE_REQUIRE((long long)H * W * kC <= 2147483647LL, "plane too large");
+        if (epi != TIGRE_CNN_EPI_RESIDUAL)
+            TIGRE_REQUIRE((reinterpret_cast<uintptr_t>(y) & 7) == 0 && (reinterpret_cast<uintptr_t>(aux) & 7) == 0,
+                          "32-channel outputs and masks must be 8-byte aligned");
         if (epi == TIGRE_CNN_EPI_BIAS_RELU) {
             hipLaunchKernelGGL((conv_zm_kernel<TIGRE_CNN_EPI_BIAS_RELU>), zgrid, dim3(kZmThreads), 0, stream, xa, w, bias,
                                aux, y, N, D, H, W);
