@@ -132,20 +132,16 @@ struct RayWalk {
 // half of the double buffer) and in elements for the staging copy in global
 // memory.  The axis of each crossing is selected without branches, the
 // principal axis P first on ties, then the other horizontal axis A, then z;
-// every step reads one voxel, issued one step before its value is needed.
+// every step reads one voxel, issued one step before its value is needed;
+// the steps are unrolled four to a guard count.
 // Leaving the slab and leaving the volume are one test, tm >= Te with
 // Te = min(T, time of the slab's last principal crossing): that time is the
 // same fma as the crossing's own, and P wins ties, so the test can only
-// pass on the principal crossing itself (or at the volume's exit).
-// min(a, b) of two finite values as one instruction: written with fminf, the
-// compiler quiets a possible signalling NaN in an operand first (an extra
-// instruction per crossing in the walk below, whose times are never NaN).
-__device__ __forceinline__ float min_nocanon(float a, float b) {
-    float r;
-    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
-
+// pass on the principal crossing itself (or at the volume's exit).  Segments
+// end at the crossing times themselves: at a slab's exit the crossing time is
+// Te exactly, and the volume's exit T is itself a crossing, whose computed
+// time differs from T by rounding only (a ray that stops short of it reads
+// the zero border for its next segment), so no segment is clamped to Te.
 template <int kP, bool kLds>
 __device__ inline bool walk(RayWalk& r, const float* __restrict__ base, unsigned slot, int guard) {
     // locals only, named by role (P principal, A other horizontal, z): a
@@ -170,9 +166,8 @@ __device__ inline bool walk(RayWalk& r, const float* __restrict__ base, unsigned
 #define TIGRE_BEAM_STEP(CUR, NEXT)                                       \
     {                                                                    \
         tm = fminf(tp, fminf(ta, tz));                                   \
-        const float tcl = min_nocanon(tm, Te);                           \
-        sum = fmaf(tcl - tc, CUR, sum);                                  \
-        tc = tcl;                                                        \
+        sum = fmaf(tm - tc, CUR, sum);                                   \
+        tc = tm;                                                         \
         const bool bp = tp == tm, ba = ta == tm;                         \
         kp += bp ? 1.0f : 0.0f;                                          \
         ka += (ba && !bp) ? 1.0f : 0.0f;                                 \
@@ -185,6 +180,8 @@ __device__ inline bool walk(RayWalk& r, const float* __restrict__ base, unsigned
         NEXT = read(off);                                                \
     }
     for (; guard > 0; --guard) {
+        TIGRE_BEAM_STEP(va, vb)
+        TIGRE_BEAM_STEP(vb, va)
         TIGRE_BEAM_STEP(va, vb)
         TIGRE_BEAM_STEP(vb, va)
     }
@@ -509,7 +506,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 r.dY = (unsigned)(sy * (kP == 1 ? sg.NZ * sg.NA : 1));
                 r.dZ = (unsigned)(sz * sg.NA);
                 r.kexit = -1;  // never reached
-                walk<kP, false>(r, stage, 0u, (nx + ny + nz + 8) / 2);
+                walk<kP, false>(r, stage, 0u, (nx + ny + nz + 10) / 4);
             }
         } else if (state == 2) {
             // box origin (-1, -1, -4): the bordered staging copy itself
@@ -575,7 +572,7 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
                 }
                 if (state == 2) {
                     // crossings inside one slab: its planes plus the box's a and z planes
-                    if (walk<kP, true>(r, sh.buf, slot, (planes + na + nzb + 5) / 2)) state = 0;
+                    if (walk<kP, true>(r, sh.buf, slot, (planes + na + nzb + 7) / 4)) state = 0;
                 }
             } else if (state == 2) {
                 if (sample_slab<kMode, true>(rs, sh.buf, slot, SP, na, lo, o.y, o.x, lo, hi, dir, (float)nP,
