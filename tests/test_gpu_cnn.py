@@ -106,6 +106,36 @@ def test_conv_relu_mask_data_gradient(cuda):
     assert err <= 2e-3
 
 
+@pytest.mark.parametrize("epi", ["bias_relu", "relu_mask"])
+def test_conv_32ch_plane_chunks(cuda, epi):
+    """32-channel convolution over more planes than one workgroup's chunk (70 = 32 + 32 + 6), with a
+    ragged last x tile (40 = 32 + 8) and a ragged last row block (9 = 4 + 4 + 1)."""
+    import torch
+
+    shape = (1, 70, 9, 40)
+    rng = np.random.default_rng(11)
+    x = _rand_bf16(rng, shape + (32,))
+    w = _rand_bf16(rng, (27, 32, 32), 0.2)
+    y = torch.empty(shape + (32,), dtype=torch.bfloat16, device="cuda")
+    if epi == "bias_relu":
+        b = rng.standard_normal(32).astype(np.float32) * 0.1
+        _conv(_dev(x, torch.bfloat16), _native.CNN_IN_BF16X32, _dev(w, torch.bfloat16), _dev(b), None, y,
+              _native.CNN_EPI_BIAS_RELU, shape)
+        ref = ocnn.bf16_round(np.maximum(ocnn.conv3d(x, w) + b, 0.0))
+    else:
+        act = np.maximum(_rand_bf16(rng, shape + (32,)), 0.0)
+        _conv(_dev(x, torch.bfloat16), _native.CNN_IN_BF16X32, _dev(w, torch.bfloat16), None,
+              _dev(act, torch.bfloat16), y, _native.CNN_EPI_RELU_MASK, shape)
+        ref = ocnn.bf16_round(ocnn.conv3d(x, w) * (act > 0))
+    got = y.float().cpu().numpy()
+    err = _rel(got, ref)
+    print("32-channel %s over 70 planes rel-L2 %.3e" % (epi, err))
+    assert err <= 2e-3
+    # every plane of every chunk is written and matches on its own
+    per_plane = [_rel(got[0, z], ref[0, z]) for z in range(shape[1])]
+    assert max(per_plane) <= 5e-3, int(np.argmax(per_plane))
+
+
 def test_conv_residual_output(cuda):
     import torch
 
