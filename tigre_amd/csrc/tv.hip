@@ -9,7 +9,8 @@
 // (w = 1 for TV, exp(-(d / delta)^2) per component for AwTV, with the fast
 // exponential as the reference's __expf), so each n is
 // computed once per voxel instead of four times.  One iteration is two passes:
-//   pass 1  g at every voxel, sum of g^2 in FP64 per workgroup;
+//   pass 1  g at every voxel, sum of g^2 per workgroup (FP32 per lane over
+//           its 32 voxels at most, FP64 across lanes);
 //   scale   one workgroup sums the partials in a fixed order (the same result
 //           every run) into alpha / sqrt(sum + norm_eps);
 //   pass 2  f' = f - scale * g, recomputing g from f.
@@ -77,21 +78,21 @@ __device__ __forceinline__ void f_halo_pos(int e, int& lx, int& ly) {
 
 // The normalised difference field at one voxel from its value c, the value
 // below it (z - 1) and its neighbours at x - 1 and y - 1 (GD_TV.cu:104-140:
-// a missing neighbour's difference is 0; `flags` bit 0 the column exists,
-// bit 1 a y neighbour, bit 2 an x neighbour).
+// a missing neighbour's difference is 0).  No masks on the differences: the
+// f planes hold, at x = -1 and y = -1, copies of the edge voxels (so those
+// differences are exactly 0), and 0 in the zero slices (plane nz, and plane
+// -1 below the first); `ok` says whether n exists at all (the column lies in
+// the volume and, for PICCS, the plane too).
 template <int kKind>
-__device__ __forceinline__ void n_field(float c, float below, float cy, float cx, int flags, bool pex, bool in,
-                                        bool has_below, const TvParams& tp, float& n0, float& n1, float& n2) {
+__device__ __forceinline__ void n_field(float c, float below, float cy, float cx, bool ok, bool has_below,
+                                        const TvParams& tp, float& n0, float& n1, float& n2) {
     constexpr bool kPadZ = kKind != 2;
     constexpr float eps = kKind == 2 ? 1e-6f : 1e-8f;
     // branch-free: where n does not exist the reciprocal is replaced by 0, so
-    // n is +-0 (every operand is finite: f planes hold 0 outside the volume)
-    const bool ok = pex && (flags & 1);
-    const float cc = in ? c : 0.0f;
-    const float bb = has_below ? below : 0.0f;
-    const float d0 = kPadZ ? cc - bb : (has_below ? cc - bb : 0.0f);
-    const float d1 = ((flags & 2) && in) ? cc - cy : 0.0f;
-    const float d2 = ((flags & 4) && in) ? cc - cx : 0.0f;
+    // n is +-0 (every operand is finite: f planes hold loaded values or 0)
+    const float d0 = kPadZ ? c - below : (has_below ? c - below : 0.0f);
+    const float d1 = c - cy;
+    const float d2 = c - cx;
     float w0 = 1.0f, w1 = 1.0f, w2 = 1.0f;
     if (kKind == 1) {
         const float a0 = d0 * tp.inv_delta, a1 = d1 * tp.inv_delta, a2 = d2 * tp.inv_delta;
@@ -150,27 +151,29 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
 
     // own columns x0 + 2 tx + c, c = 0, 1
     const int gy = y0 + ty;
-    int own_ofs[2], own_flags[2];
+    int own_ofs[2];
     bool col[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int gx = x0 + 2 * tx + c;
         col[c] = gx < nx && gy < ny;
         own_ofs[c] = col[c] ? gy * nx + gx : -1;
-        own_flags[c] = (col[c] ? 1 : 0) | (gy >= 1 ? 2 : 0) | (gx >= 1 ? 4 : 0);
     }
     const int own_f = f_index(2 * tx, ty), own_n = n_index(2 * tx, ty);  // even: 8-byte aligned pairs
     // halo entry of the f plane (lanes tid < kFHalo); the other lanes store
     // their unused halo value to their own first column just before the own
     // pair, so the store needs no branch (a store under a branch would pull
-    // its load into the branch)
-    int hf_idx = own_f, hf_ofs = -1;
+    // its load into the branch).  Positions outside the volume read the
+    // nearest voxel: at x = -1 / y = -1 that makes the edge difference exactly
+    // 0 (GD_TV.cu:104-114); beyond the high edges the value only enters n of
+    // columns outside the volume, which is 0.
+    int hf_idx = own_f, hf_ofs = 0;
     if (tid < kFHalo) {
         int lx, ly;
         f_halo_pos(tid, lx, ly);
         hf_idx = f_index(lx, ly);
-        const int hx = x0 + lx, hy = y0 + ly;
-        hf_ofs = (hx >= 0 && hx < nx && hy >= 0 && hy < ny) ? hy * nx + hx : -1;
+        const int hx = min(max(x0 + lx, 0), nx - 1), hy = min(max(y0 + ly, 0), ny - 1);
+        hf_ofs = hy * nx + hx;
     }
     // halo of the n plane, one entry per lane: row kTY, column 2 tx (last
     // wavefront) and 2 tx + 1 (the one before), and column kTX, row tx (first
@@ -188,12 +191,12 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         hn_ly = tx;
     }
     const bool hn_on = hn_lx >= 0;
-    int hn_ofs = -1, hn_flags = 0, hn_f = 0, hn_n = 0;
+    int hn_ofs = -1, hn_f = 0, hn_n = 0;
+    bool hn_col = false;
     if (hn_on) {
         const int hx = x0 + hn_lx, hy = y0 + hn_ly;
-        const bool e = hx < nx && hy < ny;
-        hn_ofs = e ? hy * nx + hx : -1;
-        hn_flags = (e ? 1 : 0) | (hy >= 1 ? 2 : 0) | (hx >= 1 ? 4 : 0);
+        hn_col = hx < nx && hy < ny;
+        hn_ofs = hn_col ? hy * nx + hx : -1;
         hn_f = f_index(hn_lx, hn_ly);
         hn_n = n_index(hn_lx, hn_ly);
     }
@@ -204,23 +207,37 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     // The image (and prior) of plane p at the lane's two columns and its
     // f-halo entry, fetched kAhead planes before use into a ring of registers.
     // The loads are unconditional, from clamped in-bounds addresses, and raw:
-    // the masks and the PICCS difference are applied when the slot is used, so
-    // nothing has to wait for a load before then.
+    // the plane mask and the PICCS difference are applied when the slot is
+    // used, so nothing has to wait for a load before then.  Loads and stores
+    // go through a buffer descriptor per plane (a uniform base) with a 32-bit
+    // byte offset per lane (the plane is below 2^30 voxels), so no 64-bit
+    // address arithmetic runs per access.
     const float* src[2] = {f, tp.prior};
-    const int ld0 = max(own_ofs[0], 0), ld1 = max(own_ofs[1], 0), hf_ld = max(hf_ofs, 0);
+    const unsigned ob0 = (unsigned)max(own_ofs[0], 0) * 4u, ob1 = (unsigned)max(own_ofs[1], 0) * 4u,
+                   obh = (unsigned)hf_ofs * 4u;
+    const int plane_bytes = (int)(plane * 4);
+    auto rsrc = [&](const float* pl) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(pl), (short)0, plane_bytes, 0x00020000);
+    };
+    auto at = [](__amdgpu_buffer_rsrc_t r, unsigned ob) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, ob, 0, 0));
+    };
     float ring[kAhead][2][3];  // [slot][image, prior][column 0, column 1, halo]
     auto fetch = [&](int p, float (*v)[3]) {
         const long long base = (long long)min(max(p, 0), nz - 1) * plane;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            v[k][0] = src[k][base + ld0];
-            v[k][1] = src[k][base + ld1];
-            v[k][2] = src[k][base + hf_ld];
+            const __amdgpu_buffer_rsrc_t r = rsrc(src[k] + base);
+            v[k][0] = at(r, ob0);
+            v[k][1] = at(r, ob1);
+            v[k][2] = at(r, obh);
         }
     };
-    // field k of entry e of a fetched slot: 0 outside the volume, f - prior for PICCS
-    auto field = [&](float (*v)[3], int k, int e, bool ok) -> float {
-        return ok ? (k == 0 ? v[0][e] : v[0][e] - v[1][e]) : 0.0f;
+    // field k of entry e of a fetched slot: 0 in the zero slices, f - prior
+    // for PICCS (columns outside the volume keep the clamped value: their n
+    // is 0)
+    auto field = [&](float (*v)[3], int k, int e, bool pin) -> float {
+        return pin ? (k == 0 ? v[0][e] : v[0][e] - v[1][e]) : 0.0f;
     };
 
     // f of the previous plane: own columns, n-halo entries
@@ -239,7 +256,9 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
 #pragma unroll
     for (int j = 0; j < kAhead; ++j) fetch(min(z0 + j, z1), ring[j]);
     float nprev[2][3];  // (combined) n of the previous plane at the own columns (z, y, x)
-    double acc = 0.0;
+    // sum of g^2 over the lane's (at most 2 x 16) voxels in FP32, then FP64
+    // across lanes and workgroups
+    float acc = 0.0f;
     // fully unrolled (planes z0 .. z0 + kZC, at most): ring slot jj % kAhead is
     // a fixed register set and the compiler sees every load it must wait for
 #pragma unroll
@@ -253,17 +272,20 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         float fc[NF][2];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            fc[k][0] = field(ring[j], k, 0, pin && own_ofs[0] >= 0);
-            fc[k][1] = field(ring[j], k, 1, pin && own_ofs[1] >= 0);
-            F[b][k][hf_idx] = field(ring[j], k, 2, pin && hf_ofs >= 0);
+            fc[k][0] = field(ring[j], k, 0, pin);
+            fc[k][1] = field(ring[j], k, 1, pin);
+            F[b][k][hf_idx] = field(ring[j], k, 2, pin);
             *reinterpret_cast<float2*>(&F[b][k][own_f]) = make_float2(fc[k][0], fc[k][1]);
         }
         // unconditional (a branch around the loads would make the compiler
         // wait for all of them); past the chunk it re-reads plane z1 from cache
         fetch(min(p + kAhead, z1), ring[j]);
         __syncthreads();
-        const bool pex = p >= 0 && (kPadZ ? p <= nz : p < nz);
-        const bool in = p < nz, has_below = p >= 1;
+        // n exists in every plane up to the zero slice nz (TV, AwTV) or in the
+        // volume's planes (PICCS); the loop never runs past plane nz
+        const bool pex = kPadZ || p < nz;
+        const bool has_below = p >= 1;
+        const bool ok0 = pex && col[0], ok1 = pex && col[1];
         // n of the own columns and of the halo entries, combined over the
         // fields (PICCS: (1 - ratio) n(f) + ratio n(f - prior); g is linear in
         // n, so one n field goes through LDS; the weight is 1 for one field)
@@ -278,10 +300,8 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             const float2 up = *reinterpret_cast<const float2*>(&F[b][k][own_f - kFW]);  // y - 1 of both columns
             const float left = F[b][k][own_f - 1];                                      // x - 1 of column 0
             float t0[3], t1[3];
-            n_field<kKind>(fc[k][0], fprev[k][0], up.x, left, own_flags[0], pex, in, has_below, tp, t0[0], t0[1],
-                           t0[2]);
-            n_field<kKind>(fc[k][1], fprev[k][1], up.y, fc[k][0], own_flags[1], pex, in, has_below, tp, t1[0], t1[1],
-                           t1[2]);
+            n_field<kKind>(fc[k][0], fprev[k][0], up.x, left, ok0, has_below, tp, t0[0], t0[1], t0[2]);
+            n_field<kKind>(fc[k][1], fprev[k][1], up.y, fc[k][0], ok1, has_below, tp, t1[0], t1[1], t1[2]);
             mix(ncur[0], t0);
             mix(ncur[1], t1);
             fprev[k][0] = fc[k][0];
@@ -289,7 +309,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             if (hn_on) {
                 const float hc = F[b][k][hn_f];
                 float m[3];
-                n_field<kKind>(hc, hprev[k], F[b][k][hn_f - kFW], F[b][k][hn_f - 1], hn_flags, pex, in, has_below, tp,
+                n_field<kKind>(hc, hprev[k], F[b][k][hn_f - kFW], F[b][k][hn_f - 1], pex && hn_col, has_below, tp,
                                m[0], m[1], m[2]);
                 mix(hn3, m);
                 hprev[k] = hc;
@@ -312,17 +332,20 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                 const float nyu = Nn[b ^ 1][0][own_n + kNW + c];
                 const float gv = nprev[c][0] + nprev[c][1] + nprev[c][2] - nxr - nyu - ncur[c][0];
                 res[c] = fpp[c] - gv * scale;
-                if (!kStep && col[c]) acc += (double)gv * (double)gv;
+                if (!kStep) acc = fmaf(col[c] ? gv : 0.0f, gv, acc);
             }
             if (kStep) {
                 // the pair as one 8-byte store where both columns exist and the
                 // address allows it (col[1] implies col[0])
                 float* o = out + (long long)(p - 1) * plane;
-                if (col[1] && (reinterpret_cast<unsigned long long>(o + own_ofs[0]) & 7ull) == 0) {
-                    *reinterpret_cast<float2*>(o + own_ofs[0]) = make_float2(res[0], res[1]);
+                const __amdgpu_buffer_rsrc_t r = rsrc(o);
+                if (col[1] && ((reinterpret_cast<unsigned long long>(o) + ob0) & 7ull) == 0) {
+                    __builtin_amdgcn_raw_buffer_store_b64(
+                        __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(res[0], res[1])),
+                        r, ob0, 0, 0);
                 } else {
-                    if (col[0]) o[own_ofs[0]] = res[0];
-                    if (col[1]) o[own_ofs[1]] = res[1];
+                    if (col[0]) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(res[0]), r, ob0, 0, 0);
+                    if (col[1]) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(res[1]), r, ob1, 0, 0);
                 }
             }
         }
@@ -332,9 +355,9 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
             for (int q = 0; q < 3; ++q) nprev[c][q] = ncur[c][q];
     }
     if (!kStep) {
-        acc = block_sum(acc);
+        const double sum = block_sum((double)acc);
         if (tid == 0)
-            partial[L] = acc;
+            partial[L] = sum;
     }
 }
 
@@ -355,7 +378,8 @@ template <int kKind>
 int run_tv_descent(const float* img, float* dst, float alpha, const TvParams& tp, double norm_eps, long nx, long ny,
                    long nz, int maxiter, hipStream_t stream) {
     TIGRE_REQUIRE(nx > 0 && ny > 0 && nz > 0, "empty image");
-    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && nx * ny < (1LL << 31), "image too large");
+    // 32-bit byte offsets within a plane
+    TIGRE_REQUIRE(nx * ny * nz < (1LL << 40) && nx * ny < (1LL << 30), "image too large");
     const size_t n = (size_t)nx * ny * nz;
     if (maxiter <= 0) {
         if (dst != img) TIGRE_HIP_TRY(hipMemcpyAsync(dst, img, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
