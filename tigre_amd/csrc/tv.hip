@@ -216,29 +216,35 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
     const unsigned ob0 = (unsigned)max(own_ofs[0], 0) * 4u, ob1 = (unsigned)max(own_ofs[1], 0) * 4u,
                    obh = (unsigned)hf_ofs * 4u;
     const int plane_bytes = (int)(plane * 4);
-    auto rsrc = [&](const float* pl) {
-        return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(pl), (short)0, plane_bytes, 0x00020000);
+    auto rsrc = [&](const float* pl, int bytes) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(pl), (short)0, bytes, 0x00020000);
     };
     auto at = [](__amdgpu_buffer_rsrc_t r, unsigned ob) {
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, ob, 0, 0));
     };
     float ring[kAhead][2][3];  // [slot][image, prior][column 0, column 1, halo]
+    // The zero slice p = nz (the loop never fetches further) is read through
+    // a descriptor of no records: every load of it returns 0, so no select
+    // runs on the loaded values.
     auto fetch = [&](int p, float (*v)[3]) {
         const long long base = (long long)min(max(p, 0), nz - 1) * plane;
+        const int bytes = plane_bytes;
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
-            const __amdgpu_buffer_rsrc_t r = rsrc(src[k] + base);
+            const __amdgpu_buffer_rsrc_t r = rsrc(src[k] + base, bytes);
             v[k][0] = at(r, ob0);
             v[k][1] = at(r, ob1);
             v[k][2] = at(r, obh);
         }
     };
-    // field k of entry e of a fetched slot: 0 in the zero slices, f - prior
-    // for PICCS (columns outside the volume keep the clamped value: their n
-    // is 0)
+    // field k of entry e of a fetched slot: f, or f - prior for PICCS
+    // (columns outside the volume keep the clamped value: their n is 0)
     auto field = [&](float (*v)[3], int k, int e, bool pin) -> float {
         return pin ? (k == 0 ? v[0][e] : v[0][e] - v[1][e]) : 0.0f;
     };
+    // 8-byte pair stores need col[1] and an 8-aligned address: the lane's
+    // offset parity is fixed, the plane base's is uniform
+    const bool pair_at0 = col[1] && (ob0 & 4u) == 0, pair_at4 = col[1] && (ob0 & 4u) != 0;
 
     // f of the previous plane: own columns, n-halo entries
     float fprev[NF][2], hprev[NF];
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
         if (p > z1) break;
         const int b = p & 1;
         const float fpp[2] = {fprev[0][0], fprev[0][1]};  // f at plane p - 1 (the image, also for PICCS)
-        const bool pin = p >= 0 && p < nz;
+        const bool pin = p < nz;
         float fc[NF][2];
 #pragma unroll
         for (int k = 0; k < NF; ++k) {
@@ -338,8 +344,8 @@ __global__ __launch_bounds__(kTvThreads) void tv_tile_kernel(const float* __rest
                 // the pair as one 8-byte store where both columns exist and the
                 // address allows it (col[1] implies col[0])
                 float* o = out + (long long)(p - 1) * plane;
-                const __amdgpu_buffer_rsrc_t r = rsrc(o);
-                if (col[1] && ((reinterpret_cast<unsigned long long>(o) + ob0) & 7ull) == 0) {
+                const __amdgpu_buffer_rsrc_t r = rsrc(o, plane_bytes);
+                if ((reinterpret_cast<unsigned long long>(o) & 4ull) == 0 ? pair_at0 : pair_at4) {
                     __builtin_amdgcn_raw_buffer_store_b64(
                         __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, make_float2(res[0], res[1])),
                         r, ob0, 0, 0);
