@@ -339,7 +339,7 @@ __device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ base
 template <int kP, int kMode>
 __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGeom& sg, int nx, int ny, int nz,
                                   const FwdView* __restrict__ views, const BeamGroup& grp, int tile_u, int tile_v,
-                                  int nU, int nV, long long su, long long sv, long long sview,
+                                  bool stack, int nU, int nV, long long su, long long sv, long long sview,
                                   float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
                                   int max_planes, SweepShared& sh) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -351,11 +351,18 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     const int tap = kMode == 0 ? 0 : 1;
 
     // ---- the lane's ray (FP64 set-up) ----
+    // Slot s (four wavefronts) traces view first + s of the group in tile
+    // (tile_u, tile_v); stacked (groups of one view): the slots trace the
+    // 2 x 2 tiles (2 tile_u + (s & 1), 2 tile_v + (s >> 1)) of the one view, so
+    // every wavefront has rays.
     const int slot = wave >> 2;
-    const int u = tile_u * kBeamTile + (lane & 15);
-    const int v = tile_v * kBeamTile + (wave & 3) * 4 + (lane >> 4);
-    const bool active = slot < grp.count && u < nU && v < nV;
-    const FwdView& w = views[grp.first + min(slot, grp.count - 1)];
+    auto slot_view = [&](int s) { return stack ? grp.first : grp.first + min(s, grp.count - 1); };
+    auto slot_tu = [&](int s) { return stack ? 2 * tile_u + (s & 1) : tile_u; };
+    auto slot_tv = [&](int s) { return stack ? 2 * tile_v + (s >> 1) : tile_v; };
+    const int u = slot_tu(slot) * kBeamTile + (lane & 15);
+    const int v = slot_tv(slot) * kBeamTile + (wave & 3) * 4 + (lane >> 4);
+    const bool active = (stack || slot < grp.count) && u < nU && v < nV;
+    const FwdView& w = views[slot_view(slot)];
     RayWalk r;
     r.sum = 0.0f;
     RaySamples rs;
@@ -447,10 +454,10 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     float cA, kA, cz, kz;
     {
         const int c = lane & 15;
-        const int cs = min(c >> 2, grp.count - 1);
-        const FwdView& cw = views[grp.first + cs];
-        const int cu = min(tile_u * kBeamTile + ((c & 1) ? kBeamTile - 1 : 0), nU - 1);
-        const int cvv = min(tile_v * kBeamTile + ((c & 2) ? kBeamTile - 1 : 0), nV - 1);
+        const int cs = c >> 2;
+        const FwdView& cw = views[slot_view(cs)];
+        const int cu = min(slot_tu(cs) * kBeamTile + ((c & 1) ? kBeamTile - 1 : 0), nU - 1);
+        const int cvv = min(slot_tv(cs) * kBeamTile + ((c & 2) ? kBeamTile - 1 : 0), nV - 1);
         const double pv = (double)(nV - 1 - cvv);
         double csrc[3];
         ray_source(cw, cu, pv, csrc);
@@ -593,15 +600,17 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     }
 }
 
-// Workgroup L of one projection: tile L % tiles of group L / tiles.
+// Workgroup L of one projection: tile L % tiles of group L / tiles (stacked:
+// tiles of 2 x 2 detector tiles).
 template <int kMode>
 __device__ __forceinline__ void beam_block(const float* __restrict__ stage_x, const float* __restrict__ stage_y,
                                            int nx, int ny, int nz, int na_x, int na_y,
                                            const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups,
                                            int ngroups, int nU, int nV, long long su, long long sv, long long sview,
                                            float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
-                                           int max_planes, int L, SweepShared& sh) {
-    const int tilesU = (nU + kBeamTile - 1) / kBeamTile, tilesV = (nV + kBeamTile - 1) / kBeamTile;
+                                           int max_planes, bool stack, int L, SweepShared& sh) {
+    const int edge = stack ? 2 * kBeamTile : kBeamTile;
+    const int tilesU = (nU + edge - 1) / edge, tilesV = (nV + edge - 1) / edge;
     const int tiles = tilesU * tilesV;
     const int gi = L / tiles, tile = L % tiles;
     if (gi >= ngroups) return;  // uniform per workgroup
@@ -610,12 +619,12 @@ __device__ __forceinline__ void beam_block(const float* __restrict__ stage_x, co
     sg.NZ = nz + 2;
     if (grp.principal == 0) {
         sg.NA = na_x + 8;
-        beam_sweep<0, kMode>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
-                             out, acc, dvx, dvy, dvz, max_planes, sh);
+        beam_sweep<0, kMode>(stage_x, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, stack, nU, nV, su, sv,
+                             sview, out, acc, dvx, dvy, dvz, max_planes, sh);
     } else {
         sg.NA = na_y + 8;
-        beam_sweep<1, kMode>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, nU, nV, su, sv, sview,
-                             out, acc, dvx, dvy, dvz, max_planes, sh);
+        beam_sweep<1, kMode>(stage_y, sg, nx, ny, nz, views, grp, tile % tilesU, tile / tilesU, stack, nU, nV, su, sv,
+                             sview, out, acc, dvx, dvy, dvz, max_planes, sh);
     }
 }
 
@@ -626,10 +635,10 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     const float* __restrict__ stage_x, const float* __restrict__ stage_y, int nx, int ny, int nz, int na_x, int na_y,
     const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups, int ngroups, int nU, int nV,
     long long su, long long sv, long long sview, float* __restrict__ out, double acc, double dvx, double dvy,
-    double dvz, int max_planes) {
+    double dvz, int max_planes, int stack) {
     __shared__ SweepShared sh;
     beam_block<kMode>(stage_x, stage_y, nx, ny, nz, na_x, na_y, views, groups, ngroups, nU, nV, su, sv, sview, out,
-                      acc, dvx, dvy, dvz, max_planes, xcd_logical_block(blockIdx.x, gridDim.x), sh);
+                      acc, dvx, dvy, dvz, max_planes, stack != 0, xcd_logical_block(blockIdx.x, gridDim.x), sh);
 }
 
 // Siddon projections of many independent problems (Python detector layout) in
@@ -643,7 +652,8 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     const BeamJob& J = jobs[j];
     beam_block<0>(J.stage_x, J.stage_y, J.nx, J.ny, J.nz, J.na_x, J.na_y, J.views,
                   reinterpret_cast<const BeamGroup*>(J.groups), J.ngroups, J.nU, J.nV, 1, J.nU,
-                  (long long)J.nU * J.nV, J.out, J.acc, J.dvx, J.dvy, J.dvz, J.max_planes, L - J.begin, sh);
+                  (long long)J.nU * J.nV, J.out, J.acc, J.dvx, J.dvy, J.dvz, J.max_planes, J.stack != 0, L - J.begin,
+                  sh);
 }
 
 // Staging copy for principal axis P: out[p][z][a] over the bordered ranges
@@ -721,8 +731,9 @@ int stage_job_blocks(const HostGeometry& g, int principal, int* gx, int* gy) {
     return *gx * *gy * (g.nVoxel[2] + 2);
 }
 
-long long beam_job_blocks(const HostGeometry& g, int ngroups) {
-    const long long tiles = ((g.nDetec[0] + kBeamTile - 1) / kBeamTile) * ((g.nDetec[1] + kBeamTile - 1) / kBeamTile);
+long long beam_job_blocks(const HostGeometry& g, int ngroups, bool stack) {
+    const int edge = stack ? 2 * kBeamTile : kBeamTile;
+    const long long tiles = ((g.nDetec[0] + edge - 1) / edge) * ((g.nDetec[1] + edge - 1) / edge);
     return tiles * ngroups;
 }
 
@@ -758,8 +769,8 @@ int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int p
 }
 
 int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                   const int* d_groups, int ngroups, int projection_type, int layout, float* proj, int max_planes,
-                   hipStream_t stream) {
+                   const int* d_groups, int ngroups, bool stack, int projection_type, int layout, float* proj,
+                   int max_planes, hipStream_t stream) {
     TIGRE_REQUIRE(max_planes >= 1 && max_planes <= kSlabPlanesMax, "slab planes out of range");
     const long long nU = g.nDetec[0], nV = g.nDetec[1];
     long long su = 1, sv = nU;
@@ -767,8 +778,7 @@ int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometr
         su = nV;
         sv = 1;
     }
-    const long long tiles = ((nU + kBeamTile - 1) / kBeamTile) * ((nV + kBeamTile - 1) / kBeamTile);
-    const long long blocks = tiles * ngroups;
+    const long long blocks = beam_job_blocks(g, ngroups, stack);
     TIGRE_REQUIRE(blocks < (1LL << 31), "too many detector tiles for one launch");
     // LDS offsets are 32-bit; global element offsets of the staging copy too
     TIGRE_REQUIRE(stage_elements(g, 0) < (1LL << 32) && stage_elements(g, 1) < (1LL << 32),
@@ -780,7 +790,7 @@ int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometr
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y, g.nVoxel[0],
                        g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
                        reinterpret_cast<const BeamGroup*>(d_groups), ngroups, g.nDetec[0], g.nDetec[1], su, sv,
-                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2], max_planes);
+                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2], max_planes, stack ? 1 : 0);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

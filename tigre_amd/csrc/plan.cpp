@@ -353,7 +353,8 @@ int tigre_batch_create(const TigreBatchStep* steps, int nsteps, TigreBatch* out)
             j.nV = g.nDetec[1];
             j.max_planes = p->max_planes;
             j.begin = (int)b_beam;
-            b_beam += beam_job_blocks(g, rec.ngroups());
+            j.stack = rec.stack ? 1 : 0;
+            b_beam += beam_job_blocks(g, rec.ngroups(), rec.stack);
             beam.push_back(j);
         }
         if (rec.nrest() > 0) rest_steps.push_back(i);
