@@ -251,22 +251,29 @@ def test_minPICCS(cuda, shape):
 
 
 @pytest.mark.parametrize("ptype", ["Siddon", "interpolated"])
-def test_ax_sparse_views_stacked_tiles(cuda, ptype):
-    """Sparse views (24 over 2*pi) of a fine detector: the beam of a 16-pixel
-    tile (2.1 mm at the axis) moves 2.5 tile widths at the volume's edge
-    (45 mm from the axis) within 6.6 degrees, less than the 15-degree view
-    spacing, so every view forms a group of its own; the 100 x 90 detector has
-    4 x 3 tiles of 32 x 32 pixels, 288 >= 256 workgroups over the 24 views, so
-    each workgroup traces 2 x 2 tiles of one view, the last column and row of
-    tiles partly off the detector."""
+@pytest.mark.parametrize("nviews,pixel", [(24, 0.2), (48, 0.3)])
+def test_ax_sparse_views_stacked_tiles(cuda, ptype, nviews, pixel):
+    """Sparse views of a fine detector: a group of views spans at most the
+    angle over which the beam of a 16-pixel tile moves 2.5 tile widths at the
+    volume's edge (45 mm from the axis): 6.6 degrees for 0.2 mm pixels, less
+    than the 15-degree spacing of 24 views, so every view forms a group of its
+    own and each workgroup traces 2 x 2 tiles of one view (the 100 x 90
+    detector has 4 x 3 tiles of 32 x 32 pixels, 288 >= 256 workgroups); 9.9
+    degrees for 0.3 mm pixels, between one and two spacings of 48 views, so
+    views go in pairs and each workgroup traces two tiles of each (4 x 6 tiles
+    of 32 x 16 pixels, 24 pairs).  The last column and row of tiles lie partly
+    off the detector."""
     import torch
 
-    geo = cone_geometry((64, 64, 64), (64.0, 64.0, 64.0), (100, 90), (0.2, 0.2), 1000.0, 1536.0)
-    angles = np.linspace(0, 2 * np.pi, 24, endpoint=False) + 0.1
+    geo = cone_geometry((64, 64, 64), (64.0, 64.0, 64.0), (100, 90), (pixel, pixel), 1000.0, 1536.0)
+    angles = np.linspace(0, 2 * np.pi, nviews, endpoint=False) + 0.1
     img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 8, seed=33)
-    tile_mm = 16 * 0.2 * 1000.0 / 1536.0
-    assert 2.5 * tile_mm / (0.5 * np.hypot(64.0, 64.0)) < 2 * np.pi / 24
-    assert ((100 + 31) // 32) * ((90 + 31) // 32) * 24 >= 256
+    cap = 2.5 * (16 * pixel * 1000.0 / 1536.0) / (0.5 * np.hypot(64.0, 64.0))
+    spacing = 2 * np.pi / nviews
+    if nviews == 24:
+        assert cap < spacing and ((100 + 31) // 32) * ((90 + 31) // 32) * 24 >= 256
+    else:
+        assert spacing < cap < 2 * spacing and ((100 + 31) // 32) * ((90 + 15) // 16) * 24 >= 256
     d_img = torch.from_numpy(img).to(cuda)
     got = tigre_amd.Ax(d_img, geo, angles, ptype).cpu().numpy()
     ref = oracle.ax(img, prepared(geo, angles), ptype)
@@ -275,7 +282,7 @@ def test_ax_sparse_views_stacked_tiles(cuda, ptype):
     print("Ax %s sparse views rel-L2 %.3e max-rel %.3e" % (ptype, err, worst))
     assert err <= TOL and worst <= 1e-4
     # the same views one call each (one group, one tile per workgroup): same bits
-    one = np.concatenate([tigre_amd.Ax(d_img, geo, angles[i:i + 1], ptype).cpu().numpy() for i in range(24)])
+    one = np.concatenate([tigre_amd.Ax(d_img, geo, angles[i:i + 1], ptype).cpu().numpy() for i in range(nviews)])
     assert np.array_equal(one, got)
 
 

@@ -339,7 +339,7 @@ __device__ inline bool sample_slab(RaySamples& r, const float* __restrict__ base
 template <int kP, int kMode>
 __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGeom& sg, int nx, int ny, int nz,
                                   const FwdView* __restrict__ views, const BeamGroup& grp, int tile_u, int tile_v,
-                                  bool stack, int nU, int nV, long long su, long long sv, long long sview,
+                                  int stack, int nU, int nV, long long su, long long sv, long long sview,
                                   float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
                                   int max_planes, SweepShared& sh) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -352,16 +352,18 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
 
     // ---- the lane's ray (FP64 set-up) ----
     // Slot s (four wavefronts) traces view first + s of the group in tile
-    // (tile_u, tile_v); stacked (groups of one view): the slots trace the
-    // 2 x 2 tiles (2 tile_u + (s & 1), 2 tile_v + (s >> 1)) of the one view, so
-    // every wavefront has rays.
+    // (tile_u, tile_v).  Stacked, the slots trace `stack` tiles of each of the
+    // 4 / stack views a group may hold: two (groups of at most two views), the
+    // tiles (2 tile_u + (s >> 1), tile_v) of view s & 1; four (groups of one
+    // view), the 2 x 2 tiles (2 tile_u + (s & 1), 2 tile_v + (s >> 1)).
     const int slot = wave >> 2;
-    auto slot_view = [&](int s) { return stack ? grp.first : grp.first + min(s, grp.count - 1); };
-    auto slot_tu = [&](int s) { return stack ? 2 * tile_u + (s & 1) : tile_u; };
-    auto slot_tv = [&](int s) { return stack ? 2 * tile_v + (s >> 1) : tile_v; };
+    const int per = 4 / stack;  // views per workgroup
+    auto slot_view = [&](int s) { return grp.first + min(s % per, grp.count - 1); };
+    auto slot_tu = [&](int s) { return stack == 1 ? tile_u : 2 * tile_u + (stack == 2 ? s >> 1 : s & 1); };
+    auto slot_tv = [&](int s) { return stack == 4 ? 2 * tile_v + (s >> 1) : tile_v; };
     const int u = slot_tu(slot) * kBeamTile + (lane & 15);
     const int v = slot_tv(slot) * kBeamTile + (wave & 3) * 4 + (lane >> 4);
-    const bool active = (stack || slot < grp.count) && u < nU && v < nV;
+    const bool active = slot % per < grp.count && u < nU && v < nV;
     const FwdView& w = views[slot_view(slot)];
     RayWalk r;
     r.sum = 0.0f;
@@ -600,17 +602,17 @@ __device__ inline void beam_sweep(const float* __restrict__ stage, const StageGe
     }
 }
 
-// Workgroup L of one projection: tile L % tiles of group L / tiles (stacked:
-// tiles of 2 x 2 detector tiles).
+// Workgroup L of one projection: tile L % tiles of group L / tiles (stacked
+// by two: tiles of 2 x 1 detector tiles; by four: 2 x 2).
 template <int kMode>
 __device__ __forceinline__ void beam_block(const float* __restrict__ stage_x, const float* __restrict__ stage_y,
                                            int nx, int ny, int nz, int na_x, int na_y,
                                            const FwdView* __restrict__ views, const BeamGroup* __restrict__ groups,
                                            int ngroups, int nU, int nV, long long su, long long sv, long long sview,
                                            float* __restrict__ out, double acc, double dvx, double dvy, double dvz,
-                                           int max_planes, bool stack, int L, SweepShared& sh) {
-    const int edge = stack ? 2 * kBeamTile : kBeamTile;
-    const int tilesU = (nU + edge - 1) / edge, tilesV = (nV + edge - 1) / edge;
+                                           int max_planes, int stack, int L, SweepShared& sh) {
+    const int edge_u = stack >= 2 ? 2 * kBeamTile : kBeamTile, edge_v = stack == 4 ? 2 * kBeamTile : kBeamTile;
+    const int tilesU = (nU + edge_u - 1) / edge_u, tilesV = (nV + edge_v - 1) / edge_v;
     const int tiles = tilesU * tilesV;
     const int gi = L / tiles, tile = L % tiles;
     if (gi >= ngroups) return;  // uniform per workgroup
@@ -638,7 +640,7 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     double dvz, int max_planes, int stack) {
     __shared__ SweepShared sh;
     beam_block<kMode>(stage_x, stage_y, nx, ny, nz, na_x, na_y, views, groups, ngroups, nU, nV, su, sv, sview, out,
-                      acc, dvx, dvy, dvz, max_planes, stack != 0, xcd_logical_block(blockIdx.x, gridDim.x), sh);
+                      acc, dvx, dvy, dvz, max_planes, stack, xcd_logical_block(blockIdx.x, gridDim.x), sh);
 }
 
 // Siddon projections of many independent problems (Python detector layout) in
@@ -652,8 +654,7 @@ __global__ __launch_bounds__(kBeamThreads) __attribute__((amdgpu_waves_per_eu(8,
     const BeamJob& J = jobs[j];
     beam_block<0>(J.stage_x, J.stage_y, J.nx, J.ny, J.nz, J.na_x, J.na_y, J.views,
                   reinterpret_cast<const BeamGroup*>(J.groups), J.ngroups, J.nU, J.nV, 1, J.nU,
-                  (long long)J.nU * J.nV, J.out, J.acc, J.dvx, J.dvy, J.dvz, J.max_planes, J.stack != 0, L - J.begin,
-                  sh);
+                  (long long)J.nU * J.nV, J.out, J.acc, J.dvx, J.dvy, J.dvz, J.max_planes, J.stack, L - J.begin, sh);
 }
 
 // Staging copy for principal axis P: out[p][z][a] over the bordered ranges
@@ -731,9 +732,9 @@ int stage_job_blocks(const HostGeometry& g, int principal, int* gx, int* gy) {
     return *gx * *gy * (g.nVoxel[2] + 2);
 }
 
-long long beam_job_blocks(const HostGeometry& g, int ngroups, bool stack) {
-    const int edge = stack ? 2 * kBeamTile : kBeamTile;
-    const long long tiles = ((g.nDetec[0] + edge - 1) / edge) * ((g.nDetec[1] + edge - 1) / edge);
+long long beam_job_blocks(const HostGeometry& g, int ngroups, int stack) {
+    const int edge_u = stack >= 2 ? 2 * kBeamTile : kBeamTile, edge_v = stack == 4 ? 2 * kBeamTile : kBeamTile;
+    const long long tiles = ((g.nDetec[0] + edge_u - 1) / edge_u) * ((g.nDetec[1] + edge_v - 1) / edge_v);
     return tiles * ngroups;
 }
 
@@ -769,7 +770,7 @@ int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int p
 }
 
 int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                   const int* d_groups, int ngroups, bool stack, int projection_type, int layout, float* proj,
+                   const int* d_groups, int ngroups, int stack, int projection_type, int layout, float* proj,
                    int max_planes, hipStream_t stream) {
     TIGRE_REQUIRE(max_planes >= 1 && max_planes <= kSlabPlanesMax, "slab planes out of range");
     const long long nU = g.nDetec[0], nV = g.nDetec[1];
@@ -790,7 +791,7 @@ int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometr
     hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kBeamThreads), 0, stream, stage_x, stage_y, g.nVoxel[0],
                        g.nVoxel[1], g.nVoxel[2], stage_pad(g.nVoxel[1]), stage_pad(g.nVoxel[0]), d_views,
                        reinterpret_cast<const BeamGroup*>(d_groups), ngroups, g.nDetec[0], g.nDetec[1], su, sv,
-                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2], max_planes, stack ? 1 : 0);
+                       nU * nV, proj, g.accuracy, g.dVoxel[0], g.dVoxel[1], g.dVoxel[2], max_planes, stack);
     TIGRE_HIP_TRY(hipGetLastError());
     return TIGRE_OK;
 }

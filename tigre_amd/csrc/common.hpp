@@ -219,10 +219,11 @@ long long stage_elements(const HostGeometry& g, int principal);
 int launch_stage_volume(const void* vol, int dtype, const HostGeometry& g, int principal, float* stage,
                         hipStream_t stream);
 // Beam-staged forward projection (Siddon or interpolated); d_groups: ngroups
-// records {first view, view count (1..4), principal, direction}; stack: every
-// group holds one view, and a workgroup traces 2 x 2 detector tiles of it.
+// records {first view, view count (1..4), principal, direction}; stack: the
+// detector tiles a workgroup traces per view (1; 2 when no group holds more
+// than two views; 4, as 2 x 2, when every group holds one).
 int launch_ax_beam(const float* stage_x, const float* stage_y, const HostGeometry& g, const FwdView* d_views,
-                   const int* d_groups, int ngroups, bool stack, int projection_type, int layout, float* proj,
+                   const int* d_groups, int ngroups, int stack, int projection_type, int layout, float* proj,
                    int max_planes, hipStream_t stream);
 int launch_ax_interp(const float* bricked, const HostGeometry& g, const BrickGrid& b, const FwdView* d_views,
                      int nviews, int layout, float* proj, hipStream_t stream);
@@ -255,7 +256,7 @@ struct ForwardRecords {
     std::vector<int> groups;
     int nbeam = 0;
     bool stage_used[2] = {false, false};
-    bool stack = false;  // every group holds one view: workgroups of 2 x 2 tiles
+    int stack = 1;  // detector tiles per view of a beam workgroup: 1, 2 or 4
     int ngroups() const { return (int)groups.size() / 4; }
     int nrest() const { return (int)ordered.size() - nbeam; }
 };
@@ -339,7 +340,7 @@ int launch_quad_multi(const QuadJob* d_jobs, int njobs, long long blocks, hipStr
 int launch_atb_multi(const AtbJob* d_jobs, int njobs, long long blocks, int zper, hipStream_t stream);
 // Block counts of one job of each kind (host side).
 int stage_job_blocks(const HostGeometry& g, int principal, int* gx, int* gy);
-long long beam_job_blocks(const HostGeometry& g, int ngroups, bool stack);
+long long beam_job_blocks(const HostGeometry& g, int ngroups, int stack);
 int residual_job_blocks(long long n);
 int quad_job_blocks(int nU, int nV, int nviews);
 

@@ -114,16 +114,21 @@ void build_forward_records(const HostGeometry& g, bool interp, int views_per_gro
     }
     rec->nbeam = (int)rec->ordered.size();
     for (int i : rest) rec->ordered.push_back(views[i]);
-    // Groups of one view leave three of a workgroup's four view slots empty:
-    // then every workgroup traces 2 x 2 tiles of its view instead, provided
-    // the launch keeps a workgroup for every CU of the 256-CU MI355X (a single
-    // view of a 256 x 256 detector stacked into 64 workgroups measured 0.116
-    // against 0.058 ms unstacked; of a 512 x 512 one, into 256, 0.221 against
-    // 0.250 ms).
-    const long long stacked_tiles = (long long)((g.nDetec[0] + 31) / 32) * ((g.nDetec[1] + 31) / 32);
-    rec->stack = rec->ngroups() > 0 && stacked_tiles * rec->ngroups() >= 256;
-    for (int k = 0; k < rec->ngroups(); ++k)
-        if (rec->groups[4 * k + 1] != 1) rec->stack = false;
+    // Groups of fewer than four views leave view slots of a workgroup empty:
+    // when no group holds more than two views, every workgroup traces two
+    // tiles of each of its views, and when every group holds one view, 2 x 2
+    // tiles of it, provided the launch keeps a workgroup for every CU of the
+    // 256-CU MI355X (a single view of a 256 x 256 detector stacked into 64
+    // workgroups measured 0.116 against 0.058 ms unstacked; of a 512 x 512
+    // one, into 256, 0.221 against 0.250 ms).
+    int most = 0;
+    for (int k = 0; k < rec->ngroups(); ++k) most = std::max(most, rec->groups[4 * k + 1]);
+    rec->stack = 1;
+    if (most == 1 || most == 2) {
+        const int st = most == 1 ? 4 : 2, edge_v = st == 4 ? 32 : 16;
+        const long long tiles = (long long)((g.nDetec[0] + 31) / 32) * ((g.nDetec[1] + edge_v - 1) / edge_v);
+        if (tiles * rec->ngroups() >= 256) rec->stack = st;
+    }
 }
 
 namespace {

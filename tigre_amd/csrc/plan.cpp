@@ -353,7 +353,7 @@ int tigre_batch_create(const TigreBatchStep* steps, int nsteps, TigreBatch* out)
             j.nV = g.nDetec[1];
             j.max_planes = p->max_planes;
             j.begin = (int)b_beam;
-            j.stack = rec.stack ? 1 : 0;
+            j.stack = rec.stack;
             b_beam += beam_job_blocks(g, rec.ngroups(), rec.stack);
             beam.push_back(j);
         }
