@@ -91,3 +91,32 @@ def test_interpolated_ignores_stale_lds(cuda):
     assert np.isfinite(got).all()
     ref = oracle.ax(img, prepared(geo, angles), "interpolated")
     assert oracle.rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("kind", ["Siddon", "interpolated"])
+def test_parallel_ax_sparse_views_stacked_tiles(cuda, kind):
+    """Sparse parallel views of a fine detector (0.2 mm pixels, 64 mm volume):
+    a group of views spans at most the angle over which a 16-pixel tile's beam
+    moves 2.5 tile widths at the volume's edge, 2.5 * 3.2 / 45.3 rad = 10
+    degrees, less than the 15-degree spacing of 24 views over 2 pi, so each
+    view is a group of its own and every workgroup traces 2 x 2 tiles of one
+    view (the 100 x 90 detector has 4 x 3 tiles of 32 x 32 pixels, 288 >= 256
+    workgroups over the 24 views).  Against the oracle and one call per view."""
+    import torch
+
+    geo = geometry(mode="parallel", nVoxel=np.array([64, 64, 64]))
+    geo.nDetector = np.array([100, 90])
+    geo.dDetector = np.array([0.2, 0.2])
+    geo.sDetector = geo.nDetector * geo.dDetector
+    angles = np.linspace(0, 2 * np.pi, 24, endpoint=False) + 0.1
+    assert 2.5 * 16 * 0.2 / (0.5 * np.hypot(64.0, 64.0)) < 2 * np.pi / 24
+    assert ((100 + 31) // 32) * ((90 + 31) // 32) * 24 >= 256
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 8, seed=41)
+    d_img = torch.from_numpy(img).to(cuda)
+    got = tigre_amd.Ax(d_img, geo, angles, kind).cpu().numpy()
+    ref = oracle.ax(img, prepared(geo, angles), kind)
+    err = oracle.rel_l2(got, ref)
+    print("parallel Ax %s sparse views rel-L2 %.3e" % (kind, err))
+    assert err <= TOL
+    one = np.concatenate([tigre_amd.Ax(d_img, geo, angles[i:i + 1], kind).cpu().numpy() for i in range(24)])
+    assert np.array_equal(one, got)
