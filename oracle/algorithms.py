@@ -9,6 +9,9 @@ Followed files (Python/tigre/):
   algorithms/iterative_recon_alg.py:214-401     W, V, OS-SART update
   algorithms/krylov_subspace_algorithms.py:22-97 CGLS
   algorithms/pocs_algorithms.py:105-169 ASD-POCS
+  utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
+  algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
+  algorithms/ista_algorithms.py:113-209 FISTA, ISTA
 """
 from __future__ import annotations
 
@@ -17,7 +20,7 @@ import copy
 import numpy as np
 from scipy.fft import fft, ifft
 
-from . import atb, ax, pocs_tv
+from . import atb, ax, pocs_tv, tvdenoise
 
 
 def prep(geo, angles):
@@ -258,3 +261,69 @@ def asd_pocs(proj, geo, angles, niter, alpha=0.002, alpha_red=0.95, rmax=0.95, t
             trace.append((n_iter, float(dd), float(dp), float(dg), c, float(used)))
         if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter >= niter:
             return res
+
+
+# -- the ROF denoiser's callers: SART-TV, OS-SART-TV, FISTA, ISTA -------------------
+def im3ddenoise(img, tviter, lmbda):
+    """utilities/im_3d_denoise.py:7-28: shift the image to minimum 0, scale it to
+    maximum 1, ROF-denoise (tv_proximal, ``tviter`` iterations, ``lmbda``), undo
+    the scaling and the shift."""
+    img = np.asarray(img, dtype=np.float64)
+    lo = img.min()
+    x = img - lo
+    hi = x.max()
+    return tvdenoise(x / hi, lmbda, tviter) * hi + lo
+
+
+def sart_tv(proj, geo, angles, niter, blocksize=1, tviter=50, tvlambda=50.0, lmbda=1.0, noneg=True):
+    """art_family_algorithms.py:65-108 (SART_TV: blocksize 1) and :111-153
+    (OSSART_TV: blocksize 20 by default): every iteration one (OS-)SART sweep
+    of iterative_recon_alg.py:326-401, then im3ddenoise(res, tviter, tvlambda)."""
+    geo = copy.deepcopy(geo)
+    geo.check_geo(angles)
+    ang = geo.angles
+    blocks = ordered_blocks(len(ang), blocksize)
+    W = weights_w(geo, ang)
+    V = weights_v(geo, ang, blocks)
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    proj = np.asarray(proj, dtype=np.float64)
+    for _ in range(niter):
+        for j, idx in enumerate(blocks):
+            g = prep(sub_geometry(geo, idx), ang[idx])
+            r = W[idx] * (proj[idx] - ax(res.astype(np.float32), g, "Siddon"))
+            res = res + lmbda / V[j] * atb(r.astype(np.float32), g, "FDK")
+            if noneg:
+                res = np.maximum(res, 0)
+        res = im3ddenoise(res, tviter, tvlambda)
+    return res
+
+
+def fista(proj, geo, angles, niter, hyper=2.0e8, tviter=20, tvlambda=0.1, fista_p=1.0, fista_q=1.0, noneg=True,
+          momentum=True):
+    """ista_algorithms.py:113-182 (FISTA) and, with ``momentum`` False,
+    :185-209 (ISTA).  One block of all views: res += (2 / hyper) Atb_matched(proj
+    - Ax_interpolated(res)) (no W, V), clipped at 0; FISTA then denoises,
+    x = im3ddenoise(res, tviter, 1 / (2 tvlambda / hyper)), and extrapolates,
+    t' = (p + sqrt(q + 4 t^2)) / 2, res = x + (t - 1) / t' (x - x_old), with
+    x_old starting at the initial image; ISTA sets res = im3ddenoise(res, 20,
+    1 / (2 * 0.1 / hyper)) (its lambda is the fixed 0.1, its TV iterations 20)."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    bm = 1.0 / hyper
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    x_rec = res.copy()
+    t = 1.0
+    for _ in range(niter):
+        res = res + bm * 2 * atb((proj - ax(res.astype(np.float32), g, "interpolated")).astype(np.float32), g,
+                                 "matched")
+        if noneg:
+            res = np.maximum(res, 0)
+        if not momentum:
+            res = im3ddenoise(res, 20, 1.0 / (2 * bm * 0.1))
+            continue
+        x_old = x_rec
+        x_rec = im3ddenoise(res, tviter, 1.0 / (2 * bm * tvlambda))
+        t_old = t
+        t = (fista_p + np.sqrt(fista_q + 4 * t ** 2)) / 2
+        res = x_rec + (t_old - 1) / t * (x_rec - x_old)
+    return res

@@ -103,3 +103,52 @@ def test_asd_pocs_variants_run(cuda, variant):
     r = np.linalg.norm(oracle.ax(rec.astype(np.float32), gp) - proj)
     print("%s residual %.3e of %.3e" % (variant, r, r0))
     assert r < 0.5 * r0
+
+
+@pytest.mark.parametrize("blocksize", [1, 20])
+def test_sart_tv(cuda, problem, blocksize):
+    """SART-TV (one view per subset) and OS-SART-TV (20) with the reference's
+    denoising defaults (50 iterations, lambda 50) against the FP64 restatement."""
+    geo, angles, img, proj = problem
+    if blocksize == 1:
+        got = algs.sart_tv(proj, geo, angles, 2)
+    else:
+        got = algs.ossart_tv(proj, geo, angles, 2, blocksize=blocksize)
+    ref = oalg.sart_tv(proj, geo, angles, 2, blocksize=blocksize, tviter=50, tvlambda=50.0)
+    err = oracle.rel_l2(got, ref)
+    print("SART-TV blocksize %d rel-L2 %.3e" % (blocksize, err))
+    assert err <= 1e-4
+
+
+def _largest_eigenvalue(geo, angles):
+    """Largest eigenvalue of Atb_matched Ax_interpolated by power iteration on the device."""
+    import torch
+
+    from tigre_amd import _native
+
+    vs = algs.ViewSets(geo, angles, [np.arange(len(angles))])
+    x = torch.rand(vs.vol_shape, device="cuda", generator=torch.Generator(device="cuda").manual_seed(4))
+    lam = 0.0
+    for _ in range(30):
+        y = vs.atb(vs.ax(x, 0, _native.PROJ_INTERPOLATED), 0, _native.BACKPROJ_MATCHED)
+        lam = float(torch.linalg.vector_norm(y.double()) / torch.linalg.vector_norm(x.double()))
+        x = y / torch.linalg.vector_norm(y)
+    return lam
+
+
+@pytest.mark.parametrize("variant", ["fista", "ista"])
+def test_fista_ista(cuda, problem, variant):
+    """FISTA / ISTA with the step 2 / hyper = 1 / (1.2 L) (L the largest
+    eigenvalue of Atb Ax) and, for FISTA, TV denoising of strength 30 after
+    every data step, against the FP64 restatement."""
+    geo, angles, img, proj = problem
+    hyper = 2.4 * _largest_eigenvalue(geo, angles)
+    if variant == "fista":
+        got = algs.fista(proj, geo, angles, 4, hyper=hyper, tvlambda=hyper / 60.0)
+        ref = oalg.fista(proj, geo, angles, 4, hyper=hyper, tvlambda=hyper / 60.0)
+    else:
+        got = algs.ista(proj, geo, angles, 4, hyper=hyper)
+        ref = oalg.fista(proj, geo, angles, 4, hyper=hyper, momentum=False)
+    err = oracle.rel_l2(got, ref)
+    print("%s rel-L2 %.3e (vs phantom %.3f)" % (variant, err, oracle.rel_l2(got, img)))
+    assert err <= 1e-4

@@ -254,3 +254,76 @@ def test_asd_pocs_beta_stop(small_problem):
     assert len(t_half) == 8 and len(t_one) == 3
     # (the oracle's OpenMP norm sum may round differently with another thread count)
     assert np.allclose(np.array(t_half[:3], dtype=np.float64), np.array(t_one, dtype=np.float64), rtol=1e-12, atol=0)
+
+
+# -- SART-TV, OS-SART-TV, FISTA, ISTA (the ROF denoiser's callers) ----------------
+def test_im3ddenoise_identity_and_equivariance():
+    """im3ddenoise normalises its input to [0, 1] before the ROF denoiser and
+    undoes it after (im_3d_denoise.py:7-28): with no iterations it returns
+    the image, and it commutes with a positive scaling and a shift."""
+    x = np.random.default_rng(3).random((10, 11, 12))
+    assert np.abs(oalg.im3ddenoise(x, 0, 50.0) - x).max() <= 1e-14
+    a, c = 3.7, -0.4
+    y1 = oalg.im3ddenoise(a * x + c, 15, 30.0)
+    y2 = a * oalg.im3ddenoise(x, 15, 30.0) + c
+    assert np.abs(y1 - y2).max() <= 1e-12 * np.abs(y2).max()
+    # and, at the reference's defaults (50 iterations, lambda 15), it denoises:
+    # a noisy phantom loses total variation and comes closer to the clean one
+    img = ellipsoid_phantom((16, 16, 16), (16.0, 16.0, 16.0), 4, seed=5)
+    noisy = img + np.random.default_rng(1).normal(0, 0.05, img.shape)
+    tv = lambda z: sum(np.abs(np.diff(z, axis=k)).sum() for k in range(3))  # noqa: E731
+    den = oalg.im3ddenoise(noisy, 50, 15.0)
+    assert tv(den) < 0.8 * tv(noisy)
+    assert nrmse(den, img) < nrmse(noisy, img)
+
+
+@pytest.mark.parametrize("blocksize", [1, 4])
+def test_sart_tv_without_denoise_is_os_sart(small_problem, blocksize):
+    """With no denoising iterations SART-TV / OS-SART-TV are the SART / OS-SART
+    sweeps (art_family_algorithms.py:86-108, :131-153)."""
+    geo, angles, _, proj = small_problem
+    ref = oalg.os_sart(proj, geo, angles, 3, blocksize=blocksize)
+    got = oalg.sart_tv(proj, geo, angles, 3, blocksize=blocksize, tviter=0)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _dense_operators(geo, angles):
+    """Interpolated Ax and matched Atb of a tiny problem as dense matrices,
+    column by column from unit vectors."""
+    g = prepared(geo, angles)
+    nvox = int(np.prod(geo.nVoxel))
+    shape_v = tuple(int(v) for v in geo.nVoxel)
+    A = np.stack([oracle.ax(np.eye(nvox, 1, -i).reshape(shape_v).astype(np.float32), g, "interpolated").ravel()
+                  for i in range(nvox)], axis=1)
+    shape_p = (len(angles),) + tuple(int(v) for v in geo.nDetector)
+    npix = int(np.prod(shape_p))
+    B = np.stack([oracle.atb(np.eye(npix, 1, -i).reshape(shape_p).astype(np.float32), g, "matched").ravel()
+                  for i in range(npix)], axis=1)
+    return A, B
+
+
+def test_fista_data_step_and_acceleration():
+    """FISTA's data step is res + (2 / hyper) Atb_matched(b - Ax_interpolated(res))
+    (ista_algorithms.py:128-153): with no denoising and no clipping its first
+    iterate equals that step computed with the dense operators; and its
+    extrapolation accelerates the iteration: after 25 iterations its data
+    residual is below that of the plain gradient iteration with the same step."""
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.0, 1.0), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 5, endpoint=False)
+    A, B = _dense_operators(geo, angles)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 2, seed=8)
+    b = A @ img.ravel().astype(np.float64)
+    proj = b.reshape((5, 8, 8))
+    # FISTA's step 2 / hyper must not exceed 1 / L (L the largest eigenvalue of
+    # Atb Ax): hyper = 2.4 L
+    hyper = 2.4 * float(np.max(np.abs(np.linalg.eigvals(B @ A))))
+    first = oalg.fista(proj, geo, angles, 1, hyper=hyper, tviter=0, noneg=False)
+    step = 2.0 / hyper * (B @ b)
+    assert np.abs(first.ravel() - step).max() <= 1e-5 * np.abs(step).max()
+    fast = oalg.fista(proj, geo, angles, 25, hyper=hyper, tviter=0, noneg=False)
+    x = np.zeros(A.shape[1])
+    for _ in range(25):
+        x = x + 2.0 / hyper * (B @ (b - A @ x))
+    r_fista = np.linalg.norm(A @ fast.ravel() - b)
+    r_gd = np.linalg.norm(A @ x - b)
+    assert r_fista < 0.7 * r_gd, (r_fista, r_gd)

@@ -1,8 +1,9 @@
 """Reconstruction algorithms on device-resident data.
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
-``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``cgls``, ``asd_pocs``,
-``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
+``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
+``fista``, ``ista``, ``cgls``, ``asd_pocs``, ``os_asd_pocs``, ``awasd_pocs``,
+``os_awasd_pocs``, ``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -25,8 +26,8 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "cgls", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs",
-           "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
+__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "asd_pocs",
+           "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
 
 
 def _torch():
@@ -352,6 +353,105 @@ def sirt(proj, geo, angles, niter, **kwargs):
     """SIRT: all views in one subset."""
     kwargs.pop("blocksize", None)
     return _sart_family(proj, geo, angles, niter, len(np.atleast_1d(angles)), **kwargs)
+
+
+# ---------------------------------------------------------------------------
+# The ROF denoiser's callers: SART-TV / OS-SART-TV (art_family_algorithms.py
+# :65-153) and FISTA / ISTA (ista_algorithms.py:13-212)
+# ---------------------------------------------------------------------------
+def _im3ddenoise(img, tviter, lmbda):
+    """im3ddenoise (im_3d_denoise.py:7-28) on a device volume: shift to minimum
+    0, scale to maximum 1, ROF-denoise, undo; the extrema stay on the device."""
+    from ..tv import tvdenoise
+
+    torch = _torch()
+    lo = torch.min(img)
+    x = img - lo
+    hi = torch.max(x)
+    out = tvdenoise(x / hi, tviter, lmbda)
+    out *= hi
+    out += lo
+    return out
+
+
+def _sart_tv(proj, geo, angles, niter, blocksize, tvlambda, tviter, **kwargs):
+    st = _SartState(proj, geo, angles, blocksize, order=kwargs.get("OrderStrategy"), init=kwargs.get("init"),
+                    lmbda=kwargs.get("lmbda", 1.0), noneg=kwargs.get("noneg", True),
+                    verbose=kwargs.get("verbose", False))
+    for _ in range(niter):
+        st.sweep()
+        st.res = _im3ddenoise(st.res, tviter, tvlambda)
+    return _result(st.res, kwargs.get("return_tensor", False))
+
+
+def sart_tv(proj, geo, angles, niter, tvlambda=50, tviter=50, **kwargs):
+    """SART with TV regularisation (art_family_algorithms.py:65-108): every
+    iteration one SART sweep (one view per subset; a larger blocksize is
+    ignored with the reference's warning), then im3ddenoise(res, tviter,
+    tvlambda)."""
+    if kwargs.pop("blocksize", 1) > 1:
+        print("Warning: blocksize is set to 1, please use an OS version of the algorithm for blocksize > 1")
+    return _sart_tv(proj, geo, angles, niter, 1, tvlambda, tviter, **kwargs)
+
+
+def ossart_tv(proj, geo, angles, niter, blocksize=20, tvlambda=50, tviter=50, **kwargs):
+    """OS-SART with TV regularisation (art_family_algorithms.py:111-153):
+    every iteration one sweep over subsets of ``blocksize`` views, then
+    im3ddenoise(res, tviter, tvlambda)."""
+    return _sart_tv(proj, geo, angles, niter, blocksize, tvlambda, tviter, **kwargs)
+
+
+def _ista_family(proj, geo, angles, niter, momentum, hyper=2.0e8, tviter=20, tvlambda=0.1, fista_p=1.0,
+                 fista_q=1.0, init=None, noneg=True, verbose=False, return_tensor=False, **kwargs):
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    if init is None:
+        res = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+    elif isinstance(init, str) and init == "FDK":
+        res = fdk(b, g, g.angles, return_tensor=True)
+    else:
+        res = _to_device(init).reshape(vs.vol_shape).clone()
+    step = 2.0 / float(hyper)  # bm * 2, bm = 1 / hyper (ista_algorithms.py:119-122, :128-153)
+    lam = 1.0 / (2.0 / float(hyper) * (float(tvlambda) if momentum else 0.1))
+    x_rec = res.clone()
+    t = 1.0
+    for i in range(niter):
+        if verbose:
+            print("%s iteration %d of %d" % ("FISTA" if momentum else "ISTA", i + 1, niter))
+        r = b - vs.ax(res, 0, _native.PROJ_INTERPOLATED)
+        res = res + step * vs.atb(r, 0, _native.BACKPROJ_MATCHED)
+        if noneg:
+            res.clamp_(min=0)
+        if not momentum:
+            res = _im3ddenoise(res, 20, lam)
+            continue
+        x_old = x_rec
+        x_rec = _im3ddenoise(res, tviter, lam)
+        t_old = t
+        t = (fista_p + float(np.sqrt(fista_q + 4 * t ** 2))) / 2
+        res = x_rec + (t_old - 1) / t * (x_rec - x_old)
+    return _result(res, return_tensor)
+
+
+def fista(proj, geo, angles, niter, **kwargs):
+    """FISTA (ista_algorithms.py:13-182): one block of all views; per
+    iteration res += (2 / hyper) Atb_matched(proj - Ax_interpolated(res)),
+    clipped at 0, then x = im3ddenoise(res, tviter, 1 / (2 tvlambda / hyper))
+    and the extrapolation res = x + (t - 1) / t' (x - x_previous) with
+    t' = (fista_p + sqrt(fista_q + 4 t^2)) / 2.  Keywords: hyper (2e8),
+    tviter (20), tvlambda (0.1), fista_p (1), fista_q (1), init (None / 'FDK'
+    / image), noneg (True), verbose, return_tensor."""
+    return _ista_family(proj, geo, angles, niter, True, **kwargs)
+
+
+def ista(proj, geo, angles, niter, **kwargs):
+    """ISTA (ista_algorithms.py:185-209): FISTA's data step, then
+    res = im3ddenoise(res, 20, 1 / (0.2 / hyper)) (its lambda is the fixed 0.1
+    and its TV iterations the fixed 20), no extrapolation."""
+    return _ista_family(proj, geo, angles, niter, False, **kwargs)
 
 
 # ---------------------------------------------------------------------------
