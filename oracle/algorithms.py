@@ -12,6 +12,8 @@ Followed files (Python/tigre/):
   utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
   algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
   algorithms/ista_algorithms.py:113-209 FISTA, ISTA
+  algorithms/krylov_subspace_algorithms.py:101-188 LSQR
+  algorithms/statistical_algorithms.py:11-55     MLEM
 """
 from __future__ import annotations
 
@@ -326,4 +328,70 @@ def fista(proj, geo, angles, niter, hyper=2.0e8, tviter=20, tvlambda=0.1, fista_
         t_old = t
         t = (fista_p + np.sqrt(fista_q + 4 * t ** 2)) / 2
         res = x_rec + (t_old - 1) / t * (x_rec - x_old)
+    return res
+
+
+# -- LSQR and MLEM -------------------------------------------------------------------
+def lsqr(proj, geo, angles, niter, restart=True):
+    """krylov_subspace_algorithms.py:101-188, Paige and Saunders' LSQR with
+    Siddon Ax and matched Atb over all views.  After every iteration the data
+    residual |proj - Ax(res)| is taken; when it rises the step is undone
+    (res -= (phi / rho) w of the previous step) and the iteration ends (the
+    reference re-initialises and then leaves its loop).  Returns (res,
+    residual norms)."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    A = lambda x: ax(x.astype(np.float32), g, "Siddon")  # noqa: E731
+    At = lambda y: atb(y.astype(np.float32), g, "matched")  # noqa: E731
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    u = proj - A(res)
+    beta = np.linalg.norm(u)
+    u = u / beta
+    phibar = beta
+    v = At(u)
+    alpha = np.linalg.norm(v)
+    v = v / alpha
+    rhobar = alpha
+    w = v.copy()
+    norms = []
+    for i in range(niter):
+        u = A(v) - alpha * u
+        beta = np.linalg.norm(u)
+        u = u / beta
+        v = At(u) - beta * v
+        alpha = np.linalg.norm(v)
+        v = v / alpha
+        rho = np.sqrt(rhobar ** 2 + beta ** 2)
+        c, s = rhobar / rho, beta / rho
+        theta = s * alpha
+        rhobar = -c * alpha
+        phi = c * phibar
+        phibar = s * phibar
+        res = res + (phi / rho) * w
+        w_prev = w
+        w = v - (theta / rho) * w
+        norms.append(float(np.linalg.norm(proj - A(res))))
+        if i > 0 and norms[-1] > norms[-2]:
+            res = res - (phi / rho) * w_prev
+            break
+    return res, norms
+
+
+def mlem(proj, geo, angles, niter, init=None):
+    """statistical_algorithms.py:11-55: res = 1 (or init); W = Atb_matched(1),
+    infinite where <= 0; every iteration res *= Atb_matched(proj /
+    Ax_interpolated(res)) / W (a zero forward projection divides by infinity),
+    negatives clipped to 0 before and after."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    shape = tuple(int(v) for v in geo.nVoxel)
+    res = np.ones(shape) if init is None else np.array(init, dtype=np.float64)
+    W = atb(np.ones(proj.shape, dtype=np.float32), g, "matched")
+    W[W <= 0.0] = np.inf
+    res[res < 0.0] = 0.0
+    for _ in range(niter):
+        den = ax(res.astype(np.float32), g, "interpolated")
+        den[den == 0.0] = np.inf
+        res = res * (atb((proj / den).astype(np.float32), g, "matched") / W)
+        res[res < 0.0] = 0.0
     return res

@@ -152,3 +152,22 @@ def test_fista_ista(cuda, problem, variant):
     err = oracle.rel_l2(got, ref)
     print("%s rel-L2 %.3e (vs phantom %.3f)" % (variant, err, oracle.rel_l2(got, img)))
     assert err <= 1e-4
+
+
+def test_lsqr(cuda, problem):
+    geo, angles, img, proj = problem
+    got, l2 = algs.lsqr(proj, geo, angles, 4, computel2=True)
+    ref, norms = oalg.lsqr(proj, geo, angles, 4)
+    err = oracle.rel_l2(got, ref)
+    print("LSQR rel-L2 %.3e, residuals %s" % (err, l2))
+    assert err <= 1e-4
+    assert np.allclose(l2[0], norms, rtol=1e-4)
+
+
+def test_mlem(cuda, problem):
+    geo, angles, img, proj = problem
+    got = algs.mlem(proj, geo, angles, 3)
+    ref = oalg.mlem(proj, geo, angles, 3)
+    err = oracle.rel_l2(got, ref)
+    print("MLEM rel-L2 %.3e" % err)
+    assert err <= 1e-4

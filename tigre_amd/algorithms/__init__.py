@@ -2,8 +2,8 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``asd_pocs``, ``os_asd_pocs``, ``awasd_pocs``,
-``os_awasd_pocs``, ``piccs``, ``os_piccs``.
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``mlem``, ``asd_pocs``,
+``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -26,8 +26,8 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "asd_pocs",
-           "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
+__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr", "mlem",
+           "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
 
 
 def _torch():
@@ -445,6 +445,88 @@ def fista(proj, geo, angles, niter, **kwargs):
     tviter (20), tvlambda (0.1), fista_p (1), fista_q (1), init (None / 'FDK'
     / image), noneg (True), verbose, return_tensor."""
     return _ista_family(proj, geo, angles, niter, True, **kwargs)
+
+
+def lsqr(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=False, computel2=False, **kwargs):
+    """LSQR (krylov_subspace_algorithms.py:101-188), Paige and Saunders'
+    bidiagonalisation with Siddon Ax and matched Atb over all views, FP64
+    norms.  The data residual is taken after every iteration; when it rises
+    the step is undone and the iteration ends, as in the reference (which
+    re-initialises and then leaves its loop).  With ``computel2`` the
+    residual norms are returned as well."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    A = lambda x: vs.ax(x, 0, _native.PROJ_SIDDON)  # noqa: E731
+    At = lambda y: vs.atb(y, 0, _native.BACKPROJ_MATCHED)  # noqa: E731
+    res = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+    u = b - A(res)
+    beta = _norm(u)
+    u = u / beta
+    phibar = beta
+    v = At(u)
+    alpha = _norm(v)
+    v = v / alpha
+    rhobar = alpha
+    w = v.clone()
+    l2 = []
+    for i in range(niter):
+        if verbose:
+            print("LSQR iteration %d of %d" % (i + 1, niter))
+        u = torch.sub(A(v), u, alpha=alpha)
+        beta = _norm(u)
+        u = u / beta
+        v = torch.sub(At(u), v, alpha=beta)
+        alpha = _norm(v)
+        v = v / alpha
+        rho = float(np.sqrt(rhobar ** 2 + beta ** 2))
+        c, s = rhobar / rho, beta / rho
+        theta = s * alpha
+        rhobar = -c * alpha
+        phi = c * phibar
+        phibar = s * phibar
+        res = torch.add(res, w, alpha=phi / rho)
+        w_prev = w
+        w = torch.sub(v, w, alpha=theta / rho)
+        l2.append(_norm(b - A(res)))
+        if i > 0 and l2[-1] > l2[-2]:
+            res = torch.sub(res, w_prev, alpha=phi / rho)
+            if verbose or not restart:
+                print("LSQR exited due to divergence.")
+            break
+    out = _result(res, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
+def mlem(proj, geo, angles, niter, init=None, verbose=False, return_tensor=False, **kwargs):
+    """MLEM (statistical_algorithms.py:11-55): res = 1 (or ``init``);
+    W = Atb_matched(1), infinite where <= 0; every iteration res *=
+    Atb_matched(proj / Ax_interpolated(res)) / W, where a zero forward
+    projection divides by infinity; negatives clipped to 0 before and after."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    if init is None:
+        res = torch.ones(vs.vol_shape, dtype=torch.float32, device="cuda")
+    elif isinstance(init, str) and init == "FDK":
+        res = fdk(b, g, g.angles, return_tensor=True)
+    else:
+        res = _to_device(init).reshape(vs.vol_shape).clone()
+    W = vs.atb(torch.ones_like(b), 0, _native.BACKPROJ_MATCHED)
+    W = torch.where(W <= 0.0, torch.full_like(W, float("inf")), W)
+    res.clamp_(min=0.0)
+    for i in range(niter):
+        if verbose:
+            print("MLEM iteration %d of %d" % (i + 1, niter))
+        den = vs.ax(res, 0, _native.PROJ_INTERPOLATED)
+        den = torch.where(den == 0.0, torch.full_like(den, float("inf")), den)
+        res = res * (vs.atb(b / den, 0, _native.BACKPROJ_MATCHED) / W)
+        res.clamp_(min=0.0)
+    return _result(res, return_tensor)
 
 
 def ista(proj, geo, angles, niter, **kwargs):
