@@ -12,7 +12,7 @@ Followed files (Python/tigre/):
   utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
   algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
   algorithms/ista_algorithms.py:113-209 FISTA, ISTA
-  algorithms/krylov_subspace_algorithms.py:101-188 LSQR
+  algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :299-439 LSMR
   algorithms/statistical_algorithms.py:11-55     MLEM
 """
 from __future__ import annotations
@@ -395,3 +395,55 @@ def mlem(proj, geo, angles, niter, init=None):
         res = res * (atb((proj / den).astype(np.float32), g, "matched") / W)
         res[res < 0.0] = 0.0
     return res
+
+
+def lsmr(proj, geo, angles, niter, lmbda=1.0):
+    """krylov_subspace_algorithms.py:299-439, Fong and Saunders' LSMR with
+    damping ``lmbda`` (the reference passes its lmbda, 1 by default), Siddon
+    Ax and matched Atb over all views; the data residual is taken after every
+    iteration and a rise undoes the step and ends the iteration, as in LSQR.
+    Returns (res, residual norms)."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    A = lambda x: ax(x.astype(np.float32), g, "Siddon")  # noqa: E731
+    At = lambda y: atb(y.astype(np.float32), g, "matched")  # noqa: E731
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    u = proj - A(res)
+    beta = np.linalg.norm(u)
+    u = u / beta
+    v = At(u)
+    alpha = np.linalg.norm(v)
+    v = v / alpha
+    alphabar, zetabar = alpha, alpha * beta
+    rho, rhobar, cbar, sbar = 1.0, 1.0, 1.0, 0.0
+    h, hbar = v.copy(), np.zeros_like(v)
+    norms = []
+    for i in range(niter):
+        u = A(v) - alpha * u
+        beta = np.linalg.norm(u)
+        u = u / beta
+        v = At(u) - beta * v
+        alpha = np.linalg.norm(v)
+        v = v / alpha
+        alphahat = np.sqrt(alphabar ** 2 + lmbda ** 2)
+        rhopre = rho
+        rho = np.sqrt(alphahat ** 2 + beta ** 2)
+        c, s = alphahat / rho, beta / rho
+        theta = s * alpha
+        alphabar = c * alpha
+        thetabar = sbar * rho
+        rhobarpre = rhobar
+        rhobar = np.sqrt((cbar * rho) ** 2 + theta ** 2)
+        cbar = cbar * rho / rhobar
+        sbar = theta / rhobar
+        zeta = cbar * zetabar
+        zetabar = -sbar * zetabar
+        hbar = h - (thetabar * rho / (rhopre * rhobarpre)) * hbar
+        step = zeta / (rho * rhobar)
+        res = res + step * hbar
+        h = v - (theta / rho) * h
+        norms.append(float(np.linalg.norm(proj - A(res))))
+        if i > 0 and norms[-1] > norms[-2]:
+            res = res - step * hbar
+            break
+    return res, norms

@@ -171,3 +171,13 @@ def test_mlem(cuda, problem):
     err = oracle.rel_l2(got, ref)
     print("MLEM rel-L2 %.3e" % err)
     assert err <= 1e-4
+
+
+def test_lsmr(cuda, problem):
+    geo, angles, img, proj = problem
+    got, l2 = algs.lsmr(proj, geo, angles, 4, computel2=True)
+    ref, norms = oalg.lsmr(proj, geo, angles, 4)
+    err = oracle.rel_l2(got, ref)
+    print("LSMR rel-L2 %.3e, residuals %s" % (err, l2))
+    assert err <= 1e-4
+    assert np.allclose(l2[0], norms, rtol=1e-4)

@@ -384,3 +384,27 @@ def test_mlem_keeps_an_exact_solution():
     one = oalg.mlem(proj, geo, angles, 1)
     three = oalg.mlem(proj, geo, angles, 3)
     assert loglik(np.ones_like(img)) < loglik(one) < loglik(three)
+
+
+@pytest.mark.parametrize("damp", [1.0, 0.0])
+def test_lsmr_matches_scipy_lsmr(damp):
+    """The LSMR restatement follows Fong and Saunders' recurrences with the
+    damping as the reference passes it: on a tiny problem it equals scipy's
+    lsmr(damp=...) on the same operators as dense matrices."""
+    from scipy.sparse.linalg import LinearOperator, lsmr as scipy_lsmr
+
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.0, 1.0), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 5, endpoint=False)
+    A, B = _dense_siddon_matched(geo, angles)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 2, seed=8)
+    b = A @ img.ravel().astype(np.float64) + np.random.default_rng(2).normal(0, 0.01, A.shape[0])
+    op = LinearOperator(A.shape, matvec=lambda x: A @ x, rmatvec=lambda y: B @ y, dtype=np.float64)
+    compared = 0
+    for n in (1, 3, 6):
+        got, norms = oalg.lsmr(b.reshape((5, 8, 8)), geo, angles, n, lmbda=damp)
+        if len(norms) < n:  # the residual rose: the restatement stopped early
+            break
+        ref = scipy_lsmr(op, b, damp=damp, atol=0, btol=0, conlim=0, maxiter=n)[0]
+        assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
+        compared += 1
+    assert compared >= 2

@@ -2,7 +2,7 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``lsqr``, ``mlem``, ``asd_pocs``,
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``lsmr``, ``mlem``, ``asd_pocs``,
 ``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
@@ -26,8 +26,9 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr", "mlem",
-           "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
+__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr", "lsmr",
+           "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets",
+           "BatchedOSSART"]
 
 
 def _torch():
@@ -495,6 +496,67 @@ def lsqr(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
             res = torch.sub(res, w_prev, alpha=phi / rho)
             if verbose or not restart:
                 print("LSQR exited due to divergence.")
+            break
+    out = _result(res, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
+def lsmr(proj, geo, angles, niter, lmbda=1.0, restart=True, verbose=False, return_tensor=False, computel2=False,
+         **kwargs):
+    """LSMR (krylov_subspace_algorithms.py:299-439), Fong and Saunders'
+    method with damping ``lmbda`` (1 by default, as the reference passes its
+    lmbda), Siddon Ax and matched Atb over all views, FP64 scalars and norms.
+    The data residual is taken after every iteration; a rise undoes the step
+    and ends the iteration, as in LSQR."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    A = lambda x: vs.ax(x, 0, _native.PROJ_SIDDON)  # noqa: E731
+    At = lambda y: vs.atb(y, 0, _native.BACKPROJ_MATCHED)  # noqa: E731
+    res = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+    u = b - A(res)
+    beta = _norm(u)
+    u = u / beta
+    v = At(u)
+    alpha = _norm(v)
+    v = v / alpha
+    alphabar, zetabar = alpha, alpha * beta
+    rho, rhobar, cbar, sbar = 1.0, 1.0, 1.0, 0.0
+    h, hbar = v.clone(), torch.zeros_like(v)
+    l2 = []
+    for i in range(niter):
+        if verbose:
+            print("LSMR iteration %d of %d" % (i + 1, niter))
+        u = torch.sub(A(v), u, alpha=alpha)
+        beta = _norm(u)
+        u = u / beta
+        v = torch.sub(At(u), v, alpha=beta)
+        alpha = _norm(v)
+        v = v / alpha
+        alphahat = float(np.sqrt(alphabar ** 2 + lmbda ** 2))
+        rhopre = rho
+        rho = float(np.sqrt(alphahat ** 2 + beta ** 2))
+        c, s = alphahat / rho, beta / rho
+        theta = s * alpha
+        alphabar = c * alpha
+        thetabar = sbar * rho
+        rhobarpre = rhobar
+        rhobar = float(np.sqrt((cbar * rho) ** 2 + theta ** 2))
+        cbar = cbar * rho / rhobar
+        sbar = theta / rhobar
+        zeta = cbar * zetabar
+        zetabar = -sbar * zetabar
+        hbar = torch.sub(h, hbar, alpha=thetabar * rho / (rhopre * rhobarpre))
+        step = zeta / (rho * rhobar)
+        res = torch.add(res, hbar, alpha=step)
+        h = torch.sub(v, h, alpha=theta / rho)
+        l2.append(_norm(b - A(res)))
+        if i > 0 and l2[-1] > l2[-2]:
+            res = torch.sub(res, hbar, alpha=step)
+            if verbose or not restart:
+                print("LSMR exited due to divergence.")
             break
     out = _result(res, return_tensor)
     return (out, np.array([l2])) if computel2 else out
