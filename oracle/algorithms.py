@@ -12,7 +12,8 @@ Followed files (Python/tigre/):
   utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
   algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
   algorithms/ista_algorithms.py:113-209 FISTA, ISTA
-  algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :299-439 LSMR
+  algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :299-439 LSMR,
+                                        :570-695 AB-GMRES, BA-GMRES
   algorithms/statistical_algorithms.py:11-55     MLEM
 """
 from __future__ import annotations
@@ -447,3 +448,51 @@ def lsmr(proj, geo, angles, niter, lmbda=1.0):
             res = res - step * hbar
             break
     return res, norms
+
+
+def gmres(proj, geo, angles, niter, variant="ba", backprojector="matched", restart=True):
+    """krylov_subspace_algorithms.py:570-695: GMRES on the right- (AB) or
+    left-preconditioned (BA) system, Siddon Ax and B = matched Atb (or FDK).
+    BA: Arnoldi on B A over volumes from r0 = B(proj); AB: on A B over
+    projections from r0 = proj, the image x = sum_i B(w_i) y_i.  After
+    iteration k the data residual |proj - A x_k| is taken; when it rises at
+    k = 1 (or at any k without ``restart``) that iterate is returned, as the
+    reference does; after the last iteration x = sum_i w_i y_i (BA)."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    A = lambda x: ax(x.astype(np.float32), g, "Siddon")  # noqa: E731
+    if backprojector == "FDK":
+        B = lambda y: fdk(y, geo, angles)[0]  # noqa: E731
+    else:
+        B = lambda y: atb(y.astype(np.float32), g, "matched")  # noqa: E731
+    r = B(proj) if variant == "ba" else proj.copy()
+    beta = np.linalg.norm(r)
+    w = [r / beta]
+    z = []  # AB: B(w_i)
+    H = np.zeros((niter + 1, niter))
+    norms = []
+
+    def image(y):
+        basis = w if variant == "ba" else z
+        return sum(yi * bi for yi, bi in zip(y, basis))
+
+    y = None
+    for k in range(niter):
+        if variant == "ba":
+            q = B(A(w[k]))
+        else:
+            z.append(B(w[k]))
+            q = A(z[k])
+        for i in range(k + 1):
+            H[i, k] = float(np.sum(q * w[i]))
+            q = q - H[i, k] * w[i]
+        H[k + 1, k] = np.linalg.norm(q)
+        w.append(q / H[k + 1, k])
+        e1 = np.zeros(k + 2)
+        e1[0] = beta
+        y = np.linalg.lstsq(H[:k + 2, :k + 1], e1, rcond=None)[0]
+        x = image(y)
+        norms.append(float(np.linalg.norm(proj - A(x))))
+        if k > 0 and norms[-1] > norms[-2] and (k == 1 or not restart):
+            return x, norms
+    return image(y), norms

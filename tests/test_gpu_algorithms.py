@@ -181,3 +181,15 @@ def test_lsmr(cuda, problem):
     print("LSMR rel-L2 %.3e, residuals %s" % (err, l2))
     assert err <= 1e-4
     assert np.allclose(l2[0], norms, rtol=1e-4)
+
+
+@pytest.mark.parametrize("variant", ["ab", "ba"])
+def test_gmres(cuda, problem, variant):
+    geo, angles, img, proj = problem
+    fn = algs.ab_gmres if variant == "ab" else algs.ba_gmres
+    got, l2 = fn(proj, geo, angles, 4, computel2=True)
+    ref, norms = oalg.gmres(proj, geo, angles, 4, variant=variant)
+    err = oracle.rel_l2(got, ref)
+    print("%s-GMRES rel-L2 %.3e, residuals %s" % (variant.upper(), err, l2))
+    assert err <= 1e-4
+    assert np.allclose(l2[0], norms, rtol=1e-4)

@@ -408,3 +408,31 @@ def test_lsmr_matches_scipy_lsmr(damp):
         assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
         compared += 1
     assert compared >= 2
+
+
+@pytest.mark.parametrize("variant", ["ba", "ab"])
+def test_gmres_matches_scipy_gmres(variant):
+    """AB- / BA-GMRES restated: on a tiny problem the k-th iterate equals
+    scipy's gmres after k Arnoldi steps on B A x = B b (BA) or A B u = b with
+    x = B u (AB), the operators as dense matrices (Siddon forward, matched
+    backward)."""
+    from scipy.sparse.linalg import gmres as scipy_gmres
+
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.0, 1.0), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 5, endpoint=False)
+    A, B = _dense_siddon_matched(geo, angles)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 2, seed=8)
+    b = A @ img.ravel().astype(np.float64) + np.random.default_rng(2).normal(0, 0.01, A.shape[0])
+    compared = 0
+    for n in (1, 2, 4):
+        got, norms = oalg.gmres(b.reshape((5, 8, 8)), geo, angles, n, variant=variant, restart=False)
+        if len(norms) < n:
+            break
+        if variant == "ba":
+            ref = scipy_gmres(B @ A, B @ b, x0=np.zeros(A.shape[1]), rtol=0.0, atol=0.0, restart=n, maxiter=1)[0]
+        else:
+            u = scipy_gmres(A @ B, b, x0=np.zeros(A.shape[0]), rtol=0.0, atol=0.0, restart=n, maxiter=1)[0]
+            ref = B @ u
+        assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
+        compared += 1
+    assert compared >= 2

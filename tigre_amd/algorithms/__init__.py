@@ -2,8 +2,9 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``lsqr``, ``lsmr``, ``mlem``, ``asd_pocs``,
-``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``lsmr``, ``ab_gmres``, ``ba_gmres``,
+``mlem``, ``asd_pocs``, ``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``,
+``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -27,8 +28,8 @@ from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
 __all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr", "lsmr",
-           "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs", "os_piccs", "ViewSets",
-           "BatchedOSSART"]
+           "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs",
+           "os_piccs", "ViewSets", "BatchedOSSART"]
 
 
 def _torch():
@@ -560,6 +561,76 @@ def lsmr(proj, geo, angles, niter, lmbda=1.0, restart=True, verbose=False, retur
             break
     out = _result(res, return_tensor)
     return (out, np.array([l2])) if computel2 else out
+
+
+def _gmres(proj, geo, angles, niter, left, backprojector="matched", restart=True, verbose=False,
+           return_tensor=False, computel2=False, **kwargs):
+    """AB-GMRES (left False) / BA-GMRES (left True), krylov_subspace_algorithms.py:570-695."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    A = lambda x: vs.ax(x, 0, _native.PROJ_SIDDON)  # noqa: E731
+    if backprojector == "FDK":
+        B = lambda y: fdk(y, g, g.angles, return_tensor=True)  # noqa: E731
+    elif backprojector == "matched":
+        B = lambda y: vs.atb(y, 0, _native.BACKPROJ_MATCHED)  # noqa: E731
+    else:
+        raise ValueError("backprojector must be 'matched' or 'FDK'")
+    r = B(b) if left else b.clone()
+    beta = _norm(r)
+    w = [r / beta]
+    z = []  # AB: the backprojected basis B(w_i), kept instead of backprojecting it again
+    H = np.zeros((niter + 1, niter))
+    l2 = []
+
+    def image(y):
+        basis = w if left else z
+        x = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+        for yi, bi in zip(y, basis):
+            x = torch.add(x, bi, alpha=float(yi))
+        return x
+
+    y, x = None, None
+    for k in range(niter):
+        if verbose:
+            print("%s-GMRES iteration %d of %d" % ("BA" if left else "AB", k + 1, niter))
+        if left:
+            q = B(A(w[k]))
+        else:
+            z.append(B(w[k]))
+            q = A(z[k])
+        for i in range(k + 1):
+            H[i, k] = float(torch.dot(q.reshape(-1).double(), w[i].reshape(-1).double()))
+            q = torch.sub(q, w[i], alpha=H[i, k])
+        H[k + 1, k] = _norm(q)
+        w.append(q / H[k + 1, k])
+        e1 = np.zeros(k + 2)
+        e1[0] = beta
+        y = np.linalg.lstsq(H[:k + 2, :k + 1], e1, rcond=None)[0]
+        x = image(y)
+        l2.append(_norm(b - A(x)))
+        if k > 0 and l2[-1] > l2[-2] and (k == 1 or not restart):
+            print("%s-GMRES exited due to divergence at iteration %d" % ("BA" if left else "AB", k))
+            break
+    out = _result(x, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
+def ab_gmres(proj, geo, angles, niter, **kwargs):
+    """AB-GMRES (krylov_subspace_algorithms.py:570-630): GMRES on A B u = proj
+    (Siddon A; B the matched backprojector, or FDK with backprojector='FDK'),
+    image x = B u; FP64 inner products and Hessenberg solve.  The data residual
+    is taken after every iteration; a rise at the second iteration (or at any,
+    with restart=False) ends the iteration there, as in the reference."""
+    return _gmres(proj, geo, angles, niter, False, **kwargs)
+
+
+def ba_gmres(proj, geo, angles, niter, **kwargs):
+    """BA-GMRES (krylov_subspace_algorithms.py:634-695): GMRES on B A x = B
+    proj over volumes, the same operators and stopping rule as ab_gmres."""
+    return _gmres(proj, geo, angles, niter, True, **kwargs)
 
 
 def mlem(proj, geo, angles, niter, init=None, verbose=False, return_tensor=False, **kwargs):
