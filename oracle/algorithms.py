@@ -12,7 +12,7 @@ Followed files (Python/tigre/):
   utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
   algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
   algorithms/ista_algorithms.py:113-209 FISTA, ISTA
-  algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :299-439 LSMR,
+  algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :190-297 hybrid LSQR, :299-439 LSMR,
                                         :570-695 AB-GMRES, BA-GMRES
   algorithms/statistical_algorithms.py:11-55     MLEM
 """
@@ -496,3 +496,54 @@ def gmres(proj, geo, angles, niter, variant="ba", backprojector="matched", resta
         if k > 0 and norms[-1] > norms[-2] and (k == 1 or not restart):
             return x, norms
     return image(y), norms
+
+
+def hybrid_lsqr(proj, geo, angles, niter, lmbda=1.0, restart=True, elementwise_reorth=True):
+    """krylov_subspace_algorithms.py:190-297: Golub-Kahan bidiagonalisation
+    (Siddon Ax, matched Atb) whose projected problem min |B_k y - beta e1|^2 +
+    lmbda^2 |y|^2 is solved through the SVD of B_k; x = V_k y.  The reference's
+    "reorthogonalisation" against the earlier vectors j < i - 1 subtracts the
+    elementwise product (V_j * v) * V_j (not a projection); it is kept as it
+    computes (``elementwise_reorth`` False leaves it out, the textbook
+    bidiagonalisation).  The data residual of each iterate is taken; a rise at
+    the second iteration (or at any, without ``restart``) returns that
+    iterate."""
+    g = prep(geo, angles)
+    proj = np.asarray(proj, dtype=np.float64)
+    A = lambda x: ax(x.astype(np.float32), g, "Siddon")  # noqa: E731
+    At = lambda y: atb(y.astype(np.float32), g, "matched")  # noqa: E731
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    u = proj - A(res)
+    beta = np.linalg.norm(u)
+    u = u / beta
+    U, V = [u], []
+    Bm = np.zeros((niter, niter + 1))
+    rhs = np.zeros(niter + 1)
+    rhs[0] = beta
+    norms = []
+    x = res
+    for i in range(niter):
+        v = At(u)
+        if i > 0:
+            v = v - beta * V[i - 1]
+        for j in range(i - 1 if elementwise_reorth else 0):
+            v = v - (V[j] * v) * V[j]
+        alpha = np.linalg.norm(v)
+        v = v / alpha
+        V.append(v)
+        u = A(v) - alpha * u
+        for j in range(i - 1 if elementwise_reorth else 0):
+            u = u - (U[j] * u) * U[j]
+        beta = np.linalg.norm(u)
+        u = u / beta
+        U.append(u)
+        Bm[i, i] = alpha
+        Bm[i, i + 1] = beta
+        Uk, Sk, Vk = np.linalg.svd(Bm[:i + 1, :i + 2].T)
+        yhat = Sk * (Uk.T @ rhs[:i + 2])[:i + 1] / (Sk ** 2 + lmbda ** 2)
+        y = Vk.T @ yhat
+        x = res + sum(yj * vj for yj, vj in zip(y, V))
+        norms.append(float(np.linalg.norm(proj - A(x))))
+        if i > 0 and norms[-1] > norms[-2] and (i == 1 or not restart):
+            return x, norms
+    return x, norms

@@ -193,3 +193,13 @@ def test_gmres(cuda, problem, variant):
     print("%s-GMRES rel-L2 %.3e, residuals %s" % (variant.upper(), err, l2))
     assert err <= 1e-4
     assert np.allclose(l2[0], norms, rtol=1e-4)
+
+
+def test_hybrid_lsqr(cuda, problem):
+    geo, angles, img, proj = problem
+    got, l2 = algs.hybrid_lsqr(proj, geo, angles, 4, computel2=True)
+    ref, norms = oalg.hybrid_lsqr(proj, geo, angles, 4)
+    err = oracle.rel_l2(got, ref)
+    print("hybrid LSQR rel-L2 %.3e, residuals %s" % (err, l2))
+    assert err <= 1e-4
+    assert np.allclose(l2[0], norms, rtol=1e-4)

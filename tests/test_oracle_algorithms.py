@@ -436,3 +436,31 @@ def test_gmres_matches_scipy_gmres(variant):
         assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
         compared += 1
     assert compared >= 2
+
+
+@pytest.mark.parametrize("damp", [1.0, 0.0])
+def test_hybrid_lsqr_is_damped_lsqr(damp):
+    """Hybrid LSQR with a fixed Tikhonov parameter solves the damped least-squares
+    problem over the Krylov subspace of the bidiagonalisation, which is what
+    LSQR with that damping computes: on a tiny problem it equals scipy's
+    lsqr(damp=...) on the same operators as dense matrices: the reference's
+    form for one and two iterations (its elementwise update against earlier
+    vectors starts at the third), the textbook form for four."""
+    from scipy.sparse.linalg import LinearOperator, lsqr as scipy_lsqr
+
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.0, 1.0), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 5, endpoint=False)
+    A, B = _dense_siddon_matched(geo, angles)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 2, seed=8)
+    b = A @ img.ravel().astype(np.float64) + np.random.default_rng(2).normal(0, 0.01, A.shape[0])
+    op = LinearOperator(A.shape, matvec=lambda x: A @ x, rmatvec=lambda y: B @ y, dtype=np.float64)
+    compared = 0
+    for n in (1, 2, 4):
+        got, norms = oalg.hybrid_lsqr(b.reshape((5, 8, 8)), geo, angles, n, lmbda=damp, restart=False,
+                                      elementwise_reorth=n <= 2)
+        if len(norms) < n:
+            break
+        ref = scipy_lsqr(op, b, damp=damp, atol=0, btol=0, conlim=0, iter_lim=n)[0]
+        assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
+        compared += 1
+    assert compared >= 2

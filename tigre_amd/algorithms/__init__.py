@@ -2,9 +2,9 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``lsqr``, ``lsmr``, ``ab_gmres``, ``ba_gmres``,
-``mlem``, ``asd_pocs``, ``os_asd_pocs``, ``awasd_pocs``, ``os_awasd_pocs``,
-``piccs``, ``os_piccs``.
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``,
+``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
+``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -27,9 +27,9 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr", "lsmr",
-           "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs", "os_awasd_pocs", "piccs",
-           "os_piccs", "ViewSets", "BatchedOSSART"]
+__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
+           "hybrid_lsqr", "lsmr", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
+           "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
 
 
 def _torch():
@@ -499,6 +499,64 @@ def lsqr(proj, geo, angles, niter, restart=True, verbose=False, return_tensor=Fa
                 print("LSQR exited due to divergence.")
             break
     out = _result(res, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
+def hybrid_lsqr(proj, geo, angles, niter, lmbda=1.0, restart=True, verbose=False, return_tensor=False,
+                computel2=False, **kwargs):
+    """Hybrid LSQR (krylov_subspace_algorithms.py:190-297): Golub-Kahan
+    bidiagonalisation with Siddon Ax and matched Atb, the projected Tikhonov
+    problem min |B_k y - beta e1|^2 + lmbda^2 |y|^2 solved through the SVD of
+    B_k in FP64 on the host, x = V_k y.  As in the reference, each new vector
+    v (and u) has (V_j * v) * V_j subtracted elementwise for the vectors
+    j < i - 1 (its reorthogonalisation step as it computes).  The data residual
+    of each iterate is taken; a rise at the second iteration (or at any, with
+    restart=False) ends the iteration there."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    A = lambda x: vs.ax(x, 0, _native.PROJ_SIDDON)  # noqa: E731
+    At = lambda y: vs.atb(y, 0, _native.BACKPROJ_MATCHED)  # noqa: E731
+    u = b - A(torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda"))
+    beta = _norm(u)
+    u = u / beta
+    U, V = [u], []
+    Bm = np.zeros((niter, niter + 1))
+    rhs = np.zeros(niter + 1)
+    rhs[0] = beta
+    l2 = []
+    x = None
+    for i in range(niter):
+        if verbose:
+            print("hybrid LSQR iteration %d of %d" % (i + 1, niter))
+        v = At(u)
+        if i > 0:
+            v = torch.sub(v, V[i - 1], alpha=beta)
+        for j in range(i - 1):
+            v = v - (V[j] * v) * V[j]
+        alpha = _norm(v)
+        v = v / alpha
+        V.append(v)
+        u = torch.sub(A(v), u, alpha=alpha)
+        for j in range(i - 1):
+            u = u - (U[j] * u) * U[j]
+        beta = _norm(u)
+        u = u / beta
+        U.append(u)
+        Bm[i, i] = alpha
+        Bm[i, i + 1] = beta
+        Uk, Sk, Vk = np.linalg.svd(Bm[:i + 1, :i + 2].T)
+        y = Vk.T @ (Sk * (Uk.T @ rhs[:i + 2])[:i + 1] / (Sk ** 2 + lmbda ** 2))
+        x = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+        for yj, vj in zip(y, V):
+            x = torch.add(x, vj, alpha=float(yj))
+        l2.append(_norm(b - A(x)))
+        if i > 0 and l2[-1] > l2[-2] and (i == 1 or not restart):
+            print("hybrid LSQR exited due to divergence at iteration %d" % i)
+            break
+    out = _result(x, return_tensor)
     return (out, np.array([l2])) if computel2 else out
 
 
