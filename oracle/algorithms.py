@@ -8,7 +8,7 @@ Followed files (Python/tigre/):
   utilities/order_subsets.py:31-96      ordered subsets
   algorithms/iterative_recon_alg.py:214-401     W, V, OS-SART update
   algorithms/krylov_subspace_algorithms.py:22-97 CGLS
-  algorithms/pocs_algorithms.py:105-169 ASD-POCS
+  algorithms/pocs_algorithms.py:105-169 ASD-POCS, :434-646 PCSD
   utilities/im_3d_denoise.py:7-28       im3ddenoise (normalised ROF denoise)
   algorithms/art_family_algorithms.py:65-153  SART-TV, OS-SART-TV
   algorithms/ista_algorithms.py:113-209 FISTA, ISTA
@@ -23,7 +23,7 @@ import copy
 import numpy as np
 from scipy.fft import fft, ifft
 
-from . import atb, ax, pocs_tv, tvdenoise
+from . import atb, aw_pocs_tv, ax, pocs_tv, tvdenoise
 
 
 def prep(geo, angles):
@@ -547,3 +547,56 @@ def hybrid_lsqr(proj, geo, angles, niter, lmbda=1.0, restart=True, elementwise_r
         if i > 0 and norms[-1] > norms[-2] and (i == 1 or not restart):
             return x, norms
     return x, norms
+
+
+# -- PCSD ----------------------------------------------------------------------------
+def pcsd(proj, geo, angles, niter, blocksize=1, tviter=20, lmbda=1.0, lmbda_red=1.0, epsilon=None, delta=None,
+         trace=None):
+    """pocs_algorithms.py:434-646 (PCSD, AwPCSD with ``delta``, and their OS
+    forms with ``blocksize``): every iteration, when d_p^2 > epsilon (d_p the
+    data residual of the current image; the reference compares the squared
+    norm with epsilon) an (OS-)SART sweep; then the TV step (pocs_tv, or
+    aw_pocs_tv with ``delta``) of size dtvg = 1 in the first iteration and
+    d_p / d_p1 after it, d_p1 the data residual of the first pre-TV image;
+    beta = lmbda reduced by lmbda_red; the loop stops when c < -0.99 and the
+    post-sweep residual is within epsilon, when beta < 0.005, or after
+    iteration niter + 1 (the reference tests n_iter > niter).  ``trace``
+    receives (n_iter, d_p, dd, dp, dtvg, dg, c) per iteration."""
+    geo = copy.deepcopy(geo)
+    geo.check_geo(angles)
+    ang = geo.angles
+    gfull = prep(geo, ang)
+    proj = np.asarray(proj, dtype=np.float64)
+    blocks = ordered_blocks(len(ang), blocksize)
+    W = weights_w(geo, ang)
+    V = weights_v(geo, ang, blocks)
+    A = lambda x: ax(x.astype(np.float32), gfull)  # noqa: E731
+    if epsilon is None:
+        epsilon = 0.2 * np.linalg.norm(A(fdk(proj, geo, angles)[0]) - proj)
+    res = np.zeros(tuple(int(v) for v in geo.nVoxel))
+    beta, n_iter, d_p1 = lmbda, 0, 1.0
+    while True:
+        res_prev = res.copy()
+        n_iter += 1
+        d_p = np.linalg.norm(A(res) - proj)
+        if d_p ** 2 > epsilon:
+            for j, idx in enumerate(blocks):
+                g = prep(sub_geometry(geo, idx), ang[idx])
+                r = W[idx] * (proj[idx] - ax(res.astype(np.float32), g, "Siddon"))
+                res = np.maximum(res + lmbda / V[j] * atb(r.astype(np.float32), g, "FDK"), 0)
+        dd = np.linalg.norm(A(res) - proj)
+        dp_vec = res - res_prev
+        dp = np.linalg.norm(dp_vec)
+        dtvg = 1.0 if n_iter == 1 else d_p / d_p1
+        res_prev = res.copy()
+        res = pocs_tv(res, dtvg, tviter) if delta is None else aw_pocs_tv(res, dtvg, tviter, delta)
+        dg_vec = res - res_prev
+        dg = np.linalg.norm(dg_vec)
+        if n_iter == 1:
+            d_p1 = np.linalg.norm(A(res_prev) - proj)
+        beta *= lmbda_red
+        c = float(np.dot(dg_vec.ravel(), dp_vec.ravel())) / max(dg * dp, 1e-6)
+        if trace is not None:
+            trace.append((n_iter, float(d_p), float(dd), float(dp), float(dtvg), float(dg), c))
+        if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter > niter:
+            return res

@@ -203,3 +203,25 @@ def test_hybrid_lsqr(cuda, problem):
     print("hybrid LSQR rel-L2 %.3e, residuals %s" % (err, l2))
     assert err <= 1e-4
     assert np.allclose(l2[0], norms, rtol=1e-4)
+
+
+@pytest.mark.parametrize("variant", ["pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd"])
+def test_pcsd(cuda, variant):
+    """PCSD and its forms against the FP64 restatement.  The adaptive-weighted
+    forms use delta = -0.5: at the default -0.005 the weights exp(-(d / delta)^2)
+    vanish at every edge of the piecewise-constant phantom and the step is
+    driven by differences at the rounding level of FP32, which FP64 does not
+    share."""
+    geo = cone_geometry((24, 24, 24), (24.0, 24.0, 24.0), (32, 32), (1.0, 1.0), 300.0, 450.0)
+    angles = np.linspace(0, 2 * np.pi, 20, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 4, seed=3)
+    proj = oracle.ax(img, prepared(geo, angles)).astype(np.float32)
+    fn = getattr(algs, variant)
+    kw = {"blocksize": 5} if variant.startswith("os") else {}
+    if "aw" in variant:
+        kw["delta"] = -0.5
+    got = fn(proj, geo, angles, 2, **kw)
+    ref = oalg.pcsd(proj, geo, angles, 2, blocksize=kw.get("blocksize", 1), delta=kw.get("delta"))
+    err = oracle.rel_l2(got, ref)
+    print("%s rel-L2 %.3e" % (variant, err))
+    assert err <= 1e-3

@@ -464,3 +464,35 @@ def test_hybrid_lsqr_is_damped_lsqr(damp):
         assert np.abs(got.ravel() - ref).max() <= 1e-4 * np.abs(ref).max(), n
         compared += 1
     assert compared >= 2
+
+
+# -- PCSD ----------------------------------------------------------------------------
+def test_pcsd_without_tv_is_sart(small_problem):
+    """With no TV iterations and a data bound every image exceeds, PCSD is
+    SART sweeps, niter + 1 of them (the reference stops at n_iter > niter)."""
+    geo, angles, _, proj = small_problem
+    sart = oalg.os_sart(proj, geo, angles, 3, blocksize=1)
+    res = oalg.pcsd(proj, geo, angles, 2, tviter=0, epsilon=1e-12)
+    assert np.array_equal(res, sart)
+
+
+def test_pcsd_tv_step_rule(small_problem):
+    """The TV step size is 1 in the first iteration and d_p / d_p1 after it:
+    recomputed here from the data residuals of the images themselves; with one
+    TV iteration each step moves the image by that size in L2 (one POCS-TV
+    step moves it by alpha |g| / sqrt(|g|^2 + 1e-7))."""
+    geo, angles, _, proj = small_problem
+    g = prepared(geo, angles)
+    trace = []
+    oalg.pcsd(proj, geo, angles, 3, tviter=1, epsilon=1e-12, trace=trace)
+    assert len(trace) == 4
+    assert trace[0][4] == 1.0
+    # d_p1 = residual of the first pre-TV image = dd of the first iteration
+    d_p1 = trace[0][2]
+    for n_iter, d_p, dd, dp, dtvg, dg, c in trace[1:]:
+        assert abs(dtvg - d_p / d_p1) <= 1e-12 * dtvg
+    for row in trace:
+        assert abs(row[5] - row[4]) <= 1e-6 * row[4]
+    # d_p of each iteration is the residual of the previous post-TV image
+    res = oalg.pcsd(proj, geo, angles, 0, tviter=1, epsilon=1e-12)
+    assert abs(np.linalg.norm(oracle.ax(res.astype(np.float32), g) - proj) - trace[1][1]) <= 1e-9 * trace[1][1]

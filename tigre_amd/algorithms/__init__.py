@@ -4,7 +4,8 @@ Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
 ``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``,
 ``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
-``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``.
+``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``, ``pcsd``,
+``aw_pcsd``, ``os_pcsd``, ``os_aw_pcsd``.
 The volume, the projections and every intermediate stay in HBM as torch
 tensors; each projector call goes straight to the gfx950 entry points with
 per-subset geometry records prepared once (no per-call host round trip,
@@ -29,7 +30,8 @@ from ..utilities.order_subsets import order_subsets
 
 __all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
            "hybrid_lsqr", "lsmr", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
-           "os_awasd_pocs", "piccs", "os_piccs", "ViewSets", "BatchedOSSART"]
+           "os_awasd_pocs", "piccs", "os_piccs", "pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd", "ViewSets",
+           "BatchedOSSART"]
 
 
 def _torch():
@@ -904,6 +906,74 @@ def piccs(proj, geo, angles, niter, prior=None, prior_ratio=0.2, tviter=20, bloc
 def os_piccs(proj, geo, angles, niter, blocksize=20, **kwargs):
     """Ordered-subsets PICCS (pocs_algorithms.py OS_PICCS)."""
     return piccs(proj, geo, angles, niter, blocksize=blocksize, **kwargs)
+
+
+# ---------------------------------------------------------------------------
+# PCSD (pocs_algorithms.py:434-646)
+# ---------------------------------------------------------------------------
+def _pcsd(proj, geo, angles, niter, blocksize, delta=None, tviter=20, maxl2err=None, lmbda=1.0, lmbda_red=1.0,
+          OrderStrategy=None, verbose=False, return_tensor=False, **kwargs):
+    from ..tv import AwminTV, minTV
+
+    torch = _torch()
+    st = _SartState(proj, geo, angles, blocksize, order=OrderStrategy, lmbda=lmbda, noneg=True, verbose=verbose)
+    full = ViewSets(st.geo, st.angles, [np.arange(len(st.angles))])
+    if maxl2err is None:
+        epsilon = 0.2 * _norm(full.ax(fdk(st.proj, st.geo, st.angles, return_tensor=True), 0) - st.proj)
+    else:
+        epsilon = maxl2err
+    beta, n_iter, d_p1 = lmbda, 0, 1.0
+    while True:
+        res_prev = st.res.clone()
+        n_iter += 1
+        d_p = _norm(full.ax(st.res, 0) - st.proj)
+        if d_p ** 2 > epsilon:  # the reference compares the squared residual norm with epsilon
+            st.sweep()
+        dd = _norm(full.ax(st.res, 0) - st.proj)
+        dp_vec = st.res - res_prev
+        dp = _norm(dp_vec)
+        dtvg = 1.0 if n_iter == 1 else d_p / d_p1
+        res_prev = st.res.clone()
+        st.res = minTV(st.res, dtvg, tviter) if delta is None else AwminTV(st.res, dtvg, tviter, delta)
+        dg_vec = st.res - res_prev
+        dg = _norm(dg_vec)
+        if n_iter == 1:
+            d_p1 = _norm(full.ax(res_prev, 0) - st.proj)
+        beta *= lmbda_red
+        c = float(torch.dot(dg_vec.reshape(-1).double(), dp_vec.reshape(-1).double())) / max(dg * dp, 1e-6)
+        if verbose:
+            print("PCSD iteration %d: data residual %.6g, TV step %.6g" % (n_iter, dd, dtvg))
+        if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter > niter:
+            break
+    return _result(st.res, return_tensor)
+
+
+def pcsd(proj, geo, angles, niter, **kwargs):
+    """Projection-controlled steepest descent (pocs_algorithms.py:434-586):
+    every iteration a SART sweep while the squared data residual exceeds
+    epsilon (0.2 |Ax(FDK) - proj| unless maxl2err), then ``tviter`` (20) TV
+    steps of size 1 in the first iteration and d_p / d_p1 after it; the loop
+    ends on the reference's criteria, at the latest after iteration niter + 1."""
+    if kwargs.pop("blocksize", 1) > 1:
+        print("Warning: blocksize is set to 1, please use an OS version of the algorithm for blocksize > 1")
+    return _pcsd(proj, geo, angles, niter, 1, **kwargs)
+
+
+def aw_pcsd(proj, geo, angles, niter, delta=-0.005, **kwargs):
+    """Adaptive-weighted PCSD (pocs_algorithms.py:589-608): the TV step is AwminTV with ``delta``."""
+    if kwargs.pop("blocksize", 1) > 1:
+        print("Warning: blocksize is set to 1, please use an OS version of the algorithm for blocksize > 1")
+    return _pcsd(proj, geo, angles, niter, 1, delta=delta, **kwargs)
+
+
+def os_pcsd(proj, geo, angles, niter, blocksize=20, **kwargs):
+    """Ordered-subsets PCSD (pocs_algorithms.py:611-625), 20 views per subset by default."""
+    return _pcsd(proj, geo, angles, niter, blocksize, **kwargs)
+
+
+def os_aw_pcsd(proj, geo, angles, niter, blocksize=20, delta=-0.005, **kwargs):
+    """Ordered-subsets adaptive-weighted PCSD (pocs_algorithms.py:628-646)."""
+    return _pcsd(proj, geo, angles, niter, blocksize, delta=delta, **kwargs)
 
 
 from .batched import BatchedOSSART  # noqa: E402  (uses _SartState above)
