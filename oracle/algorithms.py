@@ -4,7 +4,7 @@ oracle operators.  Used to check tigre_amd.algorithms on small problems.
 
 Followed files (Python/tigre/):
   utilities/filtering.py:15-87          ramp/Shepp-Logan/... filtering
-  algorithms/single_pass_algorithms.py:12-202   FDK
+  algorithms/single_pass_algorithms.py:12-224   FDK, FBP
   utilities/order_subsets.py:31-96      ordered subsets
   algorithms/iterative_recon_alg.py:214-401     W, V, OS-SART update
   algorithms/krylov_subspace_algorithms.py:22-97 CGLS
@@ -128,6 +128,23 @@ def fdk(proj, geo, angles, filter_name=None, dowang=True):
     buf[:, :, pad:pad + nU] = p
     out = np.real(ifft(fft(buf, axis=2) * filt.astype(np.float64), axis=2))[:, :, pad:pad + nU] * scale
     return atb(out.astype(np.float32), g, "FDK"), out
+
+
+def fbp(proj, geo, angles, filter_name=None):
+    """single_pass_algorithms.py:208-224 (parallel beam): the projections
+    ramp-filtered as in FDK but without the cosine weight, backprojected and
+    scaled by DSO / DSD (in float32) of the first view."""
+    g = prep(geo, angles)
+    nV, nU = int(g.nDetector[0]), int(g.nDetector[1])
+    dU = float(g.dDetector[1])
+    DSD0, DSO0 = float(np.asarray(g.DSD)[0]), float(np.asarray(g.DSO)[0])
+    filt, L = ramp_filter(g, filter_name)
+    pad = (L - nU) // 2
+    scale = (DSD0 / DSO0) * (2 * np.pi / len(g.angles)) / (4 * dU)
+    buf = np.zeros((len(g.angles), nV, L))
+    buf[:, :, pad:pad + nU] = np.asarray(proj, dtype=np.float64)
+    out = np.real(ifft(fft(buf, axis=2) * filt.astype(np.float64), axis=2))[:, :, pad:pad + nU] * scale
+    return atb(out.astype(np.float32), g, "matched") * float(np.float32(DSO0 / DSD0))
 
 
 # -- OS-SART -----------------------------------------------------------------------

@@ -225,3 +225,24 @@ def test_pcsd(cuda, variant):
     err = oracle.rel_l2(got, ref)
     print("%s rel-L2 %.3e" % (variant, err))
     assert err <= 1e-3
+
+
+def test_fbp_parallel(cuda):
+    """Parallel-beam FBP against the FP64 restatement (FBP is linear: 1e-5);
+    a cone geometry is refused as in the reference; the filter keyword lands
+    on the geometry for the next call, as the reference does."""
+    from tigre_amd.geometry import geometry
+
+    geo = geometry(mode="parallel", nVoxel=np.array([32, 40, 40]))
+    angles = np.linspace(0, np.pi, 90, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 5, seed=12)
+    proj = oracle.ax(img, prepared(geo, angles)).astype(np.float32)
+    got = algs.fbp(proj, geo, angles, filter="hann")
+    err = oracle.rel_l2(got, oalg.fbp(proj, geo, angles))
+    print("FBP rel-L2 %.3e" % err)
+    assert err <= 1e-5
+    assert geo.filter == "hann"
+    again = algs.fbp(proj, geo, angles)
+    assert oracle.rel_l2(again, oalg.fbp(proj, geo, angles, "hann")) <= 1e-5
+    with pytest.raises(ValueError):
+        algs.fbp(proj, cone_geometry((8, 8, 8), (8.0, 8.0, 8.0), (8, 8), (1.0, 1.0), 100.0, 150.0), angles)

@@ -1,7 +1,7 @@
 """Reconstruction algorithms on device-resident data.
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
-``fdk``/``FDK``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
+``fdk``/``FDK``, ``fbp``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
 ``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``,
 ``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
 ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``, ``pcsd``,
@@ -28,7 +28,7 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
+__all__ = ["fdk", "FDK", "fbp", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
            "hybrid_lsqr", "lsmr", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
            "os_awasd_pocs", "piccs", "os_piccs", "pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd", "ViewSets",
            "BatchedOSSART"]
@@ -218,6 +218,26 @@ def fdk(proj, geo, angles, filter=None, verbose=False, return_tensor=False, **kw
 
 
 FDK = fdk
+
+
+def fbp(proj, geo, angles, filter=None, verbose=False, return_tensor=False, **kwargs):
+    """Filtered backprojection for a parallel beam (single_pass_algorithms.py
+    :208-224): the projections ramp-filtered as in FDK (no cosine weight),
+    backprojected, scaled by DSO / DSD of the first view in float32.  As in the
+    reference, the filter applied is the one ``geo`` holds on entry (ram-lak
+    when none) and ``filter`` is stored on ``geo`` for later calls."""
+    if getattr(geo, "mode", "cone") != "parallel":
+        raise ValueError("Only use FBP for parallel beam. Check geo.mode.")
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    geo.filter = filter
+    p = _to_device(proj)
+    filt = filtering(p, g, g.angles, parker=False, verbose=verbose)
+    weight = float(np.float32(float(np.asarray(g.DSO).reshape(-1)[0]) / float(np.asarray(g.DSD).reshape(-1)[0])))
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    out = vs.atb(filt, 0, _native.BACKPROJ_MATCHED)
+    out *= weight
+    return _result(out, return_tensor)
 
 
 # ---------------------------------------------------------------------------
