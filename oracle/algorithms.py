@@ -15,6 +15,7 @@ Followed files (Python/tigre/):
   algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :190-297 hybrid LSQR, :299-439 LSMR,
                                         :570-695 AB-GMRES, BA-GMRES
   algorithms/statistical_algorithms.py:11-55     MLEM
+  algorithms/krylov_subspace_algorithms.py:441-567 IRN-TV-CGLS
 """
 from __future__ import annotations
 
@@ -617,3 +618,91 @@ def pcsd(proj, geo, angles, niter, blocksize=1, tviter=20, lmbda=1.0, lmbda_red=
             trace.append((n_iter, float(d_p), float(dd), float(dp), float(dtvg), float(dg), c))
         if (c < -0.99 and dd <= epsilon) or beta < 0.005 or n_iter > niter:
             return res
+
+
+# -- IRN-TV-CGLS ---------------------------------------------------------------------
+def irn_diffs(x):
+    """The differences of krylov_subspace_algorithms.py:456-476, one array per
+    axis: d_i = x_i - x_{i+1} for i < n - 2, while the last two entries along
+    that axis keep x itself (the reference overwrites [0:-2] of a copy)."""
+    out = []
+    for a in range(3):
+        d = np.array(x, dtype=np.float64, copy=True)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[a], hi[a] = slice(0, -2), slice(1, -1)
+        d[tuple(lo)] = d[tuple(lo)] - np.asarray(x, np.float64)[tuple(hi)]
+        out.append(d)
+    return out
+
+
+def irn_weights(x):
+    """(|D x|^2 + 1e-6)^(-1/4) (krylov_subspace_algorithms.py:456-465)."""
+    d = irn_diffs(x)
+    return (d[0] ** 2 + d[1] ** 2 + d[2] ** 2 + 1e-6) ** -0.25
+
+
+def irn_lx(W, x):
+    """Lx (:467-476): W D x, three arrays."""
+    return [W * d for d in irn_diffs(x)]
+
+
+def irn_ltx(W, y):
+    """Ltx (:478-496), as the reference computes it: per axis w = W y_a, then
+    out_0 = w_0, out_i = w_i - w_{i-1} for 0 < i < n - 2, out_{n-2} = w_{n-2},
+    out_{n-1} = -w_{n-2} (not the transpose of irn_lx on its last two rows)."""
+    total = 0.0
+    for a in range(3):
+        w = W * np.asarray(y[a], np.float64)
+        o = w.copy()
+        mid, prev, last, second = ([slice(None)] * 3 for _ in range(4))
+        mid[a], prev[a], last[a], second[a] = slice(1, -2), slice(0, -3), -1, -2
+        o[tuple(mid)] = w[tuple(mid)] - w[tuple(prev)]
+        o[tuple(last)] = -w[tuple(second)]
+        total = total + o
+    return total
+
+
+def irn_cycles(b, x0, A, At, weights, lx, ltx, niter, niter_outer, lmbda):
+    """The loop of krylov_subspace_algorithms.py:498-565 over given operators:
+    each outer cycle builds W from the current image, then runs niter CGLS
+    steps on [A; sqrt(lmbda) W D] x = [b; 0] from res0, the image before the
+    previous cycle's last update (the reference's res0 is reassigned inside
+    the inner loop).  FP64 throughout; returns (x, per-step ||b - A x||)."""
+    sl = float(np.sqrt(lmbda))
+    res = np.asarray(x0, dtype=np.float64)
+    res0 = res
+    norms = []
+    for _ in range(niter_outer):
+        W = weights(res)
+        res = res0
+        r1 = b - A(res)
+        r2 = [-sl * t for t in lx(W, res)]
+        p = At(r1) + sl * ltx(W, r2)
+        gamma = float(np.sum(p * p))
+        for _ in range(niter):
+            res0 = res
+            q1 = A(p)
+            q2 = [sl * t for t in lx(W, p)]
+            alpha = gamma / (float(np.sum(q1 * q1)) + sum(float(np.sum(t * t)) for t in q2))
+            res = res + alpha * p
+            norms.append(float(np.linalg.norm(b - A(res))))
+            r1 = r1 - alpha * q1
+            r2 = [u - alpha * v for u, v in zip(r2, q2)]
+            s = At(r1) + sl * ltx(W, r2)
+            gamma1 = float(np.sum(s * s))
+            p = s + (gamma1 / gamma) * p
+            gamma = gamma1
+    return res, norms
+
+
+def irn_tv_cgls(proj, geo, angles, niter, lmbda=1.0, niter_outer=4, init=None):
+    """IRN-TV-CGLS (krylov_subspace_algorithms.py:441-567) with Siddon forward
+    and matched backprojection."""
+    g = prep(geo, angles)
+    b = np.asarray(proj, dtype=np.float64)
+    shape = tuple(int(v) for v in geo.nVoxel)
+    x0 = np.zeros(shape) if init is None else np.asarray(init, np.float64).reshape(shape)
+    return irn_cycles(b, x0, lambda x: ax(np.asarray(x, np.float32), g, "Siddon"),
+                      lambda y: atb(np.asarray(y, np.float32), g, "matched"),
+                      irn_weights, irn_lx, irn_ltx, niter, niter_outer, lmbda)

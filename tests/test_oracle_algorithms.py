@@ -511,3 +511,78 @@ def test_fbp_reconstructs_parallel_phantom():
     assert nrmse(rec, img) < 0.3
     rec2 = oalg.fbp(2.5 * proj, geo, angles)
     assert np.abs(rec2 - 2.5 * rec).max() <= 1e-5 * np.abs(rec).max()
+
+
+def test_irn_differences_closed_form():
+    """On a ramp x = 2 i along one axis the differences are -2 except the last
+    two entries, which keep x; the other axes' differences vanish except
+    there.  irn_ltx equals the transpose of irn_lx except on the last two rows
+    of each axis (the reference's form), checked on the dense matrices."""
+    n = (5, 4, 6)
+    x = np.broadcast_to(2.0 * np.arange(n[0])[:, None, None], n).copy()
+    d = oalg.irn_diffs(x)
+    assert np.all(d[0][:-2] == -2.0) and np.all(d[0][-2:] == x[-2:])
+    assert np.all(d[1][:, :-2] == 0.0) and np.all(d[1][:, -2:] == x[:, -2:])
+    rng = np.random.default_rng(3)
+    W = rng.uniform(0.5, 2.0, n)
+    N = int(np.prod(n))
+    E = np.eye(N)
+    L = np.stack([np.concatenate([t.ravel() for t in oalg.irn_lx(W, E[k].reshape(n))]) for k in range(N)], 1)
+    E3 = np.eye(3 * N)
+    Lt = np.stack([oalg.irn_ltx(W, [E3[k][j * N:(j + 1) * N].reshape(n) for j in range(3)]).ravel()
+                   for k in range(3 * N)], 1)
+    diff = np.abs(Lt - L.T).reshape(n + (3 * N,))
+    keep = np.ones(n, bool)
+    keep[-2:, :, :] = keep[:, -2:, :] = keep[:, :, -2:] = False
+    assert diff[keep].max() == 0.0 and diff[~keep].max() > 0.1
+
+
+@pytest.mark.parametrize("start", ["zero", "random"])
+def test_irn_cycle_is_lsqr_on_stacked_system(start):
+    """One outer cycle of the IRN loop is CGLS on [A; sqrt(lmbda) W D] x =
+    [b; 0]: with an exact transpose pair its k-th iterate equals scipy's lsqr
+    (k steps, no stopping tests) on the stacked dense matrix from the same
+    start; with a second cycle the restart is from the image before the last
+    update, with weights from the last one."""
+    from scipy.sparse.linalg import lsqr as sp_lsqr
+
+    rng = np.random.default_rng(11)
+    n = (3, 3, 4)
+    N = int(np.prod(n))
+    A = rng.standard_normal((30, N))
+    b = rng.standard_normal(30)
+    Ds = [rng.standard_normal((N, N)) for _ in range(3)]
+    W = rng.uniform(0.5, 1.5, N)
+    lmbda = 0.7
+    x0 = np.zeros(n) if start == "zero" else rng.standard_normal(n)
+    fA = lambda x: A @ np.ravel(x)
+    fAt = lambda y: (A.T @ y).reshape(n)
+    weights = lambda x: W.reshape(n)
+    lx = lambda Wt, x: [Wt.ravel() * (D @ np.ravel(x)) for D in Ds]
+    ltx = lambda Wt, y: sum(D.T @ (Wt.ravel() * yy) for D, yy in zip(Ds, y)).reshape(n)
+    K = np.vstack([A] + [np.sqrt(lmbda) * W[:, None] * D for D in Ds])
+    rhs = np.concatenate([b, np.zeros(3 * N)])
+    for k in (1, 3, 6):
+        x, norms = oalg.irn_cycles(b, x0, fA, fAt, weights, lx, ltx, k, 1, lmbda)
+        ref = x0.ravel() + sp_lsqr(K, rhs - K @ x0.ravel(), atol=0, btol=0, conlim=0, iter_lim=k)[0]
+        assert np.abs(x.ravel() - ref).max() <= 1e-8 * np.abs(ref).max()
+        assert abs(norms[-1] - np.linalg.norm(b - A @ x.ravel())) <= 1e-12 * norms[-1]
+    # two cycles = one cycle of 4, then one from the iterate after 3 steps
+    two, _ = oalg.irn_cycles(b, x0, fA, fAt, weights, lx, ltx, 4, 2, lmbda)
+    x3, _ = oalg.irn_cycles(b, x0, fA, fAt, weights, lx, ltx, 3, 1, lmbda)
+    again, _ = oalg.irn_cycles(b, x3, fA, fAt, weights, lx, ltx, 4, 1, lmbda)
+    assert np.abs(two - again).max() <= 1e-9 * np.abs(again).max()
+
+
+def test_irn_tv_cgls_weights_follow_the_image():
+    """On the tomography problem the first cycle (W constant from a zero
+    image) lowers the data residual every step, and later cycles reweight:
+    the result differs from a single cycle of the same total length."""
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.5, 1.5), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    img = ellipsoid_phantom(geo.nVoxel, geo.sVoxel, 3, seed=5)
+    proj = oracle.ax(img, prepared(geo, angles))
+    x, norms = oalg.irn_tv_cgls(proj, geo, angles, 5, niter_outer=1)
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(norms, norms[1:]))
+    x2, _ = oalg.irn_tv_cgls(proj, geo, angles, 5, niter_outer=2)
+    assert np.abs(x2 - x).max() > 1e-6 * np.abs(x).max()

@@ -2,7 +2,7 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``fbp``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``,
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``, ``irn_tv_cgls``,
 ``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
 ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``, ``pcsd``,
 ``aw_pcsd``, ``os_pcsd``, ``os_aw_pcsd``.
@@ -29,7 +29,7 @@ from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
 __all__ = ["fdk", "FDK", "fbp", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
-           "hybrid_lsqr", "lsmr", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
+           "hybrid_lsqr", "lsmr", "irn_tv_cgls", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
            "os_awasd_pocs", "piccs", "os_piccs", "pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd", "ViewSets",
            "BatchedOSSART"]
 
@@ -639,6 +639,91 @@ def lsmr(proj, geo, angles, niter, lmbda=1.0, restart=True, verbose=False, retur
             if verbose or not restart:
                 print("LSMR exited due to divergence.")
             break
+    out = _result(res, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
+def _irn_diffs(x):
+    """Per-axis differences of krylov_subspace_algorithms.py:456-476, stacked:
+    d_i = x_i - x_{i+1} for i < n - 2; the last two entries along each axis
+    keep x itself, as the reference's slices leave them."""
+    torch = _torch()
+    out = []
+    for a in range(3):
+        d = x.clone()
+        n = x.shape[a]
+        if n > 2:
+            d.narrow(a, 0, n - 2).sub_(x.narrow(a, 1, n - 2))
+        out.append(d)
+    return torch.stack(out)
+
+
+def _irn_weights(x):
+    """(|D x|^2 + 1e-6)^(-1/4) (krylov_subspace_algorithms.py:456-465)."""
+    return _irn_diffs(x).square().sum(0).add_(1e-6).pow_(-0.25)
+
+
+def _irn_ltx(W, y):
+    """Ltx (krylov_subspace_algorithms.py:478-496) as the reference computes it:
+    per axis w = W y_a, out_0 = w_0, out_i = w_i - w_{i-1} for 0 < i < n - 2,
+    out_{n-2} = w_{n-2}, out_{n-1} = -w_{n-2}."""
+    w = y * W
+    o = w.clone()
+    for a in range(3):
+        n = w.shape[a + 1]
+        if n > 3:
+            o[a].narrow(a, 1, n - 3).sub_(w[a].narrow(a, 0, n - 3))
+        o[a].select(a, n - 1).copy_(w[a].select(a, n - 2).neg())
+    return o.sum(0)
+
+
+def irn_tv_cgls(proj, geo, angles, niter, lmbda=1.0, niter_outer=4, init=None, verbose=False, return_tensor=False,
+                computel2=False, **kwargs):
+    """IRN-TV-CGLS (krylov_subspace_algorithms.py:441-567): niter_outer cycles
+    of niter CGLS steps on [A; sqrt(lmbda) W D] x = [b; 0], Siddon forward and
+    matched backprojection, W = (|D x|^2 + 1e-6)^(-1/4) rebuilt from the image
+    at the start of each cycle.  Kept as the reference computes it: the
+    differences keep the last two entries of each axis, Ltx is not their exact
+    transpose there, and each cycle restarts from the image before the previous
+    cycle's last update.  FP64 norms; with ``computel2`` also returns
+    ||b - A x|| after every step."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    if init is None:
+        res = torch.zeros(vs.vol_shape, dtype=torch.float32, device="cuda")
+    elif isinstance(init, str) and init == "FDK":
+        res = fdk(b, g, g.angles, return_tensor=True).clamp_(min=0.0)
+    else:
+        res = _to_device(init).reshape(vs.vol_shape).clone()
+    sl = float(np.sqrt(lmbda))
+    res0 = res
+    l2 = []
+    for outer in range(niter_outer):
+        if verbose:
+            print("IRN-TV-CGLS outer iteration %d of %d" % (outer + 1, niter_outer))
+        W = _irn_weights(res)
+        res = res0
+        r1 = torch.sub(b, vs.ax(res, 0))
+        r2 = _irn_diffs(res).mul_(W).mul_(-sl)
+        p = vs.atb(r1, 0, _native.BACKPROJ_MATCHED).add_(_irn_ltx(W, r2), alpha=sl)
+        gamma = _norm(p) ** 2
+        for _ in range(niter):
+            res0 = res
+            q1 = vs.ax(p, 0)
+            q2 = _irn_diffs(p).mul_(W).mul_(sl)
+            alpha = gamma / (_norm(q1) ** 2 + _norm(q2) ** 2)
+            res = torch.add(res, p, alpha=alpha)
+            if computel2:
+                l2.append(_norm(torch.sub(b, vs.ax(res, 0))))
+            r1 = torch.sub(r1, q1, alpha=alpha)
+            r2 = torch.sub(r2, q2, alpha=alpha)
+            s = vs.atb(r1, 0, _native.BACKPROJ_MATCHED).add_(_irn_ltx(W, r2), alpha=sl)
+            gamma1 = _norm(s) ** 2
+            p = torch.add(s, p, alpha=gamma1 / gamma)
+            gamma = gamma1
     out = _result(res, return_tensor)
     return (out, np.array([l2])) if computel2 else out
 
