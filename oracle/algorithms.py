@@ -15,7 +15,7 @@ Followed files (Python/tigre/):
   algorithms/krylov_subspace_algorithms.py:101-188 LSQR, :190-297 hybrid LSQR, :299-439 LSMR,
                                         :570-695 AB-GMRES, BA-GMRES
   algorithms/statistical_algorithms.py:11-55     MLEM
-  algorithms/krylov_subspace_algorithms.py:441-567 IRN-TV-CGLS
+  algorithms/krylov_subspace_algorithms.py:441-567 IRN-TV-CGLS, :698-860 hybrid fLSQR-TV
 """
 from __future__ import annotations
 
@@ -512,7 +512,10 @@ def gmres(proj, geo, angles, niter, variant="ba", backprojector="matched", resta
         x = image(y)
         norms.append(float(np.linalg.norm(proj - A(x))))
         if k > 0 and norms[-1] > norms[-2] and (k == 1 or not restart):
-            return x, norms
+            # the entry point returns the image the loop never assigned (the
+            # zero start): krylov_subspace_algorithms.py:623-626, 688-691,
+            # iterative_recon_alg.py:470-475
+            return np.zeros_like(x), norms
     return image(y), norms
 
 
@@ -563,7 +566,9 @@ def hybrid_lsqr(proj, geo, angles, niter, lmbda=1.0, restart=True, elementwise_r
         x = res + sum(yj * vj for yj, vj in zip(y, V))
         norms.append(float(np.linalg.norm(proj - A(x))))
         if i > 0 and norms[-1] > norms[-2] and (i == 1 or not restart):
-            return x, norms
+            # returned from the loop, not assigned to the image that the entry
+            # point returns (the zero start): :285-289, iterative_recon_alg.py:470-475
+            return np.zeros_like(x), norms
     return x, norms
 
 
@@ -706,3 +711,76 @@ def irn_tv_cgls(proj, geo, angles, niter, lmbda=1.0, niter_outer=4, init=None):
     return irn_cycles(b, x0, lambda x: ax(np.asarray(x, np.float32), g, "Siddon"),
                       lambda y: atb(np.asarray(y, np.float32), g, "matched"),
                       irn_weights, irn_lx, irn_ltx, niter, niter_outer, lmbda)
+
+
+# -- hybrid fLSQR-TV -----------------------------------------------------------------
+def hybrid_flsqr_tv(proj, geo, angles, niter, lmbda=1.0, restart=True, init=None, trace=None):
+    """Hybrid flexible LSQR with TV (krylov_subspace_algorithms.py:698-860) in
+    FP64.  The constant image that best fits the data is split off (xA0);
+    each step backprojects u, orthogonalises it against the earlier v,
+    preconditions it by two scipy lsqr solves (50 iterations at most, scipy's
+    default tolerances, as the reference calls them) with the operators
+    irn_ltx / irn_lx, projects it, extends the U basis with A z, and solves
+    min ||M_k^T y - beta e_1||^2 + lmbda^2 ||R y||^2, R the triangular factor
+    of the stacked W D z_j.  Returns (x, norms, start): x the algorithm's
+    image; start the image the reference's entry point returns, which its
+    loop never assigns (iterative_recon_alg.py:470-475)."""
+    from scipy.linalg import qr
+    from scipy.sparse.linalg import LinearOperator, lsqr as sp_lsqr
+
+    g = prep(geo, angles)
+    b = np.asarray(proj, dtype=np.float64)
+    shape = tuple(int(v) for v in geo.nVoxel)
+    N = int(np.prod(shape))
+    res = np.zeros(shape) if init is None else np.asarray(init, np.float64).reshape(shape)
+    A = lambda x: ax(np.asarray(x, np.float32), g, "Siddon")
+    At = lambda y: atb(np.asarray(y, np.float32), g, "matched")
+    u = b - A(res)
+    k = A(np.ones(shape) / np.sqrt(N))
+    c = 1.0 / (np.sqrt(N) * float(np.sum(k * k)))
+    xA0 = np.full(shape, c * float(np.sum(k * u)))
+    kv = c * At(k)
+    u = u - A(xA0)
+    normr = float(np.linalg.norm(u))
+    if trace is not None:
+        trace.update(r0=u.copy(), xA0=xA0, Z=[])
+    u = u / normr
+    W = np.ones(shape) if res.max() == 0 else irn_weights(res)
+    lx = lambda x: np.concatenate([t.ravel() for t in irn_lx(W, np.reshape(x, shape))])
+    ltx = lambda y: irn_ltx(W, list(np.reshape(y, (3,) + shape))).ravel()
+    L1 = LinearOperator((N, 3 * N), matvec=ltx, rmatvec=lx, dtype=np.float64)
+    L2 = LinearOperator((3 * N, N), matvec=lx, rmatvec=ltx, dtype=np.float64)
+    U, V, Z = [u], [], []
+    M = np.zeros((niter, niter + 1))
+    rhs = np.zeros(niter + 1)
+    rhs[0] = normr
+    norms = []
+    x = res
+    for i in range(niter):
+        v = At(u)
+        for vj in V:
+            v = v - float(np.sum(vj * v)) * vj
+        v = v / np.linalg.norm(v)
+        z = v - kv * float(np.sum(v))
+        aux = sp_lsqr(L1, z.ravel(), iter_lim=50)[0]
+        z = sp_lsqr(L2, aux, iter_lim=50)[0].reshape(shape)
+        z = z - float(np.sum(kv * z))
+        V.append(v)
+        Z.append(z)
+        u = A(z)
+        for j, uj in enumerate(U):
+            M[i, j] = float(np.sum(uj * u))
+            u = u - M[i, j] * uj
+        M[i, i + 1] = np.linalg.norm(u)
+        u = u / M[i, i + 1]
+        U.append(u)
+        R = qr(np.stack([lx(zj) for zj in Z], 1), mode="economic")[1]
+        y = np.linalg.lstsq(np.vstack([M[:i + 1, :i + 2].T, lmbda * R]),
+                            np.concatenate([rhs[:i + 2], np.zeros(i + 1)]), rcond=None)[0]
+        x = res + sum(yj * zj for yj, zj in zip(y, Z)) + xA0
+        norms.append(float(np.linalg.norm(b - A(x))))
+        if i > 0 and norms[-1] > norms[-2] and (i == 1 or not restart):
+            break
+    if trace is not None:
+        trace.update(Z=Z, W=W)
+    return x, norms, res

@@ -269,3 +269,17 @@ def test_irn_tv_cgls(cuda, problem):
     print("IRN-TV-CGLS rel-L2 %.3e, residuals %s" % (err, l2))
     assert err <= 1e-4
     assert np.allclose(l2[0], norms, rtol=1e-4)
+
+
+def test_hybrid_flsqr_tv(cuda, problem):
+    """Hybrid fLSQR-TV (three steps): the reconstruction (return_x) and the
+    residual history against the FP64 restatement; the entry point's default
+    result is the starting image, as the reference's."""
+    geo, angles, img, proj = problem
+    got, l2 = algs.hybrid_flsqr_tv(proj, geo, angles, 3, computel2=True, return_x=True)
+    ref, norms, start = oalg.hybrid_flsqr_tv(proj, geo, angles, 3)
+    err = oracle.rel_l2(got, ref)
+    print("hybrid fLSQR-TV rel-L2 %.3e, residuals %s vs %s" % (err, l2, norms))
+    assert err <= 1e-4
+    assert np.allclose(l2[0], norms, rtol=1e-4)
+    assert np.all(algs.hybrid_flsqr_tv(proj, geo, angles, 1) == 0.0)

@@ -586,3 +586,60 @@ def test_irn_tv_cgls_weights_follow_the_image():
     assert all(b <= a * (1 + 1e-12) for a, b in zip(norms, norms[1:]))
     x2, _ = oalg.irn_tv_cgls(proj, geo, angles, 5, niter_outer=2)
     assert np.abs(x2 - x).max() > 1e-6 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("case", ["least_squares", "compatible", "non_adjoint"])
+def test_device_lsqr_matches_scipy_lsqr(case):
+    """The LSQR used for the hybrid fLSQR-TV preconditioner (torch, run here on
+    CPU tensors) equals scipy's lsqr with its default stopping tests, at
+    iteration limits 3 and 50, on an inconsistent system, a compatible one
+    (stops early on the residual test) and a non-adjoint pair of operators
+    like the reference's."""
+    import torch
+    from scipy.sparse.linalg import LinearOperator, lsqr as sp_lsqr
+
+    from tigre_amd.algorithms import _lsqr_solve
+
+    rng = np.random.default_rng(21)
+    K = rng.standard_normal((40, 25))
+    Kt = K.T + (0.05 * rng.standard_normal((25, 40)) if case == "non_adjoint" else 0.0)
+    b = K @ rng.standard_normal(25) if case == "compatible" else rng.standard_normal(40)
+    op = LinearOperator(K.shape, matvec=lambda x: K @ x, rmatvec=lambda y: Kt @ y, dtype=np.float64)
+    Kd, Ktd = torch.tensor(K), torch.tensor(Kt)
+    for lim in (3, 50):
+        ref = sp_lsqr(op, b, iter_lim=lim)
+        got = _lsqr_solve(lambda x: Kd @ x, lambda y: Ktd @ y, torch.tensor(b), lim).numpy()
+        # near the Krylov dimension rounding grows (3e-11 at 20 steps, 1e-8 at 24); a
+        # different stopping step would differ by the step size, 1e-5 here
+        tol = 1e-10 if lim == 3 else 1e-6
+        assert np.abs(got - ref[0]).max() <= tol * np.abs(ref[0]).max(), (lim, ref[1], ref[2])
+    if case == "compatible":
+        assert sp_lsqr(op, b, iter_lim=50)[2] < 50
+
+
+@pytest.mark.parametrize("lmbda", [0.0, 0.7])
+def test_hybrid_flsqr_tv_solves_the_projected_problem(lmbda):
+    """The image after k steps is start + xA0 + Z y with y = argmin
+    ||A Z y - r0||^2 + lmbda^2 ||W D Z y||^2 over the k preconditioned vectors
+    (solved here by dense least squares on A Z and W D Z), so the U / M
+    recurrence and the QR factor represent the problem exactly; xA0 is the
+    constant image whose projection best fits the data (r0 is orthogonal to
+    the projection of a constant image); the entry point's image is the start."""
+    geo = cone_geometry((6, 6, 6), (6.0, 6.0, 6.0), (8, 8), (1.5, 1.5), 60.0, 90.0)
+    angles = np.linspace(0, 2 * np.pi, 12, endpoint=False)
+    # not a constant image: its data would be fitted by xA0 alone (r0 = 0)
+    img = np.random.default_rng(5).random((6, 6, 6)).astype(np.float32)
+    g = prepared(geo, angles)
+    proj = oracle.ax(img, g)
+    tr = {}
+    x, norms, start = oalg.hybrid_flsqr_tv(proj, geo, angles, 3, lmbda=lmbda, trace=tr)
+    assert np.all(start == 0.0) and len(norms) == 3
+    ones = oracle.ax(np.ones(img.shape, np.float32), g)
+    # xA0 reaches the projector rounded to float32: orthogonal to 1e-7
+    assert abs(np.sum(ones * tr["r0"])) <= 1e-6 * np.linalg.norm(ones) * np.linalg.norm(tr["r0"])
+    AZ = np.stack([oracle.ax(z.astype(np.float32), g).ravel() for z in tr["Z"]], 1)
+    LZ = np.stack([np.concatenate([t.ravel() for t in oalg.irn_lx(tr["W"], z)]) for z in tr["Z"]], 1)
+    y = np.linalg.lstsq(np.vstack([AZ, lmbda * LZ]), np.concatenate([tr["r0"].ravel(), np.zeros(LZ.shape[0])]),
+                        rcond=None)[0]
+    want = tr["xA0"] + sum(yj * zj for yj, zj in zip(y, tr["Z"]))
+    assert np.abs(x - want).max() <= 1e-6 * np.abs(want).max()

@@ -2,7 +2,7 @@
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
 ``fdk``/``FDK``, ``fbp``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
-``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``, ``irn_tv_cgls``,
+``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``, ``irn_tv_cgls``, ``hybrid_flsqr_tv``,
 ``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
 ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``, ``pcsd``,
 ``aw_pcsd``, ``os_pcsd``, ``os_aw_pcsd``.
@@ -29,7 +29,8 @@ from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
 __all__ = ["fdk", "FDK", "fbp", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
-           "hybrid_lsqr", "lsmr", "irn_tv_cgls", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
+           "hybrid_lsqr", "lsmr", "irn_tv_cgls",
+           "hybrid_flsqr_tv", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
            "os_awasd_pocs", "piccs", "os_piccs", "pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd", "ViewSets",
            "BatchedOSSART"]
 
@@ -577,6 +578,10 @@ def hybrid_lsqr(proj, geo, angles, niter, lmbda=1.0, restart=True, verbose=False
         l2.append(_norm(b - A(x)))
         if i > 0 and l2[-1] > l2[-2] and (i == 1 or not restart):
             print("hybrid LSQR exited due to divergence at iteration %d" % i)
+            # returned from the loop, never assigned to the image that the
+            # reference's entry point returns: the zero start
+            # (krylov_subspace_algorithms.py:285-289; iterative_recon_alg.py:470-475)
+            x = torch.zeros_like(x)
             break
     out = _result(x, return_tensor)
     return (out, np.array([l2])) if computel2 else out
@@ -728,6 +733,182 @@ def irn_tv_cgls(proj, geo, angles, niter, lmbda=1.0, niter_outer=4, init=None, v
     return (out, np.array([l2])) if computel2 else out
 
 
+def _sym_ortho(a, b):
+    """Givens rotation (c, s, r) with c a + s b = r >= 0, in the stable form
+    of S.-C. Choi's SymOrtho (scipy's LSQR uses the same)."""
+    if b == 0.0:
+        return float(np.sign(a)), 0.0, abs(a)
+    if a == 0.0:
+        return 0.0, float(np.sign(b)), abs(b)
+    if abs(b) > abs(a):
+        tau = a / b
+        s = float(np.sign(b)) / np.sqrt(1.0 + tau * tau)
+        return s * tau, s, b / s
+    tau = b / a
+    c = float(np.sign(a)) / np.sqrt(1.0 + tau * tau)
+    return c, c * tau, a / c
+
+
+def _lsqr_solve(matvec, rmatvec, b, iter_lim, atol=1e-6, btol=1e-6, conlim=1e8):
+    """min ||K x - b|| by LSQR (Paige & Saunders, ACM TOMS 8 (1982) 43-71)
+    from x = 0 with no damping, with the stopping tests of
+    scipy.sparse.linalg.lsqr (istop 1-7: compatible system, least-squares
+    optimality, condition limit, their machine-precision forms, iteration
+    limit), which the reference calls for its preconditioner
+    (krylov_subspace_algorithms.py:808-811).  ``matvec`` / ``rmatvec`` play K
+    and its transpose; the pair need not be adjoint.  Vectors stay on their
+    device; scalars and tests in FP64 on the host."""
+    import torch
+
+    def nrm(t):
+        return float(torch.linalg.vector_norm(t))
+
+    eps = float(np.finfo(np.float64).eps)
+    ctol = 1.0 / conlim if conlim > 0 else 0.0
+    anorm = acond = ddnorm = xnorm = xxnorm = z = 0.0
+    cs2, sn2 = -1.0, 0.0
+    bnorm = beta = nrm(b)
+    if beta > 0:
+        u = b / beta
+        v = rmatvec(u)
+        alfa = nrm(v)
+    else:
+        return torch.zeros_like(rmatvec(b))
+    x = torch.zeros_like(v)
+    if alfa > 0:
+        v = v / alfa
+    w = v.clone()
+    rhobar, phibar = alfa, beta
+    if alfa * beta == 0:
+        return x
+    itn = 0
+    while itn < iter_lim:
+        itn += 1
+        u = torch.sub(matvec(v), u, alpha=alfa)
+        beta = nrm(u)
+        if beta > 0:
+            u = u / beta
+            anorm = np.sqrt(anorm ** 2 + alfa ** 2 + beta ** 2)
+            v = torch.sub(rmatvec(u), v, alpha=beta)
+            alfa = nrm(v)
+            if alfa > 0:
+                v = v / alfa
+        cs, sn, rho = _sym_ortho(rhobar, beta)
+        theta = sn * alfa
+        rhobar = -cs * alfa
+        phi = cs * phibar
+        phibar = sn * phibar
+        tau = sn * phi
+        ddnorm += (nrm(w) / rho) ** 2
+        x = torch.add(x, w, alpha=phi / rho)
+        w = torch.add(v, w, alpha=-theta / rho)
+        delta = sn2 * rho
+        gambar = -cs2 * rho
+        rhs = phi - delta * z
+        zbar = rhs / gambar
+        xnorm = np.sqrt(xxnorm + zbar ** 2)
+        gamma = np.sqrt(gambar ** 2 + theta ** 2)
+        cs2, sn2 = gambar / gamma, theta / gamma
+        z = rhs / gamma
+        xxnorm += z ** 2
+        acond = anorm * np.sqrt(ddnorm)
+        rnorm = abs(phibar)
+        arnorm = alfa * abs(tau)
+        test1 = rnorm / bnorm
+        test2 = arnorm / (anorm * rnorm + eps)
+        test3 = 1.0 / (acond + eps)
+        t1 = test1 / (1.0 + anorm * xnorm / bnorm)
+        rtol = btol + atol * anorm * xnorm / bnorm
+        if (itn >= iter_lim or 1.0 + test3 <= 1.0 or 1.0 + test2 <= 1.0 or 1.0 + t1 <= 1.0 or test3 <= ctol
+                or test2 <= atol or test1 <= rtol):
+            break
+    return x
+
+
+def hybrid_flsqr_tv(proj, geo, angles, niter, lmbda=1.0, init=None, restart=True, verbose=False, return_tensor=False,
+                    computel2=False, return_x=False, **kwargs):
+    """Hybrid flexible LSQR with TV (krylov_subspace_algorithms.py:698-860):
+    the constant image xA0 that best fits the data is split off; each step
+    backprojects u (matched), orthogonalises it against the earlier v,
+    preconditions it by two LSQR solves of at most 50 iterations with the
+    IRN difference operators (Ltx then Lx, as the reference's scipy calls),
+    removes <k, z> (k the backprojected constant image, scaled), extends the
+    U basis with A z (Siddon) and solves the small problem
+    min ||M^T y - beta e1||^2 + lmbda^2 ||WDZ y||^2; the image is
+    start + Z y + xA0, and the data residual after each step is kept.
+
+    The reference's entry point returns ``self.res``, which this loop never
+    assigns (iterative_recon_alg.py:470-475), so it returns its starting
+    image; that is returned here too unless ``return_x`` (no counterpart in
+    the reference) asks for the reconstruction.  Preconditioner solves in
+    FP64 on the device; the small problem in FP64 on the host, its factor R
+    from the Cholesky factor of the Gram matrix (R^T R is all the problem
+    uses, the same as the reference's QR up to row signs)."""
+    torch = _torch()
+    g = copy.deepcopy(geo)
+    g.check_geo(angles)
+    vs = ViewSets(g, g.angles, [np.arange(len(g.angles))])
+    b = _to_device(proj)
+    shape = vs.vol_shape
+    N = int(np.prod(shape))
+    res = (torch.zeros(shape, dtype=torch.float32, device="cuda") if init is None
+           else _to_device(init).reshape(shape).clone())
+    A = lambda x: vs.ax(x, 0, _native.PROJ_SIDDON)  # noqa: E731
+    At = lambda y: vs.atb(y, 0, _native.BACKPROJ_MATCHED)  # noqa: E731
+    flat = lambda t: t.reshape(-1).double()  # noqa: E731
+    u = torch.sub(b, A(res))
+    k = A(torch.full(shape, 1.0 / np.sqrt(N), dtype=torch.float32, device="cuda"))
+    c = 1.0 / (np.sqrt(N) * _norm(k) ** 2)
+    xA0 = c * float(torch.dot(flat(k), flat(u)))
+    kv = flat(At(k)) * c
+    u = torch.sub(u, A(torch.full(shape, xA0, dtype=torch.float32, device="cuda")))
+    normr = _norm(u)
+    u = u / normr
+    W = (torch.ones(shape, dtype=torch.float64, device="cuda") if float(res.max()) == 0.0
+         else _irn_weights(res).double())
+    lx = lambda x: _irn_diffs(x.reshape(shape)).mul_(W).reshape(-1)  # noqa: E731
+    ltx = lambda y: _irn_ltx(W, y.reshape((3,) + shape)).reshape(-1)  # noqa: E731
+    U, V, Z, WZ = [u], [], [], []
+    M = np.zeros((niter, niter + 1))
+    rhs = np.zeros(niter + 1)
+    rhs[0] = normr
+    l2 = []
+    x = res
+    for i in range(niter):
+        if verbose:
+            print("hybrid fLSQR-TV iteration %d of %d" % (i + 1, niter))
+        v = At(u)
+        for vj in V:
+            v = torch.sub(v, vj, alpha=float(torch.dot(flat(vj), flat(v))))
+        v = v / _norm(v)
+        zd = flat(v) - kv * float(flat(v).sum())
+        zd = _lsqr_solve(lx, ltx, _lsqr_solve(ltx, lx, zd, 50), 50)
+        zd = zd - float(torch.dot(kv, zd))
+        V.append(v)
+        Z.append(zd.float().reshape(shape))
+        WZ.append(lx(flat(Z[-1])))
+        u = A(Z[-1])
+        for j, uj in enumerate(U):
+            M[i, j] = float(torch.dot(flat(uj), flat(u)))
+            u = torch.sub(u, uj, alpha=M[i, j])
+        M[i, i + 1] = _norm(u)
+        u = u / M[i, i + 1]
+        U.append(u)
+        G = torch.stack(WZ).matmul(torch.stack(WZ).T).cpu().numpy()
+        R = np.linalg.cholesky(G).T
+        y = np.linalg.lstsq(np.vstack([M[:i + 1, :i + 2].T, lmbda * R]),
+                            np.concatenate([rhs[:i + 2], np.zeros(i + 1)]), rcond=None)[0]
+        x = torch.full(shape, xA0, dtype=torch.float32, device="cuda").add_(res)
+        for yj, zj in zip(y, Z):
+            x = torch.add(x, zj, alpha=float(yj))
+        l2.append(_norm(torch.sub(b, A(x))))
+        if i > 0 and l2[-1] > l2[-2] and (i == 1 or not restart):
+            print("hybrid fLSQR-TV exited due to divergence at iteration %d" % i)
+            break
+    out = _result(x if return_x else res, return_tensor)
+    return (out, np.array([l2])) if computel2 else out
+
+
 def _gmres(proj, geo, angles, niter, left, backprojector="matched", restart=True, verbose=False,
            return_tensor=False, computel2=False, **kwargs):
     """AB-GMRES (left False) / BA-GMRES (left True), krylov_subspace_algorithms.py:570-695."""
@@ -778,6 +959,10 @@ def _gmres(proj, geo, angles, niter, left, backprojector="matched", restart=True
         l2.append(_norm(b - A(x)))
         if k > 0 and l2[-1] > l2[-2] and (k == 1 or not restart):
             print("%s-GMRES exited due to divergence at iteration %d" % ("BA" if left else "AB", k))
+            # the reference's loop returns here without assigning its image, and
+            # its entry point returns that image: the zero start
+            # (krylov_subspace_algorithms.py:623-626, 688-691; iterative_recon_alg.py:470-475)
+            x = torch.zeros_like(x)
             break
     out = _result(x, return_tensor)
     return (out, np.array([l2])) if computel2 else out
