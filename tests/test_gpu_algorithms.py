@@ -283,3 +283,9 @@ def test_hybrid_flsqr_tv(cuda, problem):
     assert err <= 1e-4
     assert np.allclose(l2[0], norms, rtol=1e-4)
     assert np.all(algs.hybrid_flsqr_tv(proj, geo, angles, 1) == 0.0)
+
+
+def test_iterativereconalg_is_ossart(cuda, problem):
+    """The base class's entry point runs the OS-SART loop with its defaults."""
+    geo, angles, img, proj = problem
+    assert np.array_equal(algs.iterativereconalg(proj, geo, angles, 2), algs.ossart(proj, geo, angles, 2))

@@ -1,7 +1,7 @@
 """Reconstruction algorithms on device-resident data.
 
 Same entry points and defaults as Python/tigre/algorithms (__init__.py:5-60):
-``fdk``/``FDK``, ``fbp``, ``ossart``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
+``fdk``/``FDK``, ``fbp``, ``ossart``, ``iterativereconalg``, ``sart``, ``sirt``, ``sart_tv``, ``ossart_tv``,
 ``fista``, ``ista``, ``cgls``, ``lsqr``, ``hybrid_lsqr``, ``lsmr``, ``irn_tv_cgls``, ``hybrid_flsqr_tv``,
 ``ab_gmres``, ``ba_gmres``, ``mlem``, ``asd_pocs``, ``os_asd_pocs``,
 ``awasd_pocs``, ``os_awasd_pocs``, ``piccs``, ``os_piccs``, ``pcsd``,
@@ -28,7 +28,7 @@ from ..operators import _stream_handle
 from ..utilities.filtering import filtering
 from ..utilities.order_subsets import order_subsets
 
-__all__ = ["fdk", "FDK", "fbp", "ossart", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
+__all__ = ["fdk", "FDK", "fbp", "ossart", "iterativereconalg", "sart", "sirt", "sart_tv", "ossart_tv", "fista", "ista", "cgls", "lsqr",
            "hybrid_lsqr", "lsmr", "irn_tv_cgls",
            "hybrid_flsqr_tv", "ab_gmres", "ba_gmres", "mlem", "asd_pocs", "os_asd_pocs", "awasd_pocs",
            "os_awasd_pocs", "piccs", "os_piccs", "pcsd", "aw_pcsd", "os_pcsd", "os_aw_pcsd", "ViewSets",
@@ -366,6 +366,13 @@ def _sart_family(proj, geo, angles, niter, blocksize, **kwargs):
 def ossart(proj, geo, angles, niter, blocksize=20, **kwargs):
     """OS-SART (art_family_algorithms.py:47-62), default 20 views per subset."""
     return _sart_family(proj, geo, angles, niter, blocksize, **kwargs)
+
+
+def iterativereconalg(proj, geo, angles, niter, **kwargs):
+    """The base class's own entry point (iterative_recon_alg.py:487): its
+    defaults (blocksize 20, lmbda 1, no reduction, noneg, the ART data step,
+    iterative_recon_alg.py:140-156) make its loop the OS-SART one."""
+    return ossart(proj, geo, angles, niter, **kwargs)
 
 
 def sart(proj, geo, angles, niter, **kwargs):
