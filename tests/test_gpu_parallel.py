@@ -120,3 +120,20 @@ def test_parallel_ax_sparse_views_stacked_tiles(cuda, kind):
     assert err <= TOL
     one = np.concatenate([tigre_amd.Ax(d_img, geo, angles[i:i + 1], kind).cpu().numpy() for i in range(24)])
     assert np.array_equal(one, got)
+
+
+def test_parallel_ossart_cgls(cuda):
+    """OS-SART and CGLS on a parallel-beam geometry go through the parallel
+    operators (V = block size) and match the FP64 restatements."""
+    import oracle.algorithms as oalg
+    import tigre_amd.algorithms as algs
+
+    geo = geometry(mode="parallel", nVoxel=np.array([24, 32, 32]))
+    angles = np.linspace(0, np.pi, 40, endpoint=False)
+    img = np.random.default_rng(9).random((24, 32, 32)).astype(np.float32)
+    proj = oracle.ax(img, prepared(geo, angles)).astype(np.float32)
+    err_os = oracle.rel_l2(algs.ossart(proj, geo, angles, 2, blocksize=10, verbose=False),
+                           oalg.os_sart(proj, geo, angles, 2, blocksize=10))
+    err_cg = oracle.rel_l2(algs.cgls(proj, geo, angles, 4), oalg.cgls(proj, geo, angles, 4)[0])
+    print("parallel OS-SART rel-L2 %.3e, CGLS %.3e" % (err_os, err_cg))
+    assert err_os <= 1e-4 and err_cg <= 1e-4
